@@ -1,0 +1,15 @@
+"""MI355X-native pseudospectral time stepping for GeophysicalFlows.jl's hot path.
+
+The per-step nonlinear term ``calcN!`` of TwoDNavierStokes, SingleLayerQG and
+MultiLayerQG and the ETDRK4 / RK4 / ForwardEuler (and Filtered) stepper stages
+behind ``stepforward!`` run as fused HIP kernels for gfx950 in ``libgfhip.so``
+(C ABI: ``include/gf.h``).  This Python package mirrors the reference's
+user-facing API on top of that ABI; it has no CPU compute path.
+"""
+from . import _lib  # noqa: F401  (loads libgfhip.so or raises ImportError)
+from . import multilayerqg as MultiLayerQG  # noqa: F401
+from . import singlelayerqg as SingleLayerQG  # noqa: F401
+from . import twodnavierstokes as TwoDNavierStokes  # noqa: F401
+from .problem import CPU, GPU, RingForcing, stepforward  # noqa: F401
+
+__all__ = ["TwoDNavierStokes", "SingleLayerQG", "MultiLayerQG", "GPU", "CPU", "RingForcing", "stepforward"]

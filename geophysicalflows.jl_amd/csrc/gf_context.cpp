@@ -1,0 +1,830 @@
+// Host side of libgfhip: problem context, setup tables and the C ABI (include/gf.h).
+//
+// The context mirrors what FourierFlows.Problem builds for the reference
+// (grid, Equation.L, time-stepper coefficients and buffers, clock) and drives
+// the fused kernels of gf_kernels_impl.hpp on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "gf_kernels.hpp"
+#include "../../include/gf.h"
+
+using namespace gf;
+using cd = std::complex<double>;
+
+namespace {
+
+struct GfError : std::runtime_error {
+  int code;
+  GfError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                  \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      throw GfError(GF_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+// FourierFlows' aliased index rule ([FF] getaliasedwavenumbers), 0-based [lo, hi).
+void aliased_range(int nk, int nkr, double f, int& lo_full, int& hi_full, int& lo_r) {
+  if (f <= 0) { lo_full = hi_full = nk; lo_r = nkr; return; }
+  const double L = (1 - f) / 2, R = (1 + f) / 2;
+  const int iL = (int)std::floor(L * nk) + 1, iR = (int)std::ceil(R * nk);
+  lo_full = iL - 1; hi_full = iR; lo_r = iL - 1;
+}
+
+inline long boff_h(int kr, int j, int ny) { return ((long)(kr >> 1) * ny + j) * 2 + (kr & 1); }
+
+template <typename F>
+void parallel_for(long n, F f) {
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4096) nt = 1;
+  std::vector<std::thread> th;
+  const long chunk = (n + nt - 1) / nt;
+  for (unsigned i = 0; i < nt; ++i) {
+    const long a = i * chunk, b = std::min(n, a + chunk);
+    if (a >= b) break;
+    th.emplace_back([=] { for (long j = a; j < b; ++j) f(j); });
+  }
+  for (auto& t : th) t.join();
+}
+
+// Kassam--Trefethen contour means ([FF] getetdcoeffs, ncirc = 32, rcirc = 1).
+void etd_coeffs(cd dtL, double dt, cd& zeta, cd& alpha, cd& beta, cd& gamma) {
+  const int M = 32;
+  cd sz = 0, sa = 0, sb = 0, sg = 0;
+  for (int m = 0; m < M; ++m) {
+    const double th = 2 * M_PI / M * (m + 0.5);
+    const cd z = dtL + cd(std::cos(th), std::sin(th));
+    const cd ez = std::exp(z), z3 = z * z * z;
+    sz += (std::exp(z / 2.0) - 1.0) / z;
+    sa += (-4.0 - z + ez * (4.0 - 3.0 * z + z * z)) / z3;
+    sb += (2.0 + z + ez * (-2.0 + z)) / z3;
+    sg += (-4.0 - 3.0 * z - z * z + ez * (4.0 - z)) / z3;
+  }
+  zeta = dt * (sz / (double)M);
+  alpha = dt * (sa / (double)M);
+  beta = dt * (sb / (double)M);
+  gamma = dt * (sg / (double)M);
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t bytes) {
+    if (p) (void)hipFree(p);
+    p = nullptr; n = bytes;
+    if (bytes) {
+      hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) throw GfError(GF_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+      e = hipMemset(p, 0, bytes);
+      if (e != hipSuccess) throw GfError(GF_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    }
+  }
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  template <typename X> X* as() const { return (X*)p; }
+};
+
+struct CtxBase {
+  virtual ~CtxBase() = default;
+  virtual void set_state(const void* src, hipMemcpyKind kind) = 0;
+  virtual void get_state(void* dst, hipMemcpyKind kind) = 0;
+  virtual void set_physical(const void* f) = 0;
+  virtual void updatevars(void* out) = 0;
+  virtual void calcN(void* N, void* sol, double t) = 0;
+  virtual void set_forcing(const void* Fh) = 0;
+  virtual void step(int n) = 0;
+  virtual int nvars() const = 0;
+  virtual void timing(int enable) = 0;
+  virtual void timing_read(int k, int64_t* n, double* ms) = 0;
+  virtual double kernel_bytes(int k, int full) const = 0;
+  virtual void sync() = 0;
+  double t = 0;
+  int64_t stepno = 0;
+};
+
+template <typename T>
+struct Ctx final : CtxBase {
+  using C = cplx<T>;
+  gf_problem_desc d;
+  Geo<T> g;
+  int nl = 1;
+  hipStream_t stream = nullptr;
+  long fsz = 0;        // complex elements of one blocked spectral/mixed field (one layer)
+  bool eb = false;     // SLQG equivalent barotropic
+  double ell2inv = 0;
+  bool advect_full = false;   // K2 with background gradients (MLQG, SLQG varying U)
+  bool ccplx = false;         // complex L (SLQG)
+  bool has_eta = false;
+  int fkx = -1, fky = -1;
+  // device buffers
+  DevBuf sol, s1, s2, n1, n23, racc, nout, mixed, flux, twx, twy, phys, jl, cinp;
+  DevBuf coef[6], filter, eta, etah, etax, etay, Uy, Qy1, sinv, Fh, sqrt_spec;
+  // timing
+  bool timing_on = false;
+  struct Ev { int k; hipEvent_t a, b; };
+  std::vector<Ev> evs;
+  int64_t tl[3] = {0, 0, 0};
+  double tms[3] = {0, 0, 0};
+
+  explicit Ctx(const gf_problem_desc& desc) : d(desc) {
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    nl = d.model == GF_MODEL_MLQG ? d.nlayers : 1;
+    const int nx = d.nx, ny = d.ny, nkr = nx / 2 + 1;
+    g.nx = nx; g.ny = ny; g.nkr = nkr; g.nkr2 = (nkr + 1) & ~1; g.nlayers = nl;
+    int lo_full, hi_full, lo_r, dum1, dum2;
+    aliased_range(nx, nkr, d.aliased_fraction, dum1, dum2, lo_r);
+    aliased_range(ny, ny / 2 + 1, d.aliased_fraction, lo_full, hi_full, dum1);
+    g.kmax = std::min(lo_r, nkr);
+    g.la0 = lo_full; g.la1 = hi_full;
+    if (d.aliased_fraction <= 0) { g.kmax = nkr; g.la0 = g.la1 = ny; }
+    // rfftfreq(nx, 2 pi/Lx * nx) multiplier, as [FF] TwoDGrid builds it
+    g.kx0 = (T)((2 * M_PI / d.Lx * nx) / nx);
+    g.ky0 = (T)((2 * M_PI / d.Ly * ny) / ny);
+    g.inv_norm = (T)(1.0 / ((double)nx * ny));
+    fsz = (long)g.nkr2 * ny;
+    g.lstride = fsz;
+    const size_t cb = sizeof(C) * fsz * nl;
+    sol.alloc(cb); s1.alloc(cb); s2.alloc(cb); n1.alloc(cb); n23.alloc(cb); racc.alloc(cb); nout.alloc(cb);
+    cinp.alloc(cb);
+    mixed.alloc(sizeof(C) * fsz * 4 * nl);
+    flux.alloc(sizeof(C) * fsz * 2 * nl);
+    phys.alloc(sizeof(T) * (size_t)nx * ny * 4 * nl);
+    jl.alloc(sizeof(C) * (size_t)nkr * ny * nl);
+    twx.alloc(sizeof(C) * nx); twy.alloc(sizeof(C) * ny);
+    upload_twiddles(twx, nx); upload_twiddles(twy, ny);
+    build_model();
+    build_stepper();
+    build_forcing();
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  ~Ctx() override {
+    for (auto& e : evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    if (stream) { (void)hipStreamSynchronize(stream); (void)hipStreamDestroy(stream); }
+  }
+
+  // ---------------- setup helpers ----------------
+  void upload_twiddles(DevBuf& b, int n) {
+    std::vector<C> h(n);
+    for (int i = 0; i < n; ++i) {
+      const long double a = -2.0L * 3.141592653589793238462643383279502884L * i / n;
+      h[i].re = (T)std::cos(a); h[i].im = (T)std::sin(a);
+    }
+    HIPCHECK(hipMemcpy(b.p, h.data(), sizeof(C) * n, hipMemcpyHostToDevice));
+  }
+  double krw(int kr) const { return (double)kr * ((2 * M_PI / d.Lx * d.nx) / d.nx); }
+  double lw(int l) const {
+    const int ny = d.ny;
+    return (double)(l < (ny + 1) / 2 ? l : l - ny) * ((2 * M_PI / d.Ly * ny) / ny);
+  }
+  // Krsq in the problem precision, as the reference's grid holds it
+  double krsq(int kr, int l) const {
+    const T a = (T)krw(kr), b = (T)lw(l);
+    return (double)(T)(a * a + b * b);
+  }
+  template <typename X>
+  void upload_blocked(DevBuf& b, const std::vector<X>& full /* [l][kr] */, int nlay = 1) {
+    std::vector<X> h((size_t)fsz * nlay, X{});
+    for (int j = 0; j < nlay; ++j)
+      for (int l = 0; l < d.ny; ++l)
+        for (int kr = 0; kr < g.nkr; ++kr)
+          h[(size_t)j * fsz + boff_h(kr, l, d.ny)] = full[((size_t)j * d.ny + l) * g.nkr + kr];
+    b.alloc(sizeof(X) * h.size());
+    HIPCHECK(hipMemcpy(b.p, h.data(), sizeof(X) * h.size(), hipMemcpyHostToDevice));
+  }
+  // device rfft2 of nf physical fields (x fastest) -> blocked spectral fields
+  void fwd_device(const T* phys_dev, C* spec_dev, int nf, int zero_mean) {
+    K2Args<T> a{};
+    a.g = g; a.g.nlayers = 1; a.mode = K2_FORWARD_IN; a.nfields = nf; a.phys_in = phys_dev;
+    a.pstride = (long)d.nx * d.ny; a.out = mixed.as<C>(); a.ostride = fsz; a.kout = g.nkr;
+    a.tw = twx.as<C>();
+    launch_k2<T>(d.nx, a, stream);
+    K3Args<T> b{};
+    b.g = g; b.g.nlayers = nf; b.model = GF_MODEL_TWODNS; b.in = mixed.as<C>(); b.fstride = fsz;
+    b.ncols = g.nkr; b.full = 1; b.epi = E_STORE; b.sol = spec_dev; b.zero_mean = zero_mean;
+    b.tw = twy.as<C>();
+    launch_k3<T>(d.ny, b, stream);
+    HIPCHECK(hipGetLastError());
+  }
+  // device irfft2 (normalised) of nf blocked spectral fields -> physical fields
+  void inv_device(const C* spec_dev, T* phys_dev, int nf) {
+    for (int f = 0; f < nf; ++f) {
+      K1Args<T> a{};
+      a.g = g; a.g.nlayers = 1; a.model = GF_MODEL_TWODNS; a.fieldset = FS_RAW;
+      a.sol = spec_dev + (size_t)f * fsz; a.out = mixed.as<C>() + (size_t)f * fsz; a.fstride = fsz;
+      a.ncols = g.nkr; a.tw = twy.as<C>();
+      launch_k1<T>(d.ny, a, stream);
+    }
+    K2Args<T> b{};
+    b.g = g; b.g.nlayers = 1; b.mode = K2_INVERSE_OUT; b.nfields = nf; b.in = mixed.as<C>();
+    b.fstride = fsz; b.kin = g.nkr; b.phys = phys_dev; b.pstride = (long)d.nx * d.ny; b.tw = twx.as<C>();
+    launch_k2<T>(d.nx, b, stream);
+    HIPCHECK(hipGetLastError());
+  }
+  // spectral derivative / multiplier of physical fields, computed with the device transforms
+  std::vector<double> spectral_op(const std::vector<double>& f /* ny*nx */, int op) {
+    const size_t np = (size_t)d.nx * d.ny;
+    std::vector<T> ft(np);
+    for (size_t i = 0; i < np; ++i) ft[i] = (T)f[i];
+    HIPCHECK(hipMemcpy(phys.p, ft.data(), sizeof(T) * np, hipMemcpyHostToDevice));
+    fwd_device(phys.as<T>(), nout.as<C>(), 1, 0);
+    std::vector<C> h(fsz);
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipMemcpy(h.data(), nout.p, sizeof(C) * fsz, hipMemcpyDeviceToHost));
+    for (int l = 0; l < d.ny; ++l)
+      for (int kr = 0; kr < g.nkr; ++kr) {
+        C& z = h[boff_h(kr, l, d.ny)];
+        const T k = (T)krw(kr), m = (T)lw(l);
+        C r;
+        if (op == 0) { r.re = -k * z.im; r.im = k * z.re; }          // i kr
+        else if (op == 1) { r.re = -m * z.im; r.im = m * z.re; }     // i l
+        else { const T m2 = -(m * m); r.re = m2 * z.re; r.im = m2 * z.im; }  // -l^2
+        z = r;
+      }
+    HIPCHECK(hipMemcpy(nout.p, h.data(), sizeof(C) * fsz, hipMemcpyHostToDevice));
+    inv_device(nout.as<C>(), phys.as<T>(), 1);
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipMemcpy(ft.data(), phys.p, sizeof(T) * np, hipMemcpyDeviceToHost));
+    std::vector<double> out(np);
+    for (size_t i = 0; i < np; ++i) out[i] = (double)ft[i];
+    return out;
+  }
+  void upload_real(DevBuf& b, const std::vector<double>& v) {
+    std::vector<T> h(v.size());
+    for (size_t i = 0; i < v.size(); ++i) h[i] = (T)v[i];
+    b.alloc(sizeof(T) * h.size());
+    HIPCHECK(hipMemcpy(b.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  }
+
+  std::vector<cd> Lop;   // [l][kr], one layer (MLQG L is the same in every layer)
+
+  void build_model() {
+    const int nx = d.nx, ny = d.ny, nkr = g.nkr;
+    const size_t np = (size_t)nx * ny;
+    Lop.assign((size_t)ny * nkr, cd(0));
+    // Krsq^n with an integer exponent (pow, correctly rounded to within an ulp)
+    auto ipow = [](double x, int n) { return n == 0 ? 1.0 : std::pow(x, (double)n); };
+    if (d.model == GF_MODEL_TWODNS) {
+      // L = -nu Krsq^nnu - mu Krsq^nmu, L[1,1] = 0 (src/twodnavierstokes.jl:150-151)
+      for (int l = 0; l < ny; ++l)
+        for (int kr = 0; kr < nkr; ++kr) {
+          const double K2 = krsq(kr, l);
+          const double pm = d.nmu >= 0 ? ipow(K2, d.nmu) : (K2 == 0 ? INFINITY : 1.0 / ipow(K2, -d.nmu));
+          Lop[(size_t)l * nkr + kr] = cd(-d.nu * ipow(K2, d.nnu) - d.mu * pm, 0);
+        }
+      Lop[0] = 0;
+      return;
+    }
+    if (d.eta) {
+      has_eta = false;
+      for (size_t i = 0; i < np; ++i) if (d.eta[i] != 0) { has_eta = true; break; }
+    }
+    std::vector<double> etav(np, 0.0);
+    if (d.eta) for (size_t i = 0; i < np; ++i) etav[i] = d.eta[i];
+    if (d.model == GF_MODEL_SLQG) {
+      eb = d.deformation_radius > 0 && std::isfinite(d.deformation_radius);
+      ell2inv = eb ? 1.0 / (d.deformation_radius * d.deformation_radius) : 0.0;
+      const bool varyingU = d.Uy != nullptr;
+      ccplx = true;
+      // L = -mu - nu Krsq^nnu + i beta kr/(Krsq [+ 1/ell^2]) [- i U kr] (src/singlelayerqg.jl:230-252)
+      for (int l = 0; l < ny; ++l)
+        for (int kr = 0; kr < nkr; ++kr) {
+          const double K2 = krsq(kr, l), k = (double)(T)krw(kr);
+          const double iK = (kr == 0 && l == 0) ? 0.0 : 1.0 / K2;
+          const double bterm = eb ? d.beta * k / (K2 + ell2inv) : d.beta * k * iK;
+          cd L(-d.mu - d.nu * ipow(K2, d.nnu), bterm);
+          if (!varyingU) L -= cd(0, d.U * k);
+          Lop[(size_t)l * nkr + kr] = L;
+        }
+      Lop[0] = 0;
+      if (!varyingU) {
+        if (has_eta) {
+          upload_real(eta, etav);
+          if (d.U != 0) {
+            // etah = rfft(eta) for the -i kr U etah term
+            std::vector<T> ft(np);
+            for (size_t i = 0; i < np; ++i) ft[i] = (T)etav[i];
+            HIPCHECK(hipMemcpy(phys.p, ft.data(), sizeof(T) * np, hipMemcpyHostToDevice));
+            etah.alloc(sizeof(C) * fsz);
+            fwd_device(phys.as<T>(), etah.as<C>(), 1, 0);
+            HIPCHECK(hipStreamSynchronize(stream));
+          }
+        }
+      } else {
+        // varying U(y): Qx = d eta/dx, Qy = d eta/dy - U_yy (src/singlelayerqg.jl:182-189)
+        advect_full = true;
+        std::vector<double> U2(np);
+        for (int y = 0; y < ny; ++y) for (int x = 0; x < nx; ++x) U2[(size_t)y * nx + x] = d.Uy[y];
+        std::vector<double> Uyy = spectral_op(U2, 2);
+        std::vector<double> ex = spectral_op(etav, 0), ey = spectral_op(etav, 1);
+        for (size_t i = 0; i < np; ++i) ey[i] -= Uyy[i];
+        upload_real(etax, ex); upload_real(etay, ey);
+        upload_real(Uy, std::vector<double>(d.Uy, d.Uy + ny));
+        upload_real(Qy1, std::vector<double>(ny, 0.0));
+      }
+      return;
+    }
+    // MultiLayerQG (src/multilayerqg.jl:255-320, 338-347, 603-634)
+    advect_full = true;
+    const int n = nl;
+    for (int l = 0; l < ny; ++l)
+      for (int kr = 0; kr < nkr; ++kr) Lop[(size_t)l * nkr + kr] = cd(-d.nu * ipow(krsq(kr, l), d.nnu), 0);
+    Lop[0] = 0;
+    std::vector<double> H(d.H, d.H + n), b(d.b, d.b + n), gp(n - 1), Fm(n - 1), Fp(n - 1);
+    for (int j = 0; j < n - 1; ++j) {
+      gp[j] = b[j] - b[j + 1];
+      Fm[j] = d.f0 * d.f0 / (gp[j] * H[j + 1]);
+      Fp[j] = d.f0 * d.f0 / (gp[j] * H[j]);
+    }
+    std::vector<double> F((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+      F[j * n + j] = -((j < n - 1 ? Fp[j] : 0.0) + (j > 0 ? Fm[j - 1] : 0.0));
+      if (j < n - 1) F[j * n + j + 1] = Fp[j];
+      if (j > 0) F[j * n + j - 1] = Fm[j - 1];
+    }
+    // S^-1 = I / (-k^2 I + F), k^2 = 0 -> 1, S^-1[1,1] = 0
+    std::vector<double> S((size_t)fsz * n * n, 0.0);
+    for (int l = 0; l < ny; ++l)
+      for (int kr = 0; kr < nkr; ++kr) {
+        double k2 = krsq(kr, l);
+        if (k2 == 0) k2 = 1;
+        std::vector<double> A((size_t)n * n), Inv((size_t)n * n, 0.0);
+        for (int i = 0; i < n * n; ++i) A[i] = F[i];
+        for (int i = 0; i < n; ++i) A[i * n + i] -= k2, Inv[i * n + i] = 1.0;
+        // Gauss-Jordan with partial pivoting
+        for (int c = 0; c < n; ++c) {
+          int p = c;
+          for (int r = c + 1; r < n; ++r) if (std::fabs(A[r * n + c]) > std::fabs(A[p * n + c])) p = r;
+          if (p != c) for (int k = 0; k < n; ++k) { std::swap(A[c * n + k], A[p * n + k]); std::swap(Inv[c * n + k], Inv[p * n + k]); }
+          const double piv = A[c * n + c];
+          for (int k = 0; k < n; ++k) { A[c * n + k] /= piv; Inv[c * n + k] /= piv; }
+          for (int r = 0; r < n; ++r) if (r != c) {
+            const double fct = A[r * n + c];
+            for (int k = 0; k < n; ++k) { A[r * n + k] -= fct * A[c * n + k]; Inv[r * n + k] -= fct * Inv[c * n + k]; }
+          }
+        }
+        if (kr == 0 && l == 0) std::fill(Inv.begin(), Inv.end(), 0.0);
+        const long o = boff_h(kr, l, ny);
+        for (int i = 0; i < n * n; ++i) S[(size_t)o * n * n + i] = Inv[i];
+      }
+    upload_real(sinv, S);
+    // U(y) per layer, U_yy, Qy (src/multilayerqg.jl:271-316)
+    std::vector<double> U((size_t)n * ny, 0.0);
+    if (d.Uy) for (int i = 0; i < n * ny; ++i) U[i] = d.Uy[i];
+    std::vector<double> Qy((size_t)n * ny);
+    std::vector<double> Uyy((size_t)n * ny);
+    for (int j = 0; j < n; ++j) {
+      std::vector<double> U2(np);
+      for (int y = 0; y < ny; ++y) for (int x = 0; x < nx; ++x) U2[(size_t)y * nx + x] = U[(size_t)j * ny + y];
+      std::vector<double> uyy = spectral_op(U2, 2);
+      for (int y = 0; y < ny; ++y) Uyy[(size_t)j * ny + y] = uyy[(size_t)y * nx];
+    }
+    for (int j = 0; j < n; ++j)
+      for (int y = 0; y < ny; ++y) {
+        double q = d.beta - Uyy[(size_t)j * ny + y];
+        auto Uj = [&](int jj) { return U[(size_t)jj * ny + y]; };
+        if (j == 0) q -= Fp[0] * (Uj(1) - Uj(0));
+        else if (j == n - 1) q -= Fm[n - 2] * (Uj(n - 2) - Uj(n - 1));
+        else q = q - Fp[j] * (Uj(j + 1) - Uj(j)) - Fm[j - 1] * (Uj(j - 1) - Uj(j));
+        Qy[(size_t)j * ny + y] = q;
+      }
+    upload_real(Uy, U);
+    upload_real(Qy1, Qy);
+    std::vector<double> ex = has_eta ? spectral_op(etav, 0) : std::vector<double>(np, 0.0);
+    std::vector<double> ey = has_eta ? spectral_op(etav, 1) : std::vector<double>(np, 0.0);
+    for (size_t i = 0; i < np; ++i) { ex[i] += d.topo_grad_x; ey[i] += d.topo_grad_y; }
+    upload_real(etax, ex); upload_real(etay, ey);
+  }
+
+  bool filtered() const {
+    return d.stepper == GF_FILTERED_ETDRK4 || d.stepper == GF_FILTERED_RK4 || d.stepper == GF_FILTERED_FORWARD_EULER;
+  }
+  bool etd() const { return d.stepper == GF_ETDRK4 || d.stepper == GF_FILTERED_ETDRK4; }
+
+  void build_stepper() {
+    const int ny = d.ny, nkr = g.nkr;
+    const size_t nm = (size_t)ny * nkr;
+    auto up = [&](DevBuf& b, const std::vector<cd>& v) {
+      if (ccplx) {
+        std::vector<C> h(nm);
+        for (size_t i = 0; i < nm; ++i) { h[i].re = (T)v[i].real(); h[i].im = (T)v[i].imag(); }
+        upload_blocked(b, h);
+      } else {
+        std::vector<T> h(nm);
+        for (size_t i = 0; i < nm; ++i) h[i] = (T)v[i].real();
+        upload_blocked(b, h);
+      }
+    };
+    if (etd()) {
+      std::vector<cd> E(nm), E2(nm), Z(nm), A(nm), B(nm), G(nm);
+      const double dt = d.dt;
+      parallel_for((long)nm, [&](long i) {
+        const cd L = Lop[i];
+        E[i] = std::exp(dt * L);
+        E2[i] = std::exp(dt * L / 2.0);
+        etd_coeffs(dt * L, dt, Z[i], A[i], B[i], G[i]);
+        if (!ccplx) { Z[i] = Z[i].real(); A[i] = A[i].real(); B[i] = B[i].real(); G[i] = G[i].real(); }
+      });
+      up(coef[0], E2); up(coef[1], Z); up(coef[2], E); up(coef[3], A); up(coef[4], B); up(coef[5], G);
+    } else {
+      up(coef[0], Lop);
+    }
+    if (filtered()) {
+      // [FF] makefilter: K = sqrt((kr dx/pi)^2 + (l dy/pi)^2), exp(-decay (K - 0.65)^4) for K >= 0.65
+      const double dx = d.Lx / d.nx, dy = d.Ly / d.ny;
+      const double decay = -std::log(1e-15) / std::pow(1 - 0.65, 4);
+      std::vector<T> h(nm);
+      for (int l = 0; l < ny; ++l)
+        for (int kr = 0; kr < nkr; ++kr) {
+          const double a = (double)(T)krw(kr) * dx / M_PI, b = (double)(T)lw(l) * dy / M_PI;
+          const double K = std::sqrt(a * a + b * b);
+          h[(size_t)l * nkr + kr] = (T)(K < 0.65 ? 1.0 : std::exp(-decay * std::pow(K - 0.65, 4)));
+        }
+      upload_blocked(filter, h);
+    }
+  }
+
+  void build_forcing() {
+    if (d.forcing != GF_FORCING_RING) return;
+    // ring spectrum (examples/twodnavierstokes_stochasticforcing.jl:53-66)
+    const int ny = d.ny, nkr = g.nkr;
+    std::vector<double> spec((size_t)ny * nkr);
+    double U = 0;
+    for (int l = 0; l < ny; ++l)
+      for (int kr = 0; kr < nkr; ++kr) {
+        const double K2 = krsq(kr, l), K = std::sqrt(K2);
+        double s = std::exp(-(K - d.kf) * (K - d.kf) / (2 * d.dkf * d.dkf));
+        if (K2 == 0) s = 0;
+        spec[(size_t)l * nkr + kr] = s;
+        const double iK = K2 == 0 ? 0 : 1.0 / K2;
+        U += (kr == 0 ? 1.0 : 2.0) * s * iK / 2;
+      }
+    const double eps0 = U * d.Lx * d.Ly / ((double)d.nx * d.nx * (double)ny * ny) / (d.Lx * d.Ly);
+    std::vector<T> ss((size_t)ny * nkr);
+    fkx = -1; fky = -1;
+    for (int l = 0; l < ny; ++l)
+      for (int kr = 0; kr < nkr; ++kr) {
+        const double s = spec[(size_t)l * nkr + kr] * (d.eps / eps0);
+        ss[(size_t)l * nkr + kr] = (T)std::sqrt(s);
+        if (s > 0) {
+          fkx = std::max(fkx, kr);
+          fky = std::max(fky, l < (ny + 1) / 2 ? l : ny - l);
+        }
+      }
+    upload_blocked(sqrt_spec, ss);
+  }
+
+  // ---------------- hot path ----------------
+  void rec(int k, bool start) {
+    if (!timing_on) return;
+    if (start) {
+      Ev e; e.k = k;
+      HIPCHECK(hipEventCreate(&e.a)); HIPCHECK(hipEventCreate(&e.b));
+      HIPCHECK(hipEventRecord(e.a, stream));
+      evs.push_back(e);
+    } else {
+      HIPCHECK(hipEventRecord(evs.back().b, stream));
+    }
+  }
+
+  void calcN_launch(const C* in, int epi, bool full) {
+    K1Args<T> a{};
+    a.g = g; a.model = d.model; a.fieldset = FS_CALCN; a.sol = in; a.out = mixed.as<C>(); a.fstride = fsz;
+    a.ncols = g.kmax; a.eb = eb; a.ell2inv = (T)ell2inv; a.sinv = sinv.as<T>(); a.tw = twy.as<C>();
+    rec(0, true); launch_k1<T>(d.ny, a, stream); rec(0, false);
+    K2Args<T> b{};
+    b.g = g; b.mode = advect_full ? K2_ADVECT_FULL : K2_ADVECT; b.in = mixed.as<C>(); b.fstride = fsz;
+    b.kin = g.kmax; b.eta = (d.model == GF_MODEL_SLQG && !advect_full && has_eta) ? eta.as<T>() : nullptr;
+    b.Uy = Uy.as<T>(); b.Qy1 = Qy1.as<T>(); b.etax = etax.as<T>(); b.etay = etay.as<T>(); b.has_bg = advect_full;
+    b.out = flux.as<C>(); b.ostride = fsz; b.kout = full ? g.nkr : g.kmax; b.tw = twx.as<C>();
+    rec(1, true); launch_k2<T>(d.nx, b, stream); rec(1, false);
+    K3Args<T> c{};
+    c.g = g; c.model = d.model; c.in = flux.as<C>(); c.fstride = fsz; c.ncols = full ? g.nkr : g.kmax;
+    c.full = full; c.epi = epi; c.apply_kr = !advect_full;
+    c.etah = etah.as<C>(); c.Ueta = (T)d.U;
+    c.Fh = (d.forcing == GF_FORCING_FIXED) ? Fh.as<C>() : nullptr;
+    c.sqrt_spec = (d.forcing == GF_FORCING_RING && fkx >= 0) ? sqrt_spec.as<T>() : nullptr;
+    c.seed = d.seed; c.fstep = stepno; c.sqrt_dt = (T)std::sqrt(d.dt); c.fkx = fkx; c.fky = fky;
+    c.instance = d.instance;
+    if (d.model == GF_MODEL_MLQG && d.mu != 0) { c.mu_drag = (T)d.mu; c.drag_src = in; c.sinv = sinv.as<T>(); }
+    c.sol = sol.as<C>(); c.s1 = s1.as<C>(); c.s2 = s2.as<C>(); c.n1 = n1.as<C>(); c.n23 = n23.as<C>();
+    c.racc = racc.as<C>(); c.nout = nout.as<C>();
+    c.c0 = coef[0].p; c.c1 = coef[1].p; c.c2 = coef[2].p; c.c3 = coef[3].p; c.c4 = coef[4].p; c.c5 = coef[5].p;
+    c.ccplx = ccplx; c.cl_layered = 0; c.dt = (T)d.dt; c.filter = filtered() ? filter.as<T>() : nullptr;
+    c.tw = twy.as<C>();
+    rec(2, true); launch_k3<T>(d.ny, c, stream); rec(2, false);
+    HIPCHECK(hipGetLastError());
+  }
+
+  void step(int n) override {
+    for (int s = 0; s < n; ++s) {
+      const bool full = s == n - 1;
+      if (etd()) {
+        calcN_launch(sol.as<C>(), E_ETD1, full);
+        calcN_launch(s1.as<C>(), E_ETD2, full);
+        calcN_launch(s2.as<C>(), E_ETD3, full);
+        calcN_launch(s2.as<C>(), E_ETD4, full);
+      } else if (d.stepper == GF_RK4 || d.stepper == GF_FILTERED_RK4) {
+        calcN_launch(sol.as<C>(), E_RK1, full);
+        calcN_launch(s1.as<C>(), E_RK2, full);
+        calcN_launch(s1.as<C>(), E_RK3, full);
+        calcN_launch(s1.as<C>(), E_RK4, full);
+      } else {
+        calcN_launch(sol.as<C>(), E_FE, full);
+      }
+      t += d.dt;
+      stepno += 1;
+    }
+  }
+
+  void set_state(const void* src, hipMemcpyKind kind) override {
+    const size_t bytes = sizeof(C) * (size_t)g.nkr * d.ny * nl;
+    if (kind == hipMemcpyHostToDevice) {
+      HIPCHECK(hipMemcpyAsync(jl.p, src, bytes, hipMemcpyHostToDevice, stream));
+    } else {
+      HIPCHECK(hipMemcpyAsync(jl.p, src, bytes, hipMemcpyDeviceToDevice, stream));
+    }
+    launch_from_julia<T>(g, jl.as<C>(), sol.as<C>(), nl, stream);
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  void get_state(void* dst, hipMemcpyKind kind) override {
+    const size_t bytes = sizeof(C) * (size_t)g.nkr * d.ny * nl;
+    launch_to_julia<T>(g, sol.as<C>(), jl.as<C>(), nl, stream);
+    HIPCHECK(hipMemcpyAsync(dst, jl.p, bytes, kind == hipMemcpyDeviceToHost ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  void set_physical(const void* f) override {
+    const size_t np = (size_t)d.nx * d.ny * nl;
+    HIPCHECK(hipMemcpyAsync(phys.p, f, sizeof(T) * np, hipMemcpyHostToDevice, stream));
+    fwd_device(phys.as<T>(), sol.as<C>(), nl, d.model != GF_MODEL_SLQG);
+    launch_mask<T>(g, sol.as<C>(), nl, stream);   // updatevars!' dealias!
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  int nvars() const override { return d.model == GF_MODEL_TWODNS ? 3 : 4; }
+  void updatevars(void* out) override {
+    launch_mask<T>(g, sol.as<C>(), nl, stream);
+    K1Args<T> a{};
+    a.g = g; a.model = d.model; a.fieldset = FS_OUT; a.sol = sol.as<C>(); a.out = mixed.as<C>(); a.fstride = fsz;
+    a.ncols = g.kmax; a.eb = eb; a.ell2inv = (T)ell2inv; a.sinv = sinv.as<T>(); a.tw = twy.as<C>();
+    launch_k1<T>(d.ny, a, stream);
+    const int nf = nvars();
+    K2Args<T> b{};
+    b.g = g; b.g.nlayers = 1; b.mode = K2_INVERSE_OUT; b.nfields = nf * nl; b.in = mixed.as<C>();
+    b.fstride = fsz; b.kin = g.kmax; b.phys = phys.as<T>(); b.pstride = (long)d.nx * d.ny; b.tw = twx.as<C>();
+    launch_k2<T>(d.nx, b, stream);
+    HIPCHECK(hipGetLastError());
+    const size_t np = (size_t)d.nx * d.ny;
+    for (int f = 0; f < nf; ++f)
+      for (int j = 0; j < nl; ++j)
+        HIPCHECK(hipMemcpyAsync((char*)out + sizeof(T) * np * (f * nl + j), phys.as<T>() + np * (j * nf + f),
+                                sizeof(T) * np, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  void calcN(void* N, void* solh, double tt) override {
+    (void)tt;
+    const size_t bytes = sizeof(C) * (size_t)g.nkr * d.ny * nl;
+    HIPCHECK(hipMemcpyAsync(jl.p, solh, bytes, hipMemcpyHostToDevice, stream));
+    launch_from_julia<T>(g, jl.as<C>(), cinp.as<C>(), nl, stream);
+    calcN_launch(cinp.as<C>(), E_CALCN, true);
+    launch_to_julia<T>(g, nout.as<C>(), jl.as<C>(), nl, stream);
+    HIPCHECK(hipMemcpyAsync(N, jl.p, bytes, hipMemcpyDeviceToHost, stream));
+    launch_mask<T>(g, cinp.as<C>(), nl, stream);
+    launch_to_julia<T>(g, cinp.as<C>(), jl.as<C>(), nl, stream);
+    HIPCHECK(hipMemcpyAsync(solh, jl.p, bytes, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  void set_forcing(const void* F) override {
+    if (d.forcing != GF_FORCING_FIXED) throw GfError(GF_EINVAL, "gf_set_forcing: problem was not created with GF_FORCING_FIXED");
+    const size_t bytes = sizeof(C) * (size_t)g.nkr * d.ny * nl;
+    if (!Fh.p) Fh.alloc(sizeof(C) * fsz * nl);
+    HIPCHECK(hipMemcpyAsync(jl.p, F, bytes, hipMemcpyHostToDevice, stream));
+    launch_from_julia<T>(g, jl.as<C>(), Fh.as<C>(), nl, stream);
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  void timing(int enable) override {
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (auto& e : evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    evs.clear();
+    for (int k = 0; k < 3; ++k) { tl[k] = 0; tms[k] = 0; }
+    timing_on = enable != 0;
+  }
+  void timing_read(int k, int64_t* n, double* ms) override {
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (auto& e : evs) {
+      float x = 0;
+      HIPCHECK(hipEventElapsedTime(&x, e.a, e.b));
+      tl[e.k] += 1; tms[e.k] += x;
+      (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b);
+    }
+    evs.clear();
+    if (k < 0 || k > 2) throw GfError(GF_EINVAL, "timing class must be 0, 1 or 2");
+    *n = tl[k]; *ms = tms[k];
+  }
+  // compulsory HBM bytes per launch of kernel class k
+  double kernel_bytes(int k, int full) const override {
+    const double cs = sizeof(C), rs = sizeof(T);
+    const double ny = d.ny, kmax = g.kmax, nkr = g.nkr;
+    const double kept_l = (double)(g.la0 + (d.ny - g.la1));
+    const double kept = kmax * kept_l;                // surviving modes
+    const double cols = full ? nkr : kmax;
+    const double out_modes = full ? nkr * ny : kept;  // modes whose N is used
+    const double L = nl;
+    if (k == 0) {
+      double b = kept * cs * L + 3 * kmax * ny * cs * L;
+      if (d.model == GF_MODEL_MLQG) b += kept * L * L * rs;
+      return b;
+    }
+    if (k == 1) {
+      double b = 3 * kmax * ny * cs * L + 2 * cols * ny * cs * L;
+      if (d.model == GF_MODEL_SLQG && has_eta && !advect_full) b += (double)d.nx * ny * rs;
+      if (advect_full && has_eta) b += 2 * (double)d.nx * ny * rs;
+      return b;
+    }
+    // K3: flux read + the average stage epilogue of the configured stepper
+    double b = 2 * cols * ny * cs * L;
+    const double cf = ccplx ? cs : rs;
+    double e;
+    if (etd()) {
+      const double s1b = kept * cs * 2 + kept * cf * 2 + out_modes * cs;                       // sol,E2,Z -> s1, n1
+      const double s2b = s1b;                                                                    // sol,E2,Z -> s2, n23
+      const double s3b = kept * cs * 3 + kept * cf * 2 + out_modes * cs * 2;                    // s1,n1,E2,Z -> s2; n23 rmw
+      const double s4b = kept * cs + out_modes * cs * 3 + out_modes * cf * 4;                    // sol,n1,n23,E,a,b,G -> sol
+      e = (s1b + s2b + s3b + s4b) / 4;
+    } else if (d.stepper == GF_RK4 || d.stepper == GF_FILTERED_RK4) {
+      const double r1 = kept * cs * 2 + kept * cf + out_modes * cs;                             // sol,L -> s1, racc
+      const double r2 = kept * cs * 3 + kept * cf + out_modes * cs * 2;                         // sol,s1,L -> s1; racc rmw
+      const double r4 = kept * cs * 2 + out_modes * cs * 2 + out_modes * cf;                    // sol,s1,racc,L -> sol
+      e = (r1 + 2 * r2 + r4) / 4;
+    } else {
+      e = kept * cs + out_modes * cs + out_modes * cf;
+    }
+    return b + e * L;
+  }
+  void sync() override { HIPCHECK(hipStreamSynchronize(stream)); }
+};
+
+int validate(const gf_problem_desc* d, std::string& msg) {
+  if (!d) { msg = "null descriptor"; return GF_EINVAL; }
+  if (d->model < 0 || d->model > 2) { msg = "model must be GF_MODEL_TWODNS, GF_MODEL_SLQG or GF_MODEL_MLQG"; return GF_EINVAL; }
+  if (d->dtype != GF_F64 && d->dtype != GF_F32) { msg = "dtype must be GF_F64 or GF_F32"; return GF_EINVAL; }
+  if (!size_supported(d->nx) || !size_supported(d->ny)) {
+    msg = "nx and ny must be powers of two between 16 and 2048 (got " + std::to_string(d->nx) + "x" + std::to_string(d->ny) + ")";
+    return GF_EUNSUPPORTED;
+  }
+  if (!(d->Lx > 0) || !(d->Ly > 0)) { msg = "Lx and Ly must be positive"; return GF_EINVAL; }
+  if (!(d->dt > 0)) { msg = "dt must be positive"; return GF_EINVAL; }
+  if (d->stepper < 0 || d->stepper > 5) { msg = "unknown stepper"; return GF_EINVAL; }
+  if (d->aliased_fraction < 0 || d->aliased_fraction >= 1) { msg = "aliased_fraction must be in [0, 1)"; return GF_EINVAL; }
+  if (d->model == GF_MODEL_MLQG) {
+    if (d->nlayers < 2 || d->nlayers > 4) { msg = "MultiLayerQG: nlayers must be 2..4 in this build"; return GF_EUNSUPPORTED; }
+    if (!d->H || !d->b) { msg = "MultiLayerQG: H and b are required"; return GF_EINVAL; }
+    if (d->linear) { msg = "MultiLayerQG: linear=true is not supported by this build"; return GF_EUNSUPPORTED; }
+    for (int j = 0; j + 1 < d->nlayers; ++j)
+      if (d->b[j] - d->b[j + 1] == 0) { msg = "MultiLayerQG: equal buoyancy in adjacent layers"; return GF_EINVAL; }
+  } else if (d->nlayers != 1 && d->nlayers != 0) {
+    msg = "nlayers must be 1 for TwoDNavierStokes and SingleLayerQG"; return GF_EINVAL;
+  }
+  if (d->forcing < 0 || d->forcing > 2) { msg = "unknown forcing kind"; return GF_EINVAL; }
+  if (d->forcing == GF_FORCING_RING && (!(d->dkf > 0) || !(d->eps >= 0))) { msg = "ring forcing needs dkf > 0, eps >= 0"; return GF_EINVAL; }
+  if (d->model == GF_MODEL_TWODNS && d->nnu < 0) { msg = "nnu must be >= 0"; return GF_EINVAL; }
+  return GF_OK;
+}
+
+}  // namespace
+
+struct gf_ctx {
+  std::unique_ptr<CtxBase> impl;
+  std::string err;
+};
+
+template <typename F>
+static int guarded(gf_ctx* c, F f) {
+  if (!c) return GF_EINVAL;
+  try {
+    f();
+    c->err.clear();
+    return GF_OK;
+  } catch (const GfError& e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return GF_EHIP;
+  }
+}
+
+static thread_local std::string g_create_err;
+
+extern "C" {
+
+const char* gf_version(void) { return "gfhip 0.1 (gfx950)"; }
+
+const char* gf_error_string(int code) {
+  switch (code) {
+    case GF_OK: return "ok";
+    case GF_EINVAL: return "invalid argument";
+    case GF_EUNSUPPORTED: return "unsupported configuration";
+    case GF_EHIP: return "HIP runtime error";
+    case GF_ENOTFINITE: return "non-finite state";
+    case GF_ENODEVICE: return "no HIP device";
+    default: return "unknown error";
+  }
+}
+
+int gf_ctx_create(gf_ctx** out, const gf_problem_desc* desc) {
+  if (!out) return GF_EINVAL;
+  *out = nullptr;
+  std::string msg;
+  int rc = validate(desc, msg);
+  if (rc != GF_OK) { g_create_err = msg; return rc; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_err = "no HIP device"; return GF_ENODEVICE; }
+  auto* c = new gf_ctx();
+  try {
+    if (desc->dtype == GF_F64) c->impl.reset(new Ctx<double>(*desc));
+    else c->impl.reset(new Ctx<float>(*desc));
+  } catch (const GfError& e) {
+    g_create_err = e.what(); delete c; return e.code;
+  } catch (const std::exception& e) {
+    g_create_err = e.what(); delete c; return GF_EHIP;
+  }
+  *out = c;
+  return GF_OK;
+}
+
+void gf_ctx_destroy(gf_ctx* ctx) { delete ctx; }
+
+const char* gf_last_error(const gf_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int gf_set_state(gf_ctx* c, const void* s) {
+  if (!s) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->set_state(s, hipMemcpyHostToDevice); });
+}
+int gf_get_state(gf_ctx* c, void* s) {
+  if (!s) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->get_state(s, hipMemcpyDeviceToHost); });
+}
+int gf_set_state_dev(gf_ctx* c, const void* s) {
+  if (!s) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->set_state(s, hipMemcpyDeviceToDevice); });
+}
+int gf_get_state_dev(gf_ctx* c, void* s) {
+  if (!s) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->get_state(s, hipMemcpyDeviceToDevice); });
+}
+int gf_set_physical(gf_ctx* c, const void* f) {
+  if (!f) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->set_physical(f); });
+}
+int gf_updatevars(gf_ctx* c, void* out) {
+  if (!out) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->updatevars(out); });
+}
+int gf_num_vars(const gf_ctx* c) { return c ? c->impl->nvars() : GF_EINVAL; }
+int gf_calcN(gf_ctx* c, void* N, void* sol, double t) {
+  if (!N || !sol) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->calcN(N, sol, t); });
+}
+int gf_set_forcing(gf_ctx* c, const void* F) {
+  if (!F) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->set_forcing(F); });
+}
+int gf_step(gf_ctx* c, int n) {
+  if (n < 0) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->step(n); });
+}
+int gf_clock(const gf_ctx* c, double* t, int64_t* step) {
+  if (!c) return GF_EINVAL;
+  if (t) *t = c->impl->t;
+  if (step) *step = c->impl->stepno;
+  return GF_OK;
+}
+int gf_set_clock(gf_ctx* c, double t, int64_t step) {
+  if (!c) return GF_EINVAL;
+  c->impl->t = t; c->impl->stepno = step;
+  return GF_OK;
+}
+int gf_synchronize(gf_ctx* c) { return guarded(c, [&] { c->impl->sync(); }); }
+int gf_timing(gf_ctx* c, int enable) { return guarded(c, [&] { c->impl->timing(enable); }); }
+int gf_timing_read(gf_ctx* c, int k, int64_t* n, double* ms) {
+  if (!n || !ms) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->timing_read(k, n, ms); });
+}
+int gf_kernel_bytes(const gf_ctx* c, int k, int full, double* bytes) {
+  if (!c || !bytes || k < 0 || k > 2) return GF_EINVAL;
+  *bytes = c->impl->kernel_bytes(k, full);
+  return GF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,29 @@
+// Instantiation unit: K2 for float (sizes 16..2048, 1..4 layers).
+#include "gf_kernels_impl.hpp"
+namespace gf {
+template <int NL>
+static void k2_nl_f(int n, const K2Args<float>& a, hipStream_t s) {
+  switch (n) {
+    case 16: k2_n<float, 16, NL>(a, s); break;
+    case 32: k2_n<float, 32, NL>(a, s); break;
+    case 64: k2_n<float, 64, NL>(a, s); break;
+    case 128: k2_n<float, 128, NL>(a, s); break;
+    case 256: k2_n<float, 256, NL>(a, s); break;
+    case 512: k2_n<float, 512, NL>(a, s); break;
+    case 1024: k2_n<float, 1024, NL>(a, s); break;
+    case 2048: k2_n<float, 2048, NL>(a, s); break;
+    default: break;
+  }
+}
+// Inverse-out and forward-in treat every layer as one more field (NL = 1).
+template <> void launch_k2<float>(int n, const K2Args<float>& a, hipStream_t s) {
+  const bool adv = a.mode == K2_ADVECT || a.mode == K2_ADVECT_FULL;
+  switch (adv ? a.g.nlayers : 1) {
+    case 1: k2_nl_f<1>(n, a, s); break;
+    case 2: k2_nl_f<2>(n, a, s); break;
+    case 3: k2_nl_f<3>(n, a, s); break;
+    case 4: k2_nl_f<4>(n, a, s); break;
+    default: break;
+  }
+}
+}  // namespace gf

@@ -1,0 +1,127 @@
+// Kernel argument blocks and launch entry points shared by the host context
+// (gf_context.cpp) and the kernels (gf_kernels.hip).
+//
+// Device data layout ("column-blocked"): a spectral or mixed field with
+// nkr columns (kr) and ny rows (l or y) is stored as
+//     off(kr, j) = ((kr >> 1) * ny + j) * 2 + (kr & 1)
+// i.e. pairs of kr columns are interleaved and each pair runs contiguously
+// along y.  The y-direction passes (K1, K3) read and write one column pair
+// as one contiguous 32*ny-byte run; the x-direction pass (K2) reads and writes
+// 64-byte pieces (2 rows x 2 columns).  Layers (MultiLayerQG) and extra fields
+// follow at fixed strides.
+#pragma once
+#include <stdint.h>
+#include "gf_fft.hpp"
+
+namespace gf {
+
+enum Model { M_TWODNS = 0, M_SLQG = 1, M_MLQG = 2 };
+
+// Field codes computed by K1 from a spectral state.
+enum Field { F_U = 0, F_V = 1, F_Q = 2, F_PSI = 3, F_RAW = 4 };
+
+// Field sets produced by K1: FS_CALCN = (u, v, q) for the advection path,
+// FS_OUT = the physical outputs of updatevars! ((zeta, u, v) for TwoDNS,
+// (q, psi, u, v) for the QG models), FS_RAW = the state itself, unmasked.
+enum FieldSet { FS_CALCN = 0, FS_OUT = 1, FS_RAW = 2 };
+
+// K2 modes
+enum K2Mode { K2_ADVECT = 0, K2_INVERSE_OUT = 1, K2_FORWARD_IN = 2, K2_ADVECT_FULL = 3 };
+
+// K3 epilogues
+enum Epi {
+  E_CALCN = 0, E_STORE = 1,
+  E_ETD1 = 2, E_ETD2 = 3, E_ETD3 = 4, E_ETD4 = 5,
+  E_RK1 = 6, E_RK2 = 7, E_RK3 = 8, E_RK4 = 9,
+  E_FE = 10
+};
+
+constexpr int MAXL = 8;  // max layers
+
+template <typename T>
+struct Geo {
+  int nx, ny, nkr, nkr2;    // nkr2: nkr rounded up to even
+  int kmax;                 // kr < kmax survive dealias
+  int la0, la1;             // l-index range [la0, la1) is aliased
+  int nlayers;
+  T kx0, ky0;               // 2 pi/Lx, 2 pi/Ly
+  T inv_norm;               // 1/(nx ny)
+  long lstride;             // complex elements between layers of a spectral field (nkr2*ny)
+};
+
+template <typename T>
+struct K1Args {
+  Geo<T> g;
+  int model;
+  int fieldset;             // FS_CALCN (u, v, q), FS_OUT (updatevars! outputs), FS_RAW (state)
+  const cplx<T>* sol;       // input state (masked on read)
+  cplx<T>* out;             // fields of every layer, stride fstride
+  long fstride;
+  int ncols;                // kr columns to process (kr < ncols)
+  int eb;                   // SLQG equivalent-barotropic: psi = -q/(K^2 + ell2inv)
+  T ell2inv;
+  const T* sinv;            // MLQG: S^-1 per mode, n*n reals, blocked layout (n*n per mode)
+  const cplx<T>* tw;
+};
+
+template <typename T>
+struct K2Args {
+  Geo<T> g;
+  int mode;
+  int nfields;              // inverse-out / forward-in: number of fields per layer
+  const cplx<T>* in;        // mixed fields (u, v, q per layer), stride fstride
+  long fstride;
+  int kin;                  // kr < kin present in the input mixed fields
+  const T* eta;             // SLQG: physical eta (x fastest), nullable
+  // advect-full (MLQG / SLQG varying U): per-layer U(y), Qy(y); bottom-layer Qx/Qy fields
+  const T* Uy;              // nlayers*ny
+  const T* Qy1;             // nlayers*ny
+  const T* etax;            // ny*nx, bottom layer, nullable
+  const T* etay;            // ny*nx, bottom layer, nullable
+  int has_bg;               // any nonzero Qx/Qy/U term
+  cplx<T>* out;             // flux (2 fields per layer) or mixed (forward-in)
+  long ostride;
+  int kout;                 // write kr < kout
+  T* phys;                  // inverse-out: nfields physical fields (ny*nx each, x fastest)
+  const T* phys_in;         // forward-in
+  long pstride;
+  const cplx<T>* tw;
+};
+
+template <typename T>
+struct K3Args {
+  Geo<T> g;
+  int model;
+  const cplx<T>* in;        // flux fields (2 per layer), stride fstride
+  long fstride;
+  int ncols;                // kr columns to process (kr < ncols)
+  int full;                 // also produce the aliased band (last step / diagnostics)
+  int epi;
+  int apply_kr;             // N = -i kr A - i l B (else A - i l B)
+  const cplx<T>* etah; T Ueta;   // SLQG constant-U topography term
+  const cplx<T>* Fh;        // fixed forcing, nullable
+  const T* sqrt_spec;       // ring forcing amplitude (blocked real), nullable
+  uint64_t seed; int64_t fstep; T sqrt_dt; int fkx, fky; int instance;
+  // MLQG bottom drag mu Krsq psi_n
+  T mu_drag; const cplx<T>* drag_src;  // calcN input state (for psi_n)
+  const T* sinv;
+  // stepper state
+  cplx<T>* sol; cplx<T>* s1; cplx<T>* s2; cplx<T>* n1; cplx<T>* n23; cplx<T>* racc; cplx<T>* nout;
+  const void* c0; const void* c1; const void* c2; const void* c3; const void* c4; const void* c5;
+  int ccplx;                // coefficient tables are complex
+  int cl_layered;           // coefficient tables have one slice per layer
+  T dt;
+  const T* filter;          // nullable
+  int zero_mean;            // E_STORE: set mode (0,0) to zero
+  const cplx<T>* tw;
+};
+
+template <typename T> void launch_k1(int ny, const K1Args<T>& a, hipStream_t s);
+template <typename T> void launch_k2(int nx, const K2Args<T>& a, hipStream_t s);
+template <typename T> void launch_k3(int ny, const K3Args<T>& a, hipStream_t s);
+template <typename T> void launch_to_julia(const Geo<T>& g, const cplx<T>* src, cplx<T>* dst, int nl, hipStream_t s);
+template <typename T> void launch_from_julia(const Geo<T>& g, const cplx<T>* src, cplx<T>* dst, int nl, hipStream_t s);
+template <typename T> void launch_mask(const Geo<T>& g, cplx<T>* sol, int nl, hipStream_t s);
+bool size_supported(int n);
+
+}  // namespace gf

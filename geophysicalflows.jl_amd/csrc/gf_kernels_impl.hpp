@@ -1,0 +1,642 @@
+// Fused pseudospectral kernels for the nonlinear term and the stepper stages.
+//
+// One evaluation of calcN! (src/twodnavierstokes.jl:286-294,
+// src/singlelayerqg.jl:443-451, src/multilayerqg.jl:651-663) is three launches:
+//
+//   K1  spectral -> mixed : one thread group per kr column: masked read of the
+//       state (dealias!, P1), inversion to velocity (and S^-1 q for layers),
+//       inverse FFT along y.  Only the kr < kmax columns that survive the
+//       dealias mask are transformed.
+//   K2  mixed -> flux     : per row pair (y, y+1): Hermitian extension, c2r
+//       along x (two real rows per complex FFT), physical-space products,
+//       r2c along x (two real rows per complex FFT), split of the packed
+//       spectra.  Nothing physical touches HBM.
+//   K3  flux -> spectral  : one thread group per kr column: forward FFT along
+//       y, assembly of N (-i kr A - i l B, forcing, topography and drag terms)
+//       and the fused stepper stage (ETDRK4 / RK4 / ForwardEuler) as epilogue.
+//
+// This header holds the templates; gf_inst_*.hip instantiate them per size.
+#pragma once
+#include "gf_kernels.hpp"
+
+namespace gf {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ bool kept_l(const Geo<T>& g, int l) {
+  return l < g.la0 || l >= g.la1;
+}
+template <typename T> __device__ __forceinline__ T lwave(const Geo<T>& g, int l) {
+  return (T)(l < (g.ny + 1) / 2 ? l : l - g.ny) * g.ky0;
+}
+template <typename T> __device__ __forceinline__ T krwave(const Geo<T>& g, int kr) {
+  return (T)kr * g.kx0;
+}
+__device__ __forceinline__ long boff(int kr, int j, int ny) {
+  return ((long)(kr >> 1) * ny + j) * 2 + (kr & 1);
+}
+template <typename T> __device__ __forceinline__ cplx<T> czero() { return mk((T)0, (T)0); }
+
+// Philox-4x32-10 (Salmon et al. 2011); key (k0,k1), counter (c0..c3).
+__device__ __forceinline__ void philox(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
+                                       uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+}
+
+// Complex normal, variance 1/2 per part, keyed on (seed, step, index, instance).
+template <typename T>
+__device__ __noinline__ cplx<T> complex_normal(uint64_t seed, int64_t step, uint32_t index, uint32_t inst) {
+  uint32_t c0 = index, c1 = inst, c2 = (uint32_t)(uint64_t)step, c3 = (uint32_t)((uint64_t)step >> 32);
+  philox(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint64_t a = ((uint64_t)c0 << 32) | c1, b = ((uint64_t)c2 << 32) | c3;
+  const double u1 = ((double)(a >> 11) + 1.0) * 0x1.0p-53;
+  const double u2 = ((double)(b >> 11) + 1.0) * 0x1.0p-53;
+  const double r = sqrt(-log(u1));
+  const double th = 2.0 * 3.141592653589793 * u2;
+  double sn, cs;
+  sincos(th, &sn, &cs);
+  return mk((T)(r * cs), (T)(r * sn));
+}
+
+template <typename T> struct Coef {
+  const void* p; int cplx_;
+  __device__ __forceinline__ cplx<T> mul(long i, cplx<T> x) const {
+    if (cplx_) return cmul(((const cplx<T>*)p)[i], x);
+    return rmul(((const T*)p)[i], x);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// K1: spectral state -> mixed fields (inverse y-transform), one column / group
+// ---------------------------------------------------------------------------
+#ifndef GF_K1_WAVES
+#define GF_K1_WAVES 2
+#endif
+#ifndef GF_K2_WAVES
+#define GF_K2_WAVES 2
+#endif
+#ifndef GF_K3_WAVES
+#define GF_K3_WAVES 3
+#endif
+
+template <int MODEL, int FS> struct Fields {
+  static constexpr int N = FS == FS_CALCN ? 3 : (FS == FS_RAW ? 1 : (MODEL == M_TWODNS ? 3 : 4));
+  static constexpr int code(int f) {
+    return FS == FS_CALCN ? (f == 0 ? F_U : f == 1 ? F_V : F_Q)
+         : FS == FS_RAW ? F_RAW
+         : MODEL == M_TWODNS ? (f == 0 ? F_Q : f == 1 ? F_U : F_V)
+                             : (f == 0 ? F_Q : f == 1 ? F_PSI : f == 2 ? F_U : F_V);
+  }
+};
+
+template <typename T, int NY, int P, int G, int NL, int MODEL, int FS>
+__global__ __launch_bounds__(G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K1_WAVES)))
+void k1_kernel(const K1Args<T> a) {
+  constexpr int TPT = NY / P;
+  using FSET = Fields<MODEL, FS>;
+  __shared__ cplx<T> lds[G][NY];
+  const int grp = threadIdx.x / TPT, t = threadIdx.x % TPT;
+  const int kr = blockIdx.x * G + grp;          // column
+  const Geo<T>& g = a.g;
+  const bool active = kr < a.ncols && kr < g.nkr;
+  const bool masked = FS != FS_RAW;
+  const bool krk = kr < g.kmax || !masked;
+  const T kv = krwave(g, kr);
+  const cplx<T>* tw = a.tw;
+  cplx<T> s[NL][P];
+#pragma unroll
+  for (int j = 0; j < NL; ++j)
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const int l = t + TPT * q;
+      const bool lk = kept_l(g, l) || !masked;
+      cplx<T> val = czero<T>();
+      if (active && lk && krk) val = a.sol[j * g.lstride + boff(kr, l, NY)];
+      s[j][q] = val;
+    }
+  const T nrm = g.inv_norm;
+#pragma unroll
+  for (int jl = 0; jl < NL; ++jl) {
+    // MultiLayerQG: psi_jl = sum_k Sinv[jl][k] q_k (src/multilayerqg.jl:495-509, 563-583)
+    cplx<T> psi[P];
+    if constexpr (NL > 1 && FS != FS_RAW) {
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int l = t + TPT * q;
+        cplx<T> acc = czero<T>();
+        if (active) {
+          const T* m = a.sinv + boff(kr, l, NY) * (NL * NL) + jl * NL;
+#pragma unroll
+          for (int k = 0; k < NL; ++k) acc = acc + rmul(m[k], s[k][q]);
+        }
+        psi[q] = acc;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < FSET::N; ++f) {
+      constexpr int dummy = 0; (void)dummy;
+      const int code = FSET::code(f);
+      cplx<T> w[1][P];
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int l = t + TPT * q;
+        const T lv = lwave(g, l);
+        const cplx<T> sv = s[jl][q];
+        cplx<T> r = sv;
+        if (code == F_RAW || code == F_Q) {
+          r = sv;
+        } else if constexpr (MODEL == M_TWODNS) {
+          // uh = i l invK sol, vh = -i kr invK sol (src/twodnavierstokes.jl:254-255)
+          const T K2 = kv * kv + lv * lv;
+          const T iK = (kr == 0 && l == 0) ? (T)0 : (T)1 / K2;
+          if (code == F_U) { const T aa = lv * iK; r = mk(-aa * sv.im, aa * sv.re); }
+          else if (code == F_V) { const T bb = kv * iK; r = mk(bb * sv.im, -bb * sv.re); }
+          else { r = rmul(-iK, sv); }
+        } else {
+          cplx<T> ps;
+          if constexpr (NL > 1) {
+            ps = psi[q];
+          } else {
+            // psih = -invK qh, or -qh/(K^2 + 1/ell^2) (src/singlelayerqg.jl:484-490)
+            const T K2 = kv * kv + lv * lv;
+            if (a.eb) ps = mk(-sv.re / (K2 + a.ell2inv), -sv.im / (K2 + a.ell2inv));
+            else {
+              const T iK = (kr == 0 && l == 0) ? (T)0 : (T)1 / K2;
+              ps = rmul(-iK, sv);
+            }
+          }
+          // uh = -i l psih, vh = i kr psih (src/singlelayerqg.jl:360-361)
+          if (code == F_U) r = mk(lv * ps.im, -lv * ps.re);
+          else if (code == F_V) r = mk(-kv * ps.im, kv * ps.re);
+          else r = ps;
+        }
+        // normalised inverse = conj(F(conj(x)))/(nx ny); the scale is a power of two
+        w[0][q] = mk(r.re * nrm, -(r.im * nrm));
+      }
+      fft_forward<T, NY, P, 1>(w, &lds[grp][0], t, tw);
+      if (active) {
+        cplx<T>* o = a.out + (long)(jl * FSET::N + f) * a.fstride;
+#pragma unroll
+        for (int q = 0; q < P; ++q) o[boff(kr, t + TPT * q, NY)] = conj(w[0][q]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2: mixed -> flux (x-direction c2r, products, r2c) / forward / inverse-out
+// ---------------------------------------------------------------------------
+// Hermitian-extended entry k of a half spectrum (kr = k or NX-k).
+template <typename T, int NX>
+__device__ __forceinline__ cplx<T> herm_load(const cplx<T>* f, int k, int y, int ny, int kin, bool active) {
+  const int kk = k <= NX / 2 ? k : NX - k;
+  if (!active || kk >= kin) return czero<T>();
+  cplx<T> v = f[boff(kk, y, ny)];
+  if (kk == 0 || kk == NX / 2) v.im = (T)0;   // c2r ignores the imaginary DC/Nyquist parts
+  else if (k > NX / 2) v.im = -v.im;
+  return v;
+}
+
+// Split of a packed spectrum W = F(a + i b) of two real rows:
+// row 0: (W[k] + conj W[N-k])/2, row 1: (W[k] - conj W[N-k])/(2i).
+template <typename T>
+__device__ __forceinline__ cplx<T> split_row(cplx<T> Wk, cplx<T> Wm, int row) {
+  return row == 0 ? mk((Wk.re + Wm.re) / 2, (Wk.im - Wm.im) / 2)
+                  : mk((Wk.im + Wm.im) / 2, (Wm.re - Wk.re) / 2);
+}
+
+template <typename T, int NX, int P, int G, int MODE, int NL>
+__global__ __launch_bounds__(G * (NX / P)) __attribute__((amdgpu_waves_per_eu(GF_K2_WAVES)))
+void k2_kernel(const K2Args<T> a) {
+  constexpr int TPT = NX / P;
+  constexpr int MAXI = (NX / 2 + 1 + TPT - 1) / TPT;
+  __shared__ cplx<T> lds[G][NX];
+  const int grp = threadIdx.x / TPT, t = threadIdx.x % TPT;
+  const Geo<T>& g = a.g;
+  const int ny = g.ny;
+  const int rp = blockIdx.x * G + grp;
+  const bool active = rp < ny / 2;
+  const int y0 = 2 * rp;
+  const cplx<T>* tw = a.tw;
+  cplx<T>* L = &lds[grp][0];
+
+  if constexpr (MODE == K2_FORWARD_IN) {
+    // physical rows (y0, y0+1) of each field -> half spectra along x
+    for (int f = 0; f < a.nfields; ++f) {
+      cplx<T> w[1][P];
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int x = t + TPT * q;
+        T r0 = 0, r1 = 0;
+        if (active) {
+          r0 = a.phys_in[f * a.pstride + (long)y0 * NX + x];
+          r1 = a.phys_in[f * a.pstride + (long)(y0 + 1) * NX + x];
+        }
+        w[0][q] = mk(r0, r1);
+      }
+      fft_forward<T, NX, P, 1>(w, L, t, tw);
+#pragma unroll
+      for (int q = 0; q < P; ++q) L[swz<T>(t + TPT * q)] = w[0][q];
+      __syncthreads();
+      if (active) {
+        cplx<T>* o = a.out + f * a.ostride;
+        for (int it = t; it < 2 * a.kout; it += TPT) {
+          const int row = it & 1, kr = it >> 1;
+          o[boff(kr, y0 + row, ny)] = split_row(L[swz<T>(kr)], L[swz<T>((NX - kr) & (NX - 1))], row);
+        }
+      }
+      __syncthreads();
+    }
+  } else if constexpr (MODE == K2_INVERSE_OUT) {
+    // nfields mixed fields -> physical rows; fields paired (f, f+1) per row
+    for (int f = 0; f < a.nfields; f += 2) {
+      const bool two = f + 1 < a.nfields;
+      const cplx<T>* fa = a.in + f * a.fstride;
+      const cplx<T>* fb = a.in + (f + 1) * a.fstride;
+      for (int row = 0; row < 2; ++row) {
+        cplx<T> w[1][P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          const int k = t + TPT * q;
+          const cplx<T> A = herm_load<T, NX>(fa, k, y0 + row, ny, a.kin, active);
+          const cplx<T> B = two ? herm_load<T, NX>(fb, k, y0 + row, ny, a.kin, active) : czero<T>();
+          w[0][q] = mk(A.re - B.im, -(A.im + B.re));     // conj(A + i B)
+        }
+        fft_forward<T, NX, P, 1>(w, L, t, tw);
+        if (active) {
+#pragma unroll
+          for (int q = 0; q < P; ++q) {
+            const long o = (long)(y0 + row) * NX + t + TPT * q;
+            a.phys[f * a.pstride + o] = w[0][q].re;
+            if (two) a.phys[(f + 1) * a.pstride + o] = -w[0][q].im;
+          }
+        }
+      }
+    }
+  } else {
+    // ---- advection: per layer, u, v, q -> (u q, v q [, u Qx + v Qy]) ----
+    constexpr bool FULL = MODE == K2_ADVECT_FULL;
+    constexpr int NF = FULL ? 3 : 2;
+#pragma unroll
+    for (int jl = 0; jl < NL; ++jl) {
+      const cplx<T>* fu = a.in + (long)(3 * jl + 0) * a.fstride;
+      const cplx<T>* fv = a.in + (long)(3 * jl + 1) * a.fstride;
+      const cplx<T>* fq = a.in + (long)(3 * jl + 2) * a.fstride;
+      // inverse x-transforms, one packed pair at a time: (u0 + i v0), (u1 + i v1), (q0 + i q1)
+      cplx<T> w[3][P];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        cplx<T> wr[1][P];
+        const cplx<T>* fa = r == 2 ? fq : fu;
+        const cplx<T>* fb = r == 2 ? fq : fv;
+        const int ya = r == 1 ? y0 + 1 : y0;
+        const int yb = r == 0 ? y0 : y0 + 1;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          const int k = t + TPT * q;
+          const cplx<T> A = herm_load<T, NX>(fa, k, ya, ny, a.kin, active);
+          const cplx<T> B = herm_load<T, NX>(fb, k, yb, ny, a.kin, active);
+          wr[0][q] = mk(A.re - B.im, -(A.im + B.re));     // conj(A + i B)
+        }
+        fft_forward<T, NX, P, 1>(wr, L, t, tw);
+#pragma unroll
+        for (int q = 0; q < P; ++q) w[r][q] = wr[0][q];
+      }
+      // physical values: u_r = Re, v_r = -Im of w[r]; q_0 = Re w[2], q_1 = -Im w[2]
+      cplx<T> z[NF][P];
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int x = t + TPT * q;
+        T u0 = w[0][q].re, v0 = -w[0][q].im, u1 = w[1][q].re, v1 = -w[1][q].im;
+        T q0 = w[2][q].re, q1 = -w[2][q].im;
+        if constexpr (!FULL) {
+          if (a.eta != nullptr && active) {
+            // u (q + eta), v (q + eta) (src/singlelayerqg.jl:368-370)
+            q0 = q0 + a.eta[(long)y0 * NX + x];
+            q1 = q1 + a.eta[(long)(y0 + 1) * NX + x];
+          }
+          z[0][q] = mk(u0 * q0, v0 * q0);
+          z[1][q] = mk(u1 * q1, v1 * q1);
+        } else {
+          // u += U; N = -(u Qx)^ - (v Qy)^ - i kr (u q)^ - i l (v q)^ (src/multilayerqg.jl:695-731)
+          T c0 = 0, c1 = 0;
+          if (active && a.has_bg) {
+            u0 = u0 + a.Uy[jl * ny + y0];
+            u1 = u1 + a.Uy[jl * ny + y0 + 1];
+            T qy0 = a.Qy1[jl * ny + y0], qy1 = a.Qy1[jl * ny + y0 + 1];
+            T qx0 = 0, qx1 = 0;
+            if (jl == NL - 1) {
+              if (a.etax) { qx0 = a.etax[(long)y0 * NX + x]; qx1 = a.etax[(long)(y0 + 1) * NX + x]; }
+              if (a.etay) { qy0 = qy0 + a.etay[(long)y0 * NX + x]; qy1 = qy1 + a.etay[(long)(y0 + 1) * NX + x]; }
+            }
+            c0 = u0 * qx0 + v0 * qy0;
+            c1 = u1 * qx1 + v1 * qy1;
+          }
+          z[0][q] = mk(u0 * q0, v0 * q0);
+          z[1][q] = mk(u1 * q1, v1 * q1);
+          z[2][q] = mk(c0, c1);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < NF; ++r) {
+        cplx<T> zr[1][P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) zr[0][q] = z[r][q];
+        fft_forward<T, NX, P, 1>(zr, L, t, tw);
+#pragma unroll
+        for (int q = 0; q < P; ++q) z[r][q] = zr[0][q];
+      }
+      // split the packed spectra; rows handled one at a time through LDS
+      cplx<T>* oA = a.out + (long)(2 * jl + 0) * a.ostride;
+      cplx<T>* oB = a.out + (long)(2 * jl + 1) * a.ostride;
+      for (int row = 0; row < 2; ++row) {
+#pragma unroll
+        for (int q = 0; q < P; ++q) L[swz<T>(t + TPT * q)] = z[row][q];
+        __syncthreads();
+        cplx<T> Arow[MAXI], Brow[MAXI];
+#pragma unroll
+        for (int i = 0; i < MAXI; ++i) {
+          const int kr = t + i * TPT;
+          if (kr < a.kout) {
+            const cplx<T> Wk = L[swz<T>(kr)], Wm = L[swz<T>((NX - kr) & (NX - 1))];
+            Arow[i] = split_row(Wk, Wm, 0);
+            Brow[i] = split_row(Wk, Wm, 1);
+          }
+        }
+        __syncthreads();
+        if constexpr (FULL) {
+          // this row's C = u Qx + v Qy spectrum from the packed (C0 + i C1) transform
+#pragma unroll
+          for (int q = 0; q < P; ++q) L[swz<T>(t + TPT * q)] = z[2][q];
+          __syncthreads();
+#pragma unroll
+          for (int i = 0; i < MAXI; ++i) {
+            const int kr = t + i * TPT;
+            if (kr < a.kout) {
+              const cplx<T> Cv = split_row(L[swz<T>(kr)], L[swz<T>((NX - kr) & (NX - 1))], row);
+              const T kv = krwave(g, kr);
+              // P' = -i kr (u q)^ - (u Qx + v Qy)^
+              Arow[i] = mk(kv * Arow[i].im - Cv.re, -kv * Arow[i].re - Cv.im);
+            }
+          }
+          __syncthreads();
+        }
+        if (active) {
+#pragma unroll
+          for (int i = 0; i < MAXI; ++i) {
+            const int kr = t + i * TPT;
+            if (kr < a.kout) {
+              const long o = boff(kr, y0 + row, ny);
+              oA[o] = Arow[i];
+              oB[o] = Brow[i];
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K3: flux -> spectral N + stepper stage epilogue, one column / group
+// ---------------------------------------------------------------------------
+template <typename T, int NY, int P, int G, int NL>
+__global__ __launch_bounds__(G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K3_WAVES)))
+void k3_kernel(const K3Args<T> a) {
+  constexpr int TPT = NY / P;
+  __shared__ cplx<T> lds[G][NY];
+  const int grp = threadIdx.x / TPT, t = threadIdx.x % TPT;
+  const int kr = blockIdx.x * G + grp;
+  const Geo<T>& g = a.g;
+  const bool active = kr < a.ncols && kr < g.nkr;
+  const cplx<T>* tw = a.tw;
+  cplx<T>* L = &lds[grp][0];
+  const T kv = krwave(g, kr);
+
+  if (a.epi == E_STORE) {
+    // forward y-transform of mixed field(s) -> spectral state (set_q!/set_zeta!)
+    for (int jl = 0; jl < g.nlayers; ++jl) {
+      cplx<T> w[1][P];
+      const cplx<T>* f = a.in + (long)jl * a.fstride;
+#pragma unroll
+      for (int q = 0; q < P; ++q) w[0][q] = active ? f[boff(kr, t + TPT * q, NY)] : czero<T>();
+      fft_forward<T, NY, P, 1>(w, L, t, tw);
+      if (active) {
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          const int l = t + TPT * q;
+          cplx<T> v = w[0][q];
+          if (a.zero_mean && kr == 0 && l == 0) v = czero<T>();
+          a.sol[jl * g.lstride + boff(kr, l, NY)] = v;
+        }
+      }
+    }
+    return;
+  }
+
+  // Bottom drag mu Krsq psih_n (src/multilayerqg.jl:658) needs psi_n of the calcN
+  // input; the stage epilogue below may overwrite that input in place (ETDRK4
+  // stage 3, RK4 stages 2-3), so psi_n is formed before any layer is written.
+  cplx<T> psin[P];
+  if constexpr (NL > 1) {
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const int l = t + TPT * q;
+      cplx<T> ps = czero<T>();
+      if (active && a.drag_src != nullptr && kept_l(g, l) && kr < g.kmax) {
+        const long idx = boff(kr, l, NY);
+        const T* m = a.sinv + idx * (NL * NL) + (NL - 1) * NL;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) ps = ps + rmul(m[k], a.drag_src[k * g.lstride + idx]);
+      }
+      psin[q] = ps;
+    }
+  }
+  for (int jl = 0; jl < NL; ++jl) {
+    cplx<T> A[1][P], B[1][P];
+    const cplx<T>* fa = a.in + (long)(2 * jl) * a.fstride;
+    const cplx<T>* fb = a.in + (long)(2 * jl + 1) * a.fstride;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const int y = t + TPT * q;
+      A[0][q] = active ? fa[boff(kr, y, NY)] : czero<T>();
+      B[0][q] = active ? fb[boff(kr, y, NY)] : czero<T>();
+    }
+    fft_forward<T, NY, P, 1>(A, L, t, tw);
+    fft_forward<T, NY, P, 1>(B, L, t, tw);
+    if (!active) continue;
+    const long lo = (long)jl * g.lstride;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const int l = t + TPT * q;
+      const bool kept = kept_l(g, l) && kr < g.kmax;
+      if (!kept && !a.full) continue;
+      const T lv = lwave(g, l);
+      const long idx = boff(kr, l, NY);
+      const long ix = lo + idx;
+      cplx<T> N;
+      const cplx<T> Av = A[0][q], Bv = B[0][q];
+      if (a.apply_kr) {
+        // N = -i kr (u zeta)^ - i l (v zeta)^ (src/twodnavierstokes.jl:272)
+        N = mk(kv * Av.im + lv * Bv.im, -kv * Av.re - lv * Bv.re);
+      } else {
+        N = mk(Av.re + lv * Bv.im, Av.im - lv * Bv.re);
+      }
+      if (a.etah != nullptr) {
+        // N -= i kr U etah (src/singlelayerqg.jl:379)
+        const cplx<T> e = a.etah[idx];
+        const T cu = kv * a.Ueta;
+        N = mk(N.re + cu * e.im, N.im - cu * e.re);
+      }
+      if constexpr (NL > 1) {
+        if (a.drag_src != nullptr && jl == NL - 1) {
+          // bottom drag N_n += mu Krsq psih_n (src/multilayerqg.jl:658)
+          const T K2 = kv * kv + lv * lv;
+          N = N + rmul(a.mu_drag * K2, psin[q]);
+        }
+      }
+      if (a.Fh != nullptr) N = N + a.Fh[ix];
+      if (a.sqrt_spec != nullptr) {
+        if (kr <= a.fkx && (l <= a.fky || l >= NY - a.fky)) {
+          const T sp = a.sqrt_spec[idx];
+          if (sp != (T)0) {
+            const cplx<T> xi = complex_normal<T>(a.seed, a.fstep, (uint32_t)(l * g.nkr + kr), (uint32_t)a.instance);
+            N = N + mk((xi.re * sp) / a.sqrt_dt, (xi.im * sp) / a.sqrt_dt);
+          }
+        }
+      }
+      const long ci = a.cl_layered ? ix : idx;
+      const cplx<T> sv = kept ? a.sol[ix] : czero<T>();
+      switch (a.epi) {
+        case E_CALCN:
+          a.nout[ix] = N;
+          break;
+        case E_ETD1: {  // sol1 = E2 sol + zeta N1
+          if (kept) {
+            const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
+            a.s1[ix] = E2.mul(ci, sv) + Z.mul(ci, N);
+          }
+          a.n1[ix] = N;
+        } break;
+        case E_ETD2: {  // sol2 = E2 sol + zeta N2
+          if (kept) {
+            const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
+            a.s2[ix] = E2.mul(ci, sv) + Z.mul(ci, N);
+          }
+          a.n23[ix] = N;
+        } break;
+        case E_ETD3: {  // sol2 = E2 sol1 + zeta (2 N3 - N1)
+          if (kept) {
+            const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
+            const cplx<T> s1v = a.s1[ix];
+            const cplx<T> n1v = a.n1[ix];
+            a.s2[ix] = E2.mul(ci, s1v) + Z.mul(ci, mk((T)2 * N.re - n1v.re, (T)2 * N.im - n1v.im));
+          }
+          a.n23[ix] = a.n23[ix] + N;
+        } break;
+        case E_ETD4: {  // sol = E sol + alpha N1 + 2 beta (N2 + N3) + Gamma N4
+          const Coef<T> E{a.c2, a.ccplx}, Al{a.c3, a.ccplx}, Be{a.c4, a.ccplx}, Ga{a.c5, a.ccplx};
+          const cplx<T> n23v = a.n23[ix];
+          cplx<T> r = E.mul(ci, sv) + Al.mul(ci, a.n1[ix]);
+          r = r + Be.mul(ci, mk((T)2 * n23v.re, (T)2 * n23v.im));
+          r = r + Ga.mul(ci, N);
+          if (a.filter) r = rmul(a.filter[idx], r);
+          a.sol[ix] = r;
+        } break;
+        case E_RK1:
+        case E_RK2:
+        case E_RK3:
+        case E_RK4: {
+          const Coef<T> Lc{a.c0, a.ccplx};
+          const cplx<T> xin = (a.epi == E_RK1) ? sv : (kept ? a.s1[ix] : czero<T>());
+          const cplx<T> R = N + Lc.mul(ci, xin);   // RHS = N + L sol_i
+          if (a.epi == E_RK1) {
+            if (kept) a.s1[ix] = sv + rmul(a.dt / (T)2, R);
+            a.racc[ix] = mk(R.re / (T)6, R.im / (T)6);
+          } else if (a.epi == E_RK2) {
+            if (kept) a.s1[ix] = sv + rmul(a.dt / (T)2, R);
+            const cplx<T> ra = a.racc[ix];
+            a.racc[ix] = mk(ra.re + R.re / (T)3, ra.im + R.im / (T)3);
+          } else if (a.epi == E_RK3) {
+            if (kept) a.s1[ix] = sv + rmul(a.dt, R);
+            const cplx<T> ra = a.racc[ix];
+            a.racc[ix] = mk(ra.re + R.re / (T)3, ra.im + R.im / (T)3);
+          } else {
+            const cplx<T> ra = a.racc[ix];
+            const cplx<T> acc = mk(ra.re + R.re / (T)6, ra.im + R.im / (T)6);
+            cplx<T> r = sv + rmul(a.dt, acc);
+            if (a.filter) r = rmul(a.filter[idx], r);
+            a.sol[ix] = r;
+          }
+        } break;
+        case E_FE: {
+          const Coef<T> Lc{a.c0, a.ccplx};
+          cplx<T> r = sv + rmul(a.dt, N + Lc.mul(ci, sv));
+          if (a.filter) r = rmul(a.filter[idx], r);
+          a.sol[ix] = r;
+        } break;
+        default: break;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// size dispatch helpers
+// ---------------------------------------------------------------------------
+template <int N> struct PY { static constexpr int P = N >= 4096 ? 16 : (N >= 32 ? 8 : 4); };
+template <int N, int P> struct GW { static constexpr int G = (N / P) >= 64 ? 1 : 64 / (N / P); };
+
+template <typename T, int N, int NL>
+void k1_n(const K1Args<T>& a, hipStream_t s) {
+  constexpr int P = (NL > 1 && N >= 32) ? 4 : PY<N>::P;
+  constexpr int G = GW<N, P>::G;
+  const int nblk = (a.ncols + G - 1) / G;
+  if (nblk <= 0) return;
+  const dim3 grid(nblk), blk(G * (N / P));
+  if constexpr (NL > 1) {
+    if (a.fieldset == FS_CALCN) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_CALCN>), grid, blk, 0, s, a);
+    else if (a.fieldset == FS_OUT) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_OUT>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_RAW>), grid, blk, 0, s, a);
+  } else {
+    if (a.model == M_TWODNS) {
+      if (a.fieldset == FS_CALCN) hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_CALCN>), grid, blk, 0, s, a);
+      else if (a.fieldset == FS_OUT) hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_OUT>), grid, blk, 0, s, a);
+      else hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_RAW>), grid, blk, 0, s, a);
+    } else {
+      if (a.fieldset == FS_CALCN) hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_SLQG, FS_CALCN>), grid, blk, 0, s, a);
+      else if (a.fieldset == FS_OUT) hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_SLQG, FS_OUT>), grid, blk, 0, s, a);
+      else hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_SLQG, FS_RAW>), grid, blk, 0, s, a);
+    }
+  }
+}
+template <typename T, int N, int NL>
+void k2_n(const K2Args<T>& a, hipStream_t s) {
+  constexpr int P = PY<N>::P;
+  constexpr int G = GW<N, P>::G;
+  const dim3 grid((a.g.ny / 2 + G - 1) / G), blk(G * (N / P));
+  switch (a.mode) {
+    case K2_ADVECT: hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT, NL>), grid, blk, 0, s, a); break;
+    case K2_ADVECT_FULL: hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT_FULL, NL>), grid, blk, 0, s, a); break;
+    case K2_INVERSE_OUT: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_INVERSE_OUT, 1>), grid, blk, 0, s, a); break;
+    case K2_FORWARD_IN: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_FORWARD_IN, 1>), grid, blk, 0, s, a); break;
+    default: break;
+  }
+}
+template <typename T, int N, int NL>
+void k3_n(const K3Args<T>& a, hipStream_t s) {
+  constexpr int P = (NL > 1 && N >= 32) ? 4 : PY<N>::P;
+  constexpr int G = GW<N, P>::G;
+  const int nblk = (a.ncols + G - 1) / G;
+  if (nblk > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, G, NL>), dim3(nblk), dim3(G * (N / P)), 0, s, a);
+}
+
+}  // namespace gf

@@ -1,0 +1,221 @@
+"""Host-side mirror of the FourierFlows.jl objects the reference builds around its hot path.
+
+``Problem`` holds what ``FourierFlows.Problem(eqn, stepper, dt, grid, vars, params)``
+holds for the reference (``src/twodnavierstokes.jl:99``): the grid description,
+the parameters, a ``Clock``, ``sol`` and ``vars``.  The state itself lives in
+device memory owned by the C ABI context; ``prob.sol`` copies it out (Julia
+memory layout, NumPy shape ``(nl, nkr)`` or ``(nlayers, nl, nkr)``).
+
+Array conventions: a reference array ``f[x, y]`` (column-major, ``(nx, ny)``)
+is a NumPy array of shape ``(ny, nx)`` here -- the same bytes.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+class CPU:
+    """The reference's ``CPU()`` device: not provided by this library (HIP only)."""
+
+    def __init__(self):
+        raise NotImplementedError("this library runs on the GPU only; use GPU()")
+
+
+class GPU:
+    """The device handle (one HIP device, the current one)."""
+
+
+class RingForcing:
+    """Stochastic forcing on a wavenumber ring, regenerated once per step.
+
+    Restates ``examples/twodnavierstokes_stochasticforcing.jl:53-82``: spectrum
+    ``exp(-(K - kf)^2 / (2 dkf^2))`` (zero at K = 0) normalised to inject energy at
+    rate ``eps``; ``Fh = xi sqrt(spectrum) / sqrt(dt)`` with ``xi`` complex normal
+    (variance 1/2 per part) from a counter-based generator keyed on ``seed``.
+    """
+
+    def __init__(self, kf, dkf, eps, seed=0, instance=0):
+        self.kf, self.dkf, self.eps, self.seed, self.instance = float(kf), float(dkf), float(eps), int(seed), int(instance)
+
+
+class Clock:
+    """[FF] ``Clock``: ``dt``, ``t``, ``step`` (read from the device context)."""
+
+    def __init__(self, prob):
+        self._p = prob
+
+    @property
+    def dt(self):
+        return self._p._dt
+
+    @property
+    def t(self):
+        t = ctypes.c_double()
+        s = ctypes.c_int64()
+        check(lib.gf_clock(self._p._ctx, ctypes.byref(t), ctypes.byref(s)), self._p._ctx)
+        return t.value
+
+    @property
+    def step(self):
+        t = ctypes.c_double()
+        s = ctypes.c_int64()
+        check(lib.gf_clock(self._p._ctx, ctypes.byref(t), ctypes.byref(s)), self._p._ctx)
+        return s.value
+
+
+class Grid:
+    """Description of [FF] ``TwoDGrid``: sizes, spacings, wavenumbers, alias ranges."""
+
+    def __init__(self, nx, ny, Lx, Ly, aliased_fraction, T):
+        self.nx, self.ny, self.Lx, self.Ly = nx, ny, float(Lx), float(Ly)
+        self.nkr, self.nl = nx // 2 + 1, ny
+        self.dx, self.dy = self.Lx / nx, self.Ly / ny
+        self.T = T
+        self.aliased_fraction = aliased_fraction
+        self.x = -self.Lx / 2 + np.arange(nx) * self.dx
+        self.y = -self.Ly / 2 + np.arange(ny) * self.dy
+        self.kr = np.arange(self.nkr) * ((2 * math.pi / self.Lx * nx) / nx)
+        li = np.arange(ny)
+        self.l = np.where(li < (ny + 1) // 2, li, li - ny) * ((2 * math.pi / self.Ly * ny) / ny)
+        self.Krsq = self.kr[None, :] ** 2 + self.l[:, None] ** 2
+
+    def gridpoints(self):
+        X = np.broadcast_to(self.x[None, :], (self.ny, self.nx)).copy()
+        Y = np.broadcast_to(self.y[:, None], (self.ny, self.nx)).copy()
+        return X, Y
+
+
+class Vars:
+    """Physical fields written by ``updatevars`` (names as in the reference's Vars)."""
+
+
+class Problem:
+    """Device-resident problem; see ``TwoDNavierStokes.Problem`` etc. for the constructors."""
+
+    _names = ()
+
+    def __init__(self, desc, grid, params, nlayers=1):
+        self._desc = desc
+        self._dt = desc.dt
+        self.grid = grid
+        self.params = params
+        self.nlayers = nlayers
+        self.vars = Vars()
+        self.stepper = params.stepper
+        self._ctype = np.complex128 if desc.dtype == _lib.GF_F64 else np.complex64
+        self._rtype = np.float64 if desc.dtype == _lib.GF_F64 else np.float32
+        ctx = ctypes.c_void_p()
+        check(lib.gf_ctx_create(ctypes.byref(ctx), ctypes.byref(desc)))
+        self._ctx = ctx
+        self.clock = Clock(self)
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx is not None and ctx.value:
+            lib.gf_ctx_destroy(ctx)
+            self._ctx = None
+
+    # ---- state ----
+    @property
+    def _sshape(self):
+        g = self.grid
+        return (g.nl, g.nkr) if self.nlayers == 1 else (self.nlayers, g.nl, g.nkr)
+
+    @property
+    def _pshape(self):
+        g = self.grid
+        return (g.ny, g.nx) if self.nlayers == 1 else (self.nlayers, g.ny, g.nx)
+
+    @property
+    def sol(self):
+        out = np.empty(self._sshape, dtype=self._ctype)
+        check(lib.gf_get_state(self._ctx, out.ctypes.data), self._ctx)
+        return out
+
+    @sol.setter
+    def sol(self, value):
+        a = np.ascontiguousarray(np.asarray(value, dtype=self._ctype).reshape(self._sshape))
+        check(lib.gf_set_state(self._ctx, a.ctypes.data), self._ctx)
+
+    def set_state_device(self, ptr):
+        check(lib.gf_set_state_dev(self._ctx, ctypes.c_void_p(ptr)), self._ctx)
+
+    def get_state_device(self, ptr):
+        check(lib.gf_get_state_dev(self._ctx, ctypes.c_void_p(ptr)), self._ctx)
+
+    def _set_physical(self, f):
+        a = np.ascontiguousarray(np.asarray(f, dtype=self._rtype).reshape(self._pshape))
+        check(lib.gf_set_physical(self._ctx, a.ctypes.data), self._ctx)
+
+    def _updatevars(self):
+        nf = lib.gf_num_vars(self._ctx)
+        out = np.empty((nf,) + self._pshape, dtype=self._rtype)
+        check(lib.gf_updatevars(self._ctx, out.ctypes.data), self._ctx)
+        for name, arr in zip(self._names, out):
+            setattr(self.vars, name, arr)
+
+    def calcN(self, sol, t=None):
+        """``calcN!(N, sol, t, ...)``: returns (N, sol dealiased in place as the reference does)."""
+        s = np.ascontiguousarray(np.asarray(sol, dtype=self._ctype).reshape(self._sshape)).copy()
+        N = np.empty(self._sshape, dtype=self._ctype)
+        check(lib.gf_calcN(self._ctx, N.ctypes.data, s.ctypes.data, float(self.clock.t if t is None else t)), self._ctx)
+        return N, s
+
+    def set_forcing(self, Fh):
+        a = np.ascontiguousarray(np.asarray(Fh, dtype=self._ctype).reshape(self._sshape))
+        check(lib.gf_set_forcing(self._ctx, a.ctypes.data), self._ctx)
+
+    def synchronize(self):
+        check(lib.gf_synchronize(self._ctx), self._ctx)
+
+    def set_clock(self, t, step):
+        check(lib.gf_set_clock(self._ctx, float(t), int(step)), self._ctx)
+
+
+def stepforward(prob, nsteps=1):
+    """[FF] ``stepforward!(prob, nsteps)``: advance ``nsteps`` on the device."""
+    check(lib.gf_step(prob._ctx, int(nsteps)), prob._ctx)
+
+
+def make_desc(model, nx, ny, Lx, Ly, T, stepper, dt, aliased_fraction):
+    if stepper not in _lib.STEPPERS:
+        raise ValueError(f"unknown stepper {stepper!r}; supported: {sorted(_lib.STEPPERS)}")
+    d = _lib.ProblemDesc()
+    d.model = model
+    if T in (np.float64, float, "Float64"):
+        d.dtype = _lib.GF_F64
+    elif T in (np.float32, "Float32"):
+        d.dtype = _lib.GF_F32
+    else:
+        raise ValueError("T must be np.float64 or np.float32")
+    d.nx, d.ny, d.nlayers = int(nx), int(ny), 1
+    d.Lx, d.Ly = float(Lx), float(Ly)
+    d.aliased_fraction = float(aliased_fraction)
+    d.stepper = _lib.STEPPERS[stepper]
+    d.dt = float(dt)
+    return d
+
+
+def dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def attach_forcing(d, calcF):
+    """Translate a calcF argument into the descriptor; returns the fixed table or None."""
+    if calcF is None:
+        d.forcing = _lib.GF_FORCING_NONE
+        return None
+    if isinstance(calcF, RingForcing):
+        d.forcing = _lib.GF_FORCING_RING
+        d.kf, d.dkf, d.eps, d.seed, d.instance = calcF.kf, calcF.dkf, calcF.eps, calcF.seed, calcF.instance
+        return None
+    if callable(calcF):
+        raise NotImplementedError(
+            "arbitrary calcF closures cannot run on the device path; pass a fixed spectral "
+            "array or a RingForcing")
+    d.forcing = _lib.GF_FORCING_FIXED
+    return np.asarray(calcF)

@@ -1,0 +1,70 @@
+"""TwoDNavierStokes on the device (mirror of ``src/twodnavierstokes.jl``).
+
+``Problem`` takes the reference's keyword arguments (``src/twodnavierstokes.jl:71-89``);
+the linear operator ``L = -nu |k|^(2 nnu) - mu |k|^(2 nmu)`` (149-154), ``calcN!``
+(286-294) and the time stepper run in the HIP library.  ``set_ζ``/``updatevars``
+mirror ``set_ζ!``/``updatevars!`` (331-360).  The diagnostics (374-535) are
+Parseval sums evaluated on the host from ``prob.sol``.
+"""
+import math
+
+import numpy as np
+
+from . import _lib
+from .problem import GPU, Problem as _Problem, attach_forcing, make_desc, stepforward  # noqa: F401
+from .spectral import parsevalsum
+
+
+class Params:
+    def __init__(self, ν, nν, μ, nμ, calcF, stepper):
+        self.ν, self.nν, self.μ, self.nμ, self.calcF, self.stepper = ν, nν, μ, nμ, calcF, stepper
+
+
+class TwoDNSProblem(_Problem):
+    _names = ("ζ", "u", "v")
+
+
+def Problem(dev=None, *, nx=256, ny=None, Lx=2 * math.pi, Ly=None, ν=0.0, nν=1, μ=0.0, nμ=0,
+            dt=0.01, stepper="RK4", calcF=None, stochastic=False, aliased_fraction=1 / 3, T=np.float64):
+    from .grid import make_grid
+    ny = nx if ny is None else ny
+    Ly = Lx if Ly is None else Ly
+    d = make_desc(_lib.GF_MODEL_TWODNS, nx, ny, Lx, Ly, T, stepper, dt, aliased_fraction)
+    d.nu, d.nnu, d.mu, d.nmu = float(ν), int(nν), float(μ), int(nμ)
+    Fh = attach_forcing(d, calcF)
+    p = TwoDNSProblem(d, make_grid(nx, ny, Lx, Ly, aliased_fraction, T), Params(float(ν), int(nν), float(μ), int(nμ), calcF, stepper))
+    if Fh is not None:
+        p.set_forcing(Fh)
+    return p
+
+
+def set_ζ(prob, ζ):
+    """``set_ζ!``: rfft on the device, ``sol[1,1] = 0``, then ``updatevars!``."""
+    prob._set_physical(ζ)
+    prob._updatevars()
+
+
+def updatevars(prob):
+    prob._updatevars()
+
+
+def _norm(prob):
+    g = prob.grid
+    return 1 / (g.Lx * g.Ly)
+
+
+def energy(prob):
+    g = prob.grid
+    with np.errstate(divide="ignore"):
+        iK = np.where(g.Krsq > 0, 1.0 / np.where(g.Krsq > 0, g.Krsq, 1.0), 0.0)
+    return _norm(prob) * parsevalsum(0.5 * iK * np.abs(prob.sol) ** 2, g)
+
+
+def enstrophy(prob):
+    g = prob.grid
+    return 1 / (2 * g.Lx * g.Ly) * parsevalsum(np.abs(prob.sol) ** 2, g)
+
+
+def palinstrophy(prob):
+    g = prob.grid
+    return _norm(prob) * parsevalsum(0.5 * g.Krsq * np.abs(prob.sol) ** 2, g)

@@ -1,0 +1,151 @@
+/*
+ * gf.h -- C ABI of the MI355X pseudospectral time-stepping library (libgfhip.so).
+ *
+ * This is the drop-in boundary for GeophysicalFlows.jl's hot path: the
+ * per-step nonlinear term calcN! of TwoDNavierStokes, SingleLayerQG and
+ * MultiLayerQG and the stepper stages behind FourierFlows' stepforward!.
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the GeophysicalFlows.jl v0.16.5 source tree; [FF] = FourierFlows.jl,
+ * the reference's dependency, called at the cited line).
+ *
+ * Conventions
+ *  - Every call returns GF_OK (0) or a negative GF_E* code; nothing throws or
+ *    aborts across the ABI.  gf_last_error(ctx) gives the message.
+ *  - Spectral arrays crossing the ABI use the reference's memory layout: the
+ *    column-major Julia array sol[kr, l(, layer)] with interleaved complex
+ *    (re, im) of the problem's precision (double or float), nkr = nx/2 + 1.
+ *    Physical arrays are f[x, y(, layer)], x fastest.
+ *  - "_dev" entry points take device pointers (HIP, same process); the others
+ *    take host pointers and copy synchronously.
+ *  - One context owns one HIP stream and all of its device memory; a context
+ *    is not thread-safe, distinct contexts may be used concurrently.
+ */
+#ifndef GF_H
+#define GF_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GF_OK = 0,
+  GF_EINVAL = -1,      /* invalid argument or descriptor                     */
+  GF_EUNSUPPORTED = -2,/* configuration not supported by this build          */
+  GF_EHIP = -3,        /* HIP runtime error (allocation, launch, copy)        */
+  GF_ENOTFINITE = -4,  /* non-finite values detected in the state            */
+  GF_ENODEVICE = -5    /* no HIP device                                        */
+};
+
+enum { GF_MODEL_TWODNS = 0, GF_MODEL_SLQG = 1, GF_MODEL_MLQG = 2 };
+enum { GF_F64 = 0, GF_F32 = 1 };
+enum {
+  GF_ETDRK4 = 0, GF_FILTERED_ETDRK4 = 1, GF_RK4 = 2, GF_FILTERED_RK4 = 3,
+  GF_FORWARD_EULER = 4, GF_FILTERED_FORWARD_EULER = 5
+};
+enum { GF_FORCING_NONE = 0, GF_FORCING_FIXED = 1, GF_FORCING_RING = 2 };
+
+/* Problem description: the keyword arguments of
+ *   TwoDNavierStokes.Problem  (src/twodnavierstokes.jl:71-89),
+ *   SingleLayerQG.Problem     (src/singlelayerqg.jl:80-102),
+ *   MultiLayerQG.Problem      (src/multilayerqg.jl:87-114),
+ * plus the stepper and dt handed to FourierFlows.Problem
+ * (src/twodnavierstokes.jl:99). Arrays are borrowed for the call only. */
+typedef struct gf_problem_desc {
+  int model;                 /* GF_MODEL_*                                         */
+  int dtype;                 /* GF_F64 | GF_F32  (keyword T)                        */
+  int nx, ny, nlayers;       /* powers of two, 16..2048; nlayers: MLQG 2..4, else 1 */
+  double Lx, Ly;
+  double aliased_fraction;   /* default 1/3                                         */
+  int stepper;               /* GF_ETDRK4 ...                                       */
+  double dt;
+  double nu; int nnu;        /* (hyper)viscosity nu |k|^(2 nnu)                     */
+  double mu; int nmu;        /* TwoDNS: mu |k|^(2 nmu); SLQG: linear drag; MLQG: bottom drag */
+  /* SingleLayerQG */
+  double beta;               /* SLQG, MLQG                                          */
+  double deformation_radius; /* SLQG: <= 0 or inf => barotropic                     */
+  double U;                  /* SLQG constant U (used when Uy == NULL)               */
+  const double* eta;         /* topographic PV, nx*ny (x fastest), or NULL          */
+  const double* Uy;          /* SLQG: U(y), ny values, or NULL.
+                                MLQG: U per layer, nlayers*ny (y fastest), or NULL  */
+  /* MultiLayerQG */
+  double f0;
+  const double* H;           /* nlayers                                             */
+  const double* b;           /* nlayers (Boussinesq buoyancy)                       */
+  double topo_grad_x, topo_grad_y;   /* topographic_pv_gradient                     */
+  int linear;                /* MLQG linear equations (not supported: GF_EUNSUPPORTED) */
+  /* forcing (calcF / calcFq) */
+  int forcing;               /* GF_FORCING_*                                        */
+  double kf, dkf, eps;       /* ring spectrum (examples/twodnavierstokes_stochasticforcing.jl:53-66) */
+  uint64_t seed;             /* counter-based generator key                         */
+  int instance;              /* generator stream id (ensemble member)               */
+} gf_problem_desc;
+
+typedef struct gf_ctx gf_ctx;
+
+/* FourierFlows.Problem(Equation(params, grid), stepper, dt, grid, vars, params)
+ * with the module's Params/Equation (src/twodnavierstokes.jl:97-99,149-154;
+ * src/singlelayerqg.jl:104-120,230-252; src/multilayerqg.jl:120-134,255-377). */
+int gf_ctx_create(gf_ctx** out, const gf_problem_desc* desc);
+void gf_ctx_destroy(gf_ctx* ctx);
+const char* gf_last_error(const gf_ctx* ctx);
+const char* gf_error_string(int code);
+
+/* prob.sol read/write (Julia layout, host). */
+int gf_set_state(gf_ctx* ctx, const void* sol);
+int gf_get_state(gf_ctx* ctx, void* sol);
+/* same on device pointers (AMDGPU.jl ROCArray / torch tensors in this process) */
+int gf_set_state_dev(gf_ctx* ctx, const void* sol_dev);
+int gf_get_state_dev(gf_ctx* ctx, void* sol_dev);
+
+/* set_zeta!/set_q! (src/twodnavierstokes.jl:352-360, src/singlelayerqg.jl:525-534,
+ * src/multilayerqg.jl:845-851): forward rfft of the physical field on the
+ * device, zero mean for TwoDNS and MLQG, then updatevars!'s in-place dealias. */
+int gf_set_physical(gf_ctx* ctx, const void* field);
+
+/* updatevars! (src/twodnavierstokes.jl:331-345, src/singlelayerqg.jl:502-518,
+ * src/multilayerqg.jl:819-831): dealias sol in place, write the physical fields
+ * (TwoDNS: zeta, u, v; SLQG/MLQG: q, psi, u, v -- each nx*ny per layer, layer
+ * slowest) into out, which holds nfields*nlayers*nx*ny values. */
+int gf_updatevars(gf_ctx* ctx, void* out);
+int gf_num_vars(const gf_ctx* ctx);
+
+/* calcN! as a unit hook (src/twodnavierstokes.jl:286-294, src/singlelayerqg.jl:443-451,
+ * src/multilayerqg.jl:651-663): N = calcN(sol) with sol dealiased in place (P1).
+ * Both arrays in Julia layout on the host; sol is read and written back. */
+int gf_calcN(gf_ctx* ctx, void* N, void* sol, double t);
+
+/* fixed forcing table Fh (a calcF! that copies a constant array, e.g.
+ * test/test_twodnavierstokes.jl:239-243); Julia layout, host. */
+int gf_set_forcing(gf_ctx* ctx, const void* Fh);
+
+/* [FF] stepforward!(prob, nsteps): nsteps of the configured stepper. */
+int gf_step(gf_ctx* ctx, int nsteps);
+
+/* [FF] prob.clock */
+int gf_clock(const gf_ctx* ctx, double* t, int64_t* step);
+int gf_set_clock(gf_ctx* ctx, double t, int64_t step);
+
+/* Wait for all work queued on the context's stream. */
+int gf_synchronize(gf_ctx* ctx);
+
+/* Per-kernel timing of gf_step with HIP events on the context's stream:
+ * gf_timing(ctx, 1) starts recording, gf_timing_read() returns, for kernel
+ * class k (0 = y-inverse, 1 = x-pass, 2 = y-forward + stage), the number of
+ * launches and the summed milliseconds since recording started. */
+int gf_timing(gf_ctx* ctx, int enable);
+int gf_timing_read(gf_ctx* ctx, int k, int64_t* launches, double* ms);
+
+/* Bytes of HBM each kernel class must move per launch in the fused
+ * algorithm (compulsory traffic of its inputs and outputs), for the given
+ * step phase: full = 1 for the last step of a gf_step call. */
+int gf_kernel_bytes(const gf_ctx* ctx, int k, int full, double* bytes);
+
+/* Library version string. */
+const char* gf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GF_H */
