@@ -164,6 +164,11 @@ def etd_coefficients(dt, L, ncirc=32, rcirc=1.0):
     """
     L = np.asarray(L)
     real = not np.iscomplexobj(L)
+    # the coefficients depend on L pointwise: evaluate once per distinct value
+    uniq, inv = np.unique(L.ravel(), return_inverse=True)
+    if uniq.size < L.size:
+        out = etd_coefficients(dt, uniq, ncirc, rcirc)
+        return tuple(c[inv].reshape(L.shape) for c in out)
     circ = rcirc * np.exp(2j * np.pi / ncirc * (np.arange(ncirc) + 0.5))
     zeta = np.zeros(L.shape, complex)
     alpha = np.zeros(L.shape, complex)

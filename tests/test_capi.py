@@ -1,0 +1,91 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+declared in include/gf.h, the descriptor layout matches, and invalid
+descriptors are rejected with the documented codes before any device work."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "gf.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gf_[A-Za-z_0-9]+)\s*\(", src)))
+
+
+def test_header_declares_the_boundary():
+    names = _declared()
+    for must in ("gf_ctx_create", "gf_calcN", "gf_step", "gf_set_state", "gf_get_state", "gf_updatevars",
+                 "gf_set_physical", "gf_clock", "gf_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(gf):
+    lib = ctypes.CDLL(gf._lib.LIBPATH)
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert set(gf._lib.EXPORTS) == set(_declared())
+
+
+def test_descriptor_layout_matches_header(gf, tmp_path):
+    # compile a probe against include/gf.h with the host C compiler and compare every
+    # field offset and the struct size with the ctypes mirror
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    D = gf._lib.ProblemDesc
+    fields = [f[0] for f in D._fields_]
+    body = "\n".join(f'  printf("%s %zu\\n", "{f}", offsetof(gf_problem_desc, {f}));' for f in fields)
+    src = tmp_path / "probe.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "gf.h"\nint main(void){\n' + body +
+                   '\n  printf("sizeof %zu\\n", sizeof(gf_problem_desc));\n  return 0;\n}\n')
+    exe = tmp_path / "probe"
+    subprocess.check_call([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    out = dict(line.split() for line in subprocess.check_output([str(exe)]).decode().splitlines())
+    for f in fields:
+        assert int(out[f]) == getattr(D, f).offset, f
+    assert int(out["sizeof"]) == ctypes.sizeof(D)
+
+
+def _desc(gf, **kw):
+    d = gf._lib.ProblemDesc()
+    d.model, d.dtype, d.nx, d.ny, d.nlayers = 0, 0, 64, 64, 1
+    d.Lx = d.Ly = 6.283185307179586
+    d.aliased_fraction, d.stepper, d.dt = 1 / 3, 0, 0.01
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize("bad,code", [
+    (dict(nx=48), -2), (dict(ny=4096), -2), (dict(dt=0.0), -1), (dict(model=7), -1),
+    (dict(stepper=9), -1), (dict(dtype=3), -1), (dict(model=2, nlayers=1), -2),
+    (dict(aliased_fraction=1.5), -1), (dict(Lx=-1.0), -1), (dict(nlayers=2), -1),
+    (dict(forcing=2, dkf=0.0), -1),
+])
+def test_invalid_descriptors_are_rejected(gf, bad, code):
+    lib = gf._lib.lib
+    ctx = ctypes.c_void_p()
+    rc = lib.gf_ctx_create(ctypes.byref(ctx), ctypes.byref(_desc(gf, **bad)))
+    assert rc == code
+    assert not ctx.value
+    assert lib.gf_last_error(None)
+
+
+def test_null_arguments(gf):
+    lib = gf._lib.lib
+    assert lib.gf_ctx_create(ctypes.byref(ctypes.c_void_p()), None) == -1
+    assert lib.gf_step(None, 1) == -1
+    assert lib.gf_error_string(-2) == b"unsupported configuration"
+
+
+def test_python_mirror_rejects_unknown_stepper(gf):
+    with pytest.raises(ValueError):
+        gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, stepper="AB3")
+    with pytest.raises(NotImplementedError):
+        gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, calcF=lambda *a: None)
