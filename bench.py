@@ -30,11 +30,11 @@ CFG = dict(nx=NX, Lx=2 * np.pi, nu=1e-21, nnu=4, mu=0.1, nmu=0, dt=0.005, kf=14.
 KERNELS = ("y_inverse_K1", "x_pass_K2", "y_forward_stage_K3")
 
 
-def gpu_run(gf, steps, warmup, T):
+def gpu_run(gf, steps, warmup, T, dt=CFG["dt"]):
     lib = gf._lib.lib
     prob = gf.TwoDNavierStokes.Problem(
         gf.GPU(), nx=CFG["nx"], Lx=CFG["Lx"], ν=CFG["nu"], nν=CFG["nnu"], μ=CFG["mu"], nμ=CFG["nmu"],
-        dt=CFG["dt"], stepper="ETDRK4", T=T,
+        dt=dt, stepper="ETDRK4", T=T,
         calcF=gf.RingForcing(CFG["kf"], CFG["dkf"], CFG["eps"], seed=CFG["seed"]))
     gf.TwoDNavierStokes.set_ζ(prob, np.zeros((NX, NX)))
     if warmup:
@@ -66,6 +66,71 @@ def gpu_run(gf, steps, warmup, T):
     return elapsed, kern, finite
 
 
+# batched SingleLayerQG beta-plane ensemble: 256 x 64^2, 192 x 128^2, 64 x 256^2 (FP64, FilteredRK4)
+ENSEMBLE = ((64, 256), (128, 192), (256, 64))
+
+
+def ensemble_members(gf):
+    members, q0 = [], []
+    i = 0
+    for n, count in ENSEMBLE:
+        for _ in range(count):
+            rng = np.random.default_rng(3_000_000 + i)
+            grid = gf.problem.Grid(n, n, 2 * np.pi, 2 * np.pi, 1 / 3, np.float64)
+            eta = gf.utils.gaussian_bump(grid, 0.1, 2 * np.pi / 10, rng.uniform(-np.pi, np.pi, 2))
+            members.append((gf.SingleLayerQG.Problem, dict(nx=n, Lx=2 * np.pi, β=10.0, μ=0.01, eta=eta, dt=0.01,
+                                                           stepper="FilteredRK4")))
+            q0.append(gf.peakedisotropicspectrum(grid, 8, 1.0, rng=rng))
+            i += 1
+    return members, q0
+
+
+def ensemble_run(gf, steps, seq_steps):
+    """Batched ensemble steps against the same members stepped one after another."""
+    lib = gf._lib.lib
+    members, q0 = ensemble_members(gf)
+    ens = gf.Ensemble(members)
+    for p, q in zip(ens.problems, q0):
+        gf.SingleLayerQG.set_q(p, q)
+    pts = sum(n * n * c for n, c in ENSEMBLE)
+    gf.stepforward(ens, 3)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    gf.stepforward(ens, steps)
+    ens.synchronize()
+    tb = time.perf_counter() - t0
+    lib.gf_timing(ens._ctx, 1)
+    gf.stepforward(ens, steps)
+    ens.synchronize()
+    kern = {}
+    for k, name in enumerate(KERNELS):
+        n, ms, b0 = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        lib.gf_timing_read(ens._ctx, k, ctypes.byref(n), ctypes.byref(ms))
+        lib.gf_kernel_bytes(ens._ctx, k, 0, ctypes.byref(b0))
+        avg_s = ms.value * 1e-3 / max(n.value, 1)
+        kern[name] = dict(launches=n.value, avg_us=avg_s * 1e6, bytes_per_launch=b0.value,
+                          achieved_GBps=b0.value / avg_s / 1e9)
+    lib.gf_timing(ens._ctx, 0)
+    for p in ens.problems:                      # one member at a time, single-instance kernels
+        gf.stepforward(p, 1)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    for p in ens.problems:
+        gf.stepforward(p, seq_steps)
+    ens.synchronize()
+    ts = time.perf_counter() - t0
+    finite = all(bool(np.isfinite(p.sol).all()) for p in ens.problems[::37])
+    batched = pts * steps / tb
+    sequential = pts * seq_steps / ts
+    return {"workload": "SingleLayerQG beta-plane ensemble 256x64^2 + 192x128^2 + 64x256^2, beta=10 mu=0.01 "
+                        "FilteredRK4 dt=0.01, Gaussian-bump topography, FP64",
+            "members": sum(c for _, c in ENSEMBLE), "grid_points": pts,
+            "batched": {"value": batched, "unit": "grid-pt-steps/s", "steps": steps, "ms_per_step": tb / steps * 1e3},
+            "sequential": {"value": sequential, "unit": "grid-pt-steps/s", "steps": seq_steps,
+                           "ms_per_step": ts / seq_steps * 1e3},
+            "batched_over_sequential": batched / sequential, "kernels": kern, "state_finite": finite}
+
+
 def cpu_baseline(nsteps=3):
     """The oracle (NumPy + SciPy pocketfft with one worker per core) on the same workload."""
     from oracle import fourierflows as off
@@ -91,6 +156,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fp32", action="store_true", help="also time the Float32 variant")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--no-stable", action="store_true", help="skip the dt = 0.001 run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -130,6 +198,13 @@ def main():
                        "frac_of_peak": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "state_finite": finite,
     }
+    if not args.no_stable:
+        # at dt = 0.005 the forced 2048^2 flow exceeds the advective CFL limit within a few hundred
+        # steps; the same step count at dt = 0.001 stays finite and shows the timing is unchanged
+        es, _, fs = gpu_run(gf, steps, warmup, np.float64, dt=0.001)
+        out["stable_dt_0.001"] = {"value": NX * NX * steps / es, "ms_per_step": es / steps * 1e3, "state_finite": fs}
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_run(gf, max(1, args.ensemble_steps), 3)
     if args.fp32:
         e32, k32, f32 = gpu_run(gf, steps, warmup, np.float32)
         out["fp32"] = {"value": NX * NX * steps / e32, "ms_per_step": e32 / steps * 1e3, "state_finite": f32}

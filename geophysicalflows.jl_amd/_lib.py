@@ -21,7 +21,8 @@ EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string"
            "gf_set_state", "gf_get_state", "gf_set_state_dev", "gf_get_state_dev",
            "gf_set_physical", "gf_updatevars", "gf_num_vars", "gf_calcN", "gf_set_forcing",
            "gf_step", "gf_clock", "gf_set_clock", "gf_synchronize", "gf_timing",
-           "gf_timing_read", "gf_kernel_bytes", "gf_version")
+           "gf_timing_read", "gf_kernel_bytes", "gf_version", "gf_ctx_create_batch",
+           "gf_batch_size", "gf_ctx_instance")
 
 
 class ProblemDesc(ctypes.Structure):
@@ -80,6 +81,9 @@ def _load():
     lib.gf_timing_read.argtypes = [vp, ip, i64p, dp]
     lib.gf_kernel_bytes.argtypes = [vp, ip, ip, dp]
     lib.gf_version.restype = ctypes.c_char_p
+    lib.gf_ctx_create_batch.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ProblemDesc), ip]
+    lib.gf_batch_size.argtypes = [vp]
+    lib.gf_ctx_instance.argtypes = [vp, ip, ctypes.POINTER(vp)]
     for name in EXPORTS:
         getattr(lib, name)
     return lib

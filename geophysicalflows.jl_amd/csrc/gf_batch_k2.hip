@@ -1,0 +1,3 @@
+// Batched ensemble kernel class K2 (FP64).
+#define GF_BATCH_K 2
+#include "gf_batch_impl.hpp"

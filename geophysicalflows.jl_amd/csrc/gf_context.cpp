@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -96,6 +97,48 @@ struct DevBuf {
   template <typename X> X* as() const { return (X*)p; }
 };
 
+// HIP events around each launch of kernel classes 0..2 (K1, K2, K3), summed on read.
+struct KernelTimer {
+  struct Ev { int k; hipEvent_t a, b; };
+  bool on = false;
+  std::vector<Ev> evs;
+  int64_t launches[3] = {0, 0, 0};
+  double ms[3] = {0, 0, 0};
+  ~KernelTimer() { drop(); }
+  void drop() {
+    for (auto& e : evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    evs.clear();
+  }
+  void rec(int k, bool start, hipStream_t s) {
+    if (!on) return;
+    if (start) {
+      Ev e; e.k = k;
+      HIPCHECK(hipEventCreate(&e.a)); HIPCHECK(hipEventCreate(&e.b));
+      HIPCHECK(hipEventRecord(e.a, s));
+      evs.push_back(e);
+    } else {
+      HIPCHECK(hipEventRecord(evs.back().b, s));
+    }
+  }
+  void reset(int enable, hipStream_t s) {
+    HIPCHECK(hipStreamSynchronize(s));
+    drop();
+    for (int k = 0; k < 3; ++k) { launches[k] = 0; ms[k] = 0; }
+    on = enable != 0;
+  }
+  void read(int k, int64_t* n, double* t, hipStream_t s) {
+    if (k < 0 || k > 2) throw GfError(GF_EINVAL, "timing class must be 0, 1 or 2");
+    HIPCHECK(hipStreamSynchronize(s));
+    for (auto& e : evs) {
+      float x = 0;
+      HIPCHECK(hipEventElapsedTime(&x, e.a, e.b));
+      launches[e.k] += 1; ms[e.k] += x;
+    }
+    drop();
+    *n = launches[k]; *t = ms[k];
+  }
+};
+
 struct CtxBase {
   virtual ~CtxBase() = default;
   virtual void set_state(const void* src, hipMemcpyKind kind) = 0;
@@ -110,6 +153,10 @@ struct CtxBase {
   virtual void timing_read(int k, int64_t* n, double* ms) = 0;
   virtual double kernel_bytes(int k, int full) const = 0;
   virtual void sync() = 0;
+  // ensembles (a plain context is an ensemble of one)
+  virtual int ninstances() const { return 1; }
+  virtual CtxBase* instance(int i) { return i == 0 ? this : nullptr; }
+  virtual void set_clock(double tt, int64_t s) { t = tt; stepno = s; }
   double t = 0;
   int64_t stepno = 0;
 };
@@ -131,15 +178,12 @@ struct Ctx final : CtxBase {
   // device buffers
   DevBuf sol, s1, s2, n1, n23, racc, nout, mixed, flux, twx, twy, phys, jl, cinp;
   DevBuf coef[6], filter, eta, etah, etax, etay, Uy, Qy1, sinv, Fh, sqrt_spec;
-  // timing
-  bool timing_on = false;
-  struct Ev { int k; hipEvent_t a, b; };
-  std::vector<Ev> evs;
-  int64_t tl[3] = {0, 0, 0};
-  double tms[3] = {0, 0, 0};
+  KernelTimer tm;
 
-  explicit Ctx(const gf_problem_desc& desc) : d(desc) {
-    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  bool own_stream = true;
+  explicit Ctx(const gf_problem_desc& desc, hipStream_t shared = nullptr) : d(desc) {
+    if (shared) { stream = shared; own_stream = false; }
+    else HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     nl = d.model == GF_MODEL_MLQG ? d.nlayers : 1;
     const int nx = d.nx, ny = d.ny, nkr = nx / 2 + 1;
     g.nx = nx; g.ny = ny; g.nkr = nkr; g.nkr2 = (nkr + 1) & ~1; g.nlayers = nl;
@@ -167,11 +211,11 @@ struct Ctx final : CtxBase {
     build_model();
     build_stepper();
     build_forcing();
+    if (d.forcing == GF_FORCING_FIXED) Fh.alloc(sizeof(C) * fsz * nl);   // zero until gf_set_forcing
     HIPCHECK(hipStreamSynchronize(stream));
   }
   ~Ctx() override {
-    for (auto& e : evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
-    if (stream) { (void)hipStreamSynchronize(stream); (void)hipStreamDestroy(stream); }
+    if (stream) { (void)hipStreamSynchronize(stream); if (own_stream) (void)hipStreamDestroy(stream); }
   }
 
   // ---------------- setup helpers ----------------
@@ -486,30 +530,19 @@ struct Ctx final : CtxBase {
   }
 
   // ---------------- hot path ----------------
-  void rec(int k, bool start) {
-    if (!timing_on) return;
-    if (start) {
-      Ev e; e.k = k;
-      HIPCHECK(hipEventCreate(&e.a)); HIPCHECK(hipEventCreate(&e.b));
-      HIPCHECK(hipEventRecord(e.a, stream));
-      evs.push_back(e);
-    } else {
-      HIPCHECK(hipEventRecord(evs.back().b, stream));
-    }
-  }
+  void rec(int k, bool start) { tm.rec(k, start, stream); }
 
-  void calcN_launch(const C* in, int epi, bool full) {
-    K1Args<T> a{};
+  // argument blocks of the three kernels for one calcN! evaluation with stage epilogue epi
+  void make_args(const C* in, int epi, bool full, K1Args<T>& a, K2Args<T>& b, K3Args<T>& c) const {
+    a = K1Args<T>{};
     a.g = g; a.model = d.model; a.fieldset = FS_CALCN; a.sol = in; a.out = mixed.as<C>(); a.fstride = fsz;
     a.ncols = g.kmax; a.eb = eb; a.ell2inv = (T)ell2inv; a.sinv = sinv.as<T>(); a.tw = twy.as<C>();
-    rec(0, true); launch_k1<T>(d.ny, a, stream); rec(0, false);
-    K2Args<T> b{};
+    b = K2Args<T>{};
     b.g = g; b.mode = advect_full ? K2_ADVECT_FULL : K2_ADVECT; b.in = mixed.as<C>(); b.fstride = fsz;
     b.kin = g.kmax; b.eta = (d.model == GF_MODEL_SLQG && !advect_full && has_eta) ? eta.as<T>() : nullptr;
     b.Uy = Uy.as<T>(); b.Qy1 = Qy1.as<T>(); b.etax = etax.as<T>(); b.etay = etay.as<T>(); b.has_bg = advect_full;
     b.out = flux.as<C>(); b.ostride = fsz; b.kout = full ? g.nkr : g.kmax; b.tw = twx.as<C>();
-    rec(1, true); launch_k2<T>(d.nx, b, stream); rec(1, false);
-    K3Args<T> c{};
+    c = K3Args<T>{};
     c.g = g; c.model = d.model; c.in = flux.as<C>(); c.fstride = fsz; c.ncols = full ? g.nkr : g.kmax;
     c.full = full; c.epi = epi; c.apply_kr = !advect_full;
     c.etah = etah.as<C>(); c.Ueta = (T)d.U;
@@ -523,26 +556,40 @@ struct Ctx final : CtxBase {
     c.c0 = coef[0].p; c.c1 = coef[1].p; c.c2 = coef[2].p; c.c3 = coef[3].p; c.c4 = coef[4].p; c.c5 = coef[5].p;
     c.ccplx = ccplx; c.cl_layered = 0; c.dt = (T)d.dt; c.filter = filtered() ? filter.as<T>() : nullptr;
     c.tw = twy.as<C>();
+  }
+
+  void calcN_launch(const C* in, int epi, bool full) {
+    K1Args<T> a; K2Args<T> b; K3Args<T> c;
+    make_args(in, epi, full, a, b, c);
+    rec(0, true); launch_k1<T>(d.ny, a, stream); rec(0, false);
+    rec(1, true); launch_k2<T>(d.nx, b, stream); rec(1, false);
     rec(2, true); launch_k3<T>(d.ny, c, stream); rec(2, false);
     HIPCHECK(hipGetLastError());
   }
 
+  // the stages of one step: (calcN input, epilogue) pairs
+  int stages(const C* ins[4], int epis[4]) const {
+    if (etd()) {
+      ins[0] = sol.as<C>(); ins[1] = s1.as<C>(); ins[2] = s2.as<C>(); ins[3] = s2.as<C>();
+      epis[0] = E_ETD1; epis[1] = E_ETD2; epis[2] = E_ETD3; epis[3] = E_ETD4;
+      return 4;
+    }
+    if (d.stepper == GF_RK4 || d.stepper == GF_FILTERED_RK4) {
+      ins[0] = sol.as<C>(); ins[1] = s1.as<C>(); ins[2] = s1.as<C>(); ins[3] = s1.as<C>();
+      epis[0] = E_RK1; epis[1] = E_RK2; epis[2] = E_RK3; epis[3] = E_RK4;
+      return 4;
+    }
+    ins[0] = sol.as<C>(); epis[0] = E_FE;
+    return 1;
+  }
+
   void step(int n) override {
+    const C* ins[4];
+    int epis[4];
+    const int ns = stages(ins, epis);
     for (int s = 0; s < n; ++s) {
       const bool full = s == n - 1;
-      if (etd()) {
-        calcN_launch(sol.as<C>(), E_ETD1, full);
-        calcN_launch(s1.as<C>(), E_ETD2, full);
-        calcN_launch(s2.as<C>(), E_ETD3, full);
-        calcN_launch(s2.as<C>(), E_ETD4, full);
-      } else if (d.stepper == GF_RK4 || d.stepper == GF_FILTERED_RK4) {
-        calcN_launch(sol.as<C>(), E_RK1, full);
-        calcN_launch(s1.as<C>(), E_RK2, full);
-        calcN_launch(s1.as<C>(), E_RK3, full);
-        calcN_launch(s1.as<C>(), E_RK4, full);
-      } else {
-        calcN_launch(sol.as<C>(), E_FE, full);
-      }
+      for (int k = 0; k < ns; ++k) calcN_launch(ins[k], epis[k], full);
       t += d.dt;
       stepno += 1;
     }
@@ -607,30 +654,12 @@ struct Ctx final : CtxBase {
   void set_forcing(const void* F) override {
     if (d.forcing != GF_FORCING_FIXED) throw GfError(GF_EINVAL, "gf_set_forcing: problem was not created with GF_FORCING_FIXED");
     const size_t bytes = sizeof(C) * (size_t)g.nkr * d.ny * nl;
-    if (!Fh.p) Fh.alloc(sizeof(C) * fsz * nl);
     HIPCHECK(hipMemcpyAsync(jl.p, F, bytes, hipMemcpyHostToDevice, stream));
     launch_from_julia<T>(g, jl.as<C>(), Fh.as<C>(), nl, stream);
     HIPCHECK(hipStreamSynchronize(stream));
   }
-  void timing(int enable) override {
-    HIPCHECK(hipStreamSynchronize(stream));
-    for (auto& e : evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
-    evs.clear();
-    for (int k = 0; k < 3; ++k) { tl[k] = 0; tms[k] = 0; }
-    timing_on = enable != 0;
-  }
-  void timing_read(int k, int64_t* n, double* ms) override {
-    HIPCHECK(hipStreamSynchronize(stream));
-    for (auto& e : evs) {
-      float x = 0;
-      HIPCHECK(hipEventElapsedTime(&x, e.a, e.b));
-      tl[e.k] += 1; tms[e.k] += x;
-      (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b);
-    }
-    evs.clear();
-    if (k < 0 || k > 2) throw GfError(GF_EINVAL, "timing class must be 0, 1 or 2");
-    *n = tl[k]; *ms = tms[k];
-  }
+  void timing(int enable) override { tm.reset(enable, stream); }
+  void timing_read(int k, int64_t* n, double* ms) override { tm.read(k, n, ms, stream); }
   // compulsory HBM bytes per launch of kernel class k
   double kernel_bytes(int k, int full) const override {
     const double cs = sizeof(C), rs = sizeof(T);
@@ -674,6 +703,124 @@ struct Ctx final : CtxBase {
   void sync() override { HIPCHECK(hipStreamSynchronize(stream)); }
 };
 
+// Ensemble of independent FP64 instances advanced together: per stage, one launch per
+// kernel class covers every member (flat (instance, block) work tables, largest
+// members first).  Each member is a full Ctx on the shared stream, so the
+// single-instance entry points (state, physical fields, updatevars!, calcN!, and a
+// member-only step) work on it unchanged.
+struct BatchCtx final : CtxBase {
+  using T = double;
+  using C = cplx<T>;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<Ctx<T>>> inst;
+  int nst = 0;             // stages per step
+  DevBuf items[4][2];      // BatchItem per member for (stage, last step of the call)
+  DevBuf work[3][2];       // BatchWork table per kernel class and last-step flag
+  int nwork[3][2] = {};
+  KernelTimer tm;
+
+  BatchCtx(const gf_problem_desc* ds, int n) {
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    try {
+      inst.reserve(n);
+      for (int i = 0; i < n; ++i) inst.emplace_back(new Ctx<T>(ds[i], stream));
+      build_tables();
+    } catch (...) {
+      inst.clear();
+      (void)hipStreamDestroy(stream);
+      stream = nullptr;
+      throw;
+    }
+  }
+  ~BatchCtx() override {
+    if (stream) (void)hipStreamSynchronize(stream);
+    inst.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void build_tables() {
+    const int n = (int)inst.size();
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return inst[a]->d.nx > inst[b]->d.nx; });
+    const C* ins[4];
+    int epis[4];
+    nst = inst[0]->stages(ins, epis);
+    std::vector<BatchItem<T>> h(n);
+    for (int full = 0; full < 2; ++full) {
+      for (int k = 0; k < nst; ++k) {
+        for (int i = 0; i < n; ++i) {
+          Ctx<T>& m = *inst[i];
+          m.stages(ins, epis);
+          h[i] = BatchItem<T>{};
+          h[i].n = m.d.nx;
+          m.make_args(ins[k], epis[k], full != 0, h[i].k1, h[i].k2, h[i].k3);
+        }
+        items[k][full].alloc(sizeof(BatchItem<T>) * n);
+        HIPCHECK(hipMemcpy(items[k][full].p, h.data(), sizeof(BatchItem<T>) * n, hipMemcpyHostToDevice));
+      }
+      for (int kc = 0; kc < 3; ++kc) {
+        std::vector<BatchWork> w;
+        for (int i : ord) {
+          const Ctx<T>& m = *inst[i];
+          const int count = kc == 1 ? m.d.ny : (kc == 2 && full ? m.g.nkr : m.g.kmax);
+          const int nb = batch_blocks(kc + 1, m.d.nx, count);
+          for (int b = 0; b < nb; ++b) w.push_back(BatchWork{i, b});
+        }
+        nwork[kc][full] = (int)w.size();
+        work[kc][full].alloc(sizeof(BatchWork) * w.size());
+        HIPCHECK(hipMemcpy(work[kc][full].p, w.data(), sizeof(BatchWork) * w.size(), hipMemcpyHostToDevice));
+      }
+    }
+  }
+
+  void step(int n) override {
+    for (int s = 0; s < n; ++s) {
+      const int full = s == n - 1;
+      for (int k = 0; k < nst; ++k) {
+        const BatchItem<T>* it = items[k][full].as<BatchItem<T>>();
+        tm.rec(0, true, stream);
+        launch_batch<T, 1>(it, work[0][full].as<BatchWork>(), nwork[0][full], stream);
+        tm.rec(0, false, stream);
+        tm.rec(1, true, stream);
+        launch_batch<T, 2>(it, work[1][full].as<BatchWork>(), nwork[1][full], stream);
+        tm.rec(1, false, stream);
+        tm.rec(2, true, stream);
+        launch_batch<T, 3>(it, work[2][full].as<BatchWork>(), nwork[2][full], stream);
+        tm.rec(2, false, stream);
+      }
+      HIPCHECK(hipGetLastError());
+      for (auto& m : inst) { m->t += m->d.dt; m->stepno += 1; }
+    }
+    t = inst[0]->t;
+    stepno = inst[0]->stepno;
+  }
+  [[noreturn]] static void member_only(const char* what) {
+    throw GfError(GF_EINVAL, std::string(what) + ": this is an ensemble handle; address one member through gf_ctx_instance");
+  }
+  void set_state(const void*, hipMemcpyKind) override { member_only("set_state"); }
+  void get_state(void*, hipMemcpyKind) override { member_only("get_state"); }
+  void set_physical(const void*) override { member_only("set_physical"); }
+  void updatevars(void*) override { member_only("updatevars"); }
+  void calcN(void*, void*, double) override { member_only("calcN"); }
+  void set_forcing(const void*) override { member_only("set_forcing"); }
+  int nvars() const override { return inst[0]->nvars(); }
+  void timing(int enable) override { tm.reset(enable, stream); }
+  void timing_read(int k, int64_t* n, double* ms) override { tm.read(k, n, ms, stream); }
+  double kernel_bytes(int k, int full) const override {
+    double b = 0;
+    for (auto& m : inst) b += m->kernel_bytes(k, full);
+    return b;
+  }
+  void sync() override { HIPCHECK(hipStreamSynchronize(stream)); }
+  int ninstances() const override { return (int)inst.size(); }
+  CtxBase* instance(int i) override { return (i >= 0 && i < (int)inst.size()) ? inst[i].get() : nullptr; }
+  void set_clock(double tt, int64_t s) override {
+    t = tt; stepno = s;
+    for (auto& m : inst) { m->t = tt; m->stepno = s; }
+  }
+};
+
 int validate(const gf_problem_desc* d, std::string& msg) {
   if (!d) { msg = "null descriptor"; return GF_EINVAL; }
   if (d->model < 0 || d->model > 2) { msg = "model must be GF_MODEL_TWODNS, GF_MODEL_SLQG or GF_MODEL_MLQG"; return GF_EINVAL; }
@@ -701,10 +848,32 @@ int validate(const gf_problem_desc* d, std::string& msg) {
   return GF_OK;
 }
 
+int validate_batch(const gf_problem_desc* ds, int n, std::string& msg) {
+  if (!ds || n < 1) { msg = "an ensemble needs at least one descriptor"; return GF_EINVAL; }
+  for (int i = 0; i < n; ++i) {
+    const gf_problem_desc& d = ds[i];
+    const std::string who = "ensemble member " + std::to_string(i) + ": ";
+    if (d.model == GF_MODEL_MLQG) { msg = who + "MultiLayerQG members are not supported in ensembles"; return GF_EUNSUPPORTED; }
+    std::string m;
+    const int rc = validate(&d, m);
+    if (rc != GF_OK) { msg = who + m; return rc; }
+    if (d.dtype != GF_F64) { msg = who + "ensembles are FP64 only"; return GF_EUNSUPPORTED; }
+    if (d.nx != d.ny || !batch_size_supported(d.nx)) {
+      msg = who + "members must be square with n in {32, 64, 128, 256, 512}"; return GF_EUNSUPPORTED;
+    }
+    if (d.forcing == GF_FORCING_RING) { msg = who + "stochastic ring forcing is not supported in ensembles"; return GF_EUNSUPPORTED; }
+    if (d.stepper != ds[0].stepper) { msg = who + "all members must use the same stepper"; return GF_EINVAL; }
+  }
+  return GF_OK;
+}
+
 }  // namespace
 
 struct gf_ctx {
-  std::unique_ptr<CtxBase> impl;
+  std::unique_ptr<CtxBase> owned;
+  CtxBase* impl = nullptr;
+  gf_ctx* parent = nullptr;                      // set on the member views of an ensemble
+  std::vector<std::unique_ptr<gf_ctx>> members;  // views, created on first request
   std::string err;
 };
 
@@ -752,18 +921,65 @@ int gf_ctx_create(gf_ctx** out, const gf_problem_desc* desc) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_err = "no HIP device"; return GF_ENODEVICE; }
   auto* c = new gf_ctx();
   try {
-    if (desc->dtype == GF_F64) c->impl.reset(new Ctx<double>(*desc));
-    else c->impl.reset(new Ctx<float>(*desc));
+    if (desc->dtype == GF_F64) c->owned.reset(new Ctx<double>(*desc));
+    else c->owned.reset(new Ctx<float>(*desc));
   } catch (const GfError& e) {
     g_create_err = e.what(); delete c; return e.code;
   } catch (const std::exception& e) {
     g_create_err = e.what(); delete c; return GF_EHIP;
   }
+  c->impl = c->owned.get();
   *out = c;
   return GF_OK;
 }
 
-void gf_ctx_destroy(gf_ctx* ctx) { delete ctx; }
+int gf_ctx_create_batch(gf_ctx** out, const gf_problem_desc* descs, int n) {
+  if (!out) return GF_EINVAL;
+  *out = nullptr;
+  std::string msg;
+  int rc = validate_batch(descs, n, msg);
+  if (rc != GF_OK) { g_create_err = msg; return rc; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_err = "no HIP device"; return GF_ENODEVICE; }
+  auto* c = new gf_ctx();
+  try {
+    c->owned.reset(new BatchCtx(descs, n));
+  } catch (const GfError& e) {
+    g_create_err = e.what(); delete c; return e.code;
+  } catch (const std::exception& e) {
+    g_create_err = e.what(); delete c; return GF_EHIP;
+  }
+  c->impl = c->owned.get();
+  c->members.resize(n);
+  *out = c;
+  return GF_OK;
+}
+
+int gf_batch_size(const gf_ctx* c) { return c ? c->impl->ninstances() : GF_EINVAL; }
+
+int gf_ctx_instance(gf_ctx* c, int i, gf_ctx** member) {
+  if (!c || !member) return GF_EINVAL;
+  *member = nullptr;
+  if (c->parent || i < 0 || i >= c->impl->ninstances()) {
+    c->err = "gf_ctx_instance: index out of range or handle is not an ensemble";
+    return GF_EINVAL;
+  }
+  if (c->members.empty()) {   // a plain context is its own only member
+    *member = c;
+    return GF_OK;
+  }
+  if (!c->members[i]) {
+    c->members[i].reset(new gf_ctx());
+    c->members[i]->impl = c->impl->instance(i);
+    c->members[i]->parent = c;
+  }
+  *member = c->members[i].get();
+  return GF_OK;
+}
+
+void gf_ctx_destroy(gf_ctx* ctx) {
+  if (ctx && !ctx->parent) delete ctx;   // member views belong to their ensemble
+}
 
 const char* gf_last_error(const gf_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
@@ -812,7 +1028,7 @@ int gf_clock(const gf_ctx* c, double* t, int64_t* step) {
 }
 int gf_set_clock(gf_ctx* c, double t, int64_t step) {
   if (!c) return GF_EINVAL;
-  c->impl->t = t; c->impl->stepno = step;
+  c->impl->set_clock(t, step);
   return GF_OK;
 }
 int gf_synchronize(gf_ctx* c) { return guarded(c, [&] { c->impl->sync(); }); }

@@ -8,7 +8,7 @@
 // radix); between passes the points are exchanged through LDS.  Several
 // independent transforms per thread (C) share the twiddles of a pass; their
 // LDS exchanges go through one buffer, one transform at a time, so a
-// workgroup needs only N complex values of LDS per thread group.
+// workgroup needs one padded N-point buffer per thread group.
 //
 // The forward transform uses exp(-2 pi i jk/N) (FFTW/numpy sign, unnormalised).
 // Inverse transforms are computed as conj(F(conj(x))) by the callers.
@@ -100,12 +100,58 @@ template <int R, typename T> __device__ __forceinline__ void dft(cplx<T>* v) {
   else if constexpr (R == 16) dft16(v);
 }
 
-// XOR swizzle of LDS slots: spreads the strided Stockham writes over the banks
-// without padding (the set of slots stays 0..N-1).
-template <typename T> __device__ __forceinline__ int swz(int i) {
-  if constexpr (sizeof(T) == 8) return i ^ ((i >> 3) & 7);   // 16-byte elements
-  else return i ^ ((i >> 4) & 15);                            // 8-byte elements
+// Twiddle tables.  Pass p of the Stockham plan (stride NS_p, radix R_p) needs
+// w^r with w = exp(-2 pi i k/(NS_p R_p)), k = j mod NS_p, r = 1..R_p-1.  The
+// workgroup keeps, for every pass with NS_p > 1, the NS_p values w(k) contiguous
+// in LDS (sum over passes of NS_p < N entries) so that the lanes of a wave read
+// consecutive slots; the powers w^r are formed by squaring and multiplication
+// (at most three roundings deep for R <= 16).
+template <int N, int P>
+struct TwPlan {
+  using PL = Plan<N, P>;
+  static constexpr int ns(int p) { return p == 0 ? 1 : ns(p - 1) * PL::radix(p - 1); }
+  // offset of pass p's table (passes 1..NPASS-1)
+  static constexpr int off(int p) { return p <= 1 ? 0 : off(p - 1) + ns(p - 1); }
+  static constexpr int size() { return PL::NPASS <= 1 ? 1 : off(PL::NPASS - 1) + ns(PL::NPASS - 1); }
+};
+
+// Fill the per-pass tables from the full table twg[i] = exp(-2 pi i i/N) in global memory.
+template <typename T, int N, int P, int PASS = 1>
+__device__ __forceinline__ void load_twiddles(cplx<T>* __restrict__ dst, const cplx<T>* __restrict__ twg) {
+  using TP = TwPlan<N, P>;
+  using PL = Plan<N, P>;
+  if constexpr (PASS < PL::NPASS) {
+    constexpr int ns = TP::ns(PASS), step = N / (ns * PL::radix(PASS)), o = TP::off(PASS);
+    for (int k = threadIdx.x; k < ns; k += blockDim.x) dst[o + k] = twg[k * step];
+    load_twiddles<T, N, P, PASS + 1>(dst, twg);
+  }
 }
+
+// w^1..w^(R-1) from w^1
+template <typename T, int R>
+__device__ __forceinline__ void powers(cplx<T> (&w)[R]) {
+  if constexpr (R >= 3) w[2] = cmul(w[1], w[1]);
+  if constexpr (R >= 4) w[3] = cmul(w[2], w[1]);
+  if constexpr (R >= 5) w[4] = cmul(w[2], w[2]);
+  if constexpr (R >= 6) w[5] = cmul(w[4], w[1]);
+  if constexpr (R >= 7) w[6] = cmul(w[3], w[3]);
+  if constexpr (R >= 8) w[7] = cmul(w[4], w[3]);
+  if constexpr (R >= 9) w[8] = cmul(w[4], w[4]);
+  if constexpr (R >= 10) w[9] = cmul(w[8], w[1]);
+  if constexpr (R >= 11) w[10] = cmul(w[5], w[5]);
+  if constexpr (R >= 12) w[11] = cmul(w[8], w[3]);
+  if constexpr (R >= 13) w[12] = cmul(w[6], w[6]);
+  if constexpr (R >= 14) w[13] = cmul(w[8], w[5]);
+  if constexpr (R >= 15) w[14] = cmul(w[7], w[7]);
+  if constexpr (R >= 16) w[15] = cmul(w[8], w[7]);
+}
+
+// LDS slot of point i: one pad slot after every 8 points.  Stockham writes at
+// stride R or NS (multiples of 8 or the first pass's 8j + r) and the unit-stride
+// reads then fall on distinct banks, and every slot of a thread's accesses is
+// its base slot plus a compile-time offset (one address register per pass).
+__device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
+template <int N> struct LdsN { static constexpr int value = N + N / 8; };
 
 // One Stockham pass (index PASS, stride NS) on C transforms, then recurse.
 template <typename T, int N, int P, int C, int PASS, int NS>
@@ -121,13 +167,9 @@ __device__ __forceinline__ void stockham(cplx<T> (&v)[C][P], cplx<T>* __restrict
     const int k = j % NS;
     cplx<T> w[R];
     if constexpr (NS > 1) {
-      constexpr int STEP = N / (NS * R);
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        // 32-bit byte offset from a uniform base: SGPR-base + VGPR-offset loads
-        const uint32_t off = (uint32_t)(r * k * STEP) * (uint32_t)sizeof(cplx<T>);
-        w[r] = *reinterpret_cast<const cplx<T>*>(reinterpret_cast<const char*>(tw) + off);
-      }
+      constexpr int TOFF = TwPlan<N, P>::off(PASS);
+      w[1] = tw[TOFF + k];
+      powers<T, R>(w);
     }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -150,11 +192,11 @@ __device__ __forceinline__ void stockham(cplx<T> (&v)[C][P], cplx<T>* __restrict
         const int j = t + b * TPT;
         const int base = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[swz<T>(base + r * NS)] = v[c][b + r * M];
+        for (int r = 0; r < R; ++r) lds[pad(base + r * NS)] = v[c][b + r * M];
       }
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < P; ++q) v[c][q] = lds[swz<T>(t + TPT * q)];
+      for (int q = 0; q < P; ++q) v[c][q] = lds[pad(t + TPT * q)];
       __syncthreads();
     }
     stockham<T, N, P, C, PASS + 1, NS * R>(v, lds, t, tw);
@@ -162,7 +204,9 @@ __device__ __forceinline__ void stockham(cplx<T> (&v)[C][P], cplx<T>* __restrict
 }
 
 // Forward FFT of C transforms held as v[c][q] = x_c[t + TPT*q].
-// lds: N complex slots private to this thread group; tw[i] = exp(-2 pi i i/N).
+// lds: LdsN<N> complex slots private to this thread group; tw: the per-pass
+// twiddle tables in LDS (TwPlan<N, P>::size() slots, filled by load_twiddles and
+// followed by a barrier).
 template <typename T, int N, int P, int C>
 __device__ __forceinline__ void fft_forward(cplx<T> (&v)[C][P], cplx<T>* __restrict__ lds, int t,
                                             const cplx<T>* __restrict__ tw) {

@@ -125,3 +125,30 @@ template <typename T> void launch_mask(const Geo<T>& g, cplx<T>* sol, int nl, hi
 bool size_supported(int n);
 
 }  // namespace gf
+
+namespace gf {
+
+// One instance of a batched ensemble for one stage: the three kernels' argument
+// blocks (with the stage's input buffer, epilogue and full flag filled in).
+template <typename T>
+struct BatchItem {
+  int n;                    // grid size (nx == ny)
+  K1Args<T> k1;
+  K2Args<T> k2;
+  K3Args<T> k3;
+};
+
+// One workgroup's share of a batched launch: block blk of instance item.
+struct BatchWork {
+  int item;
+  int blk;
+};
+
+// Launch kernel class K (1, 2, 3) of one stage for every instance: workgroup b runs
+// block work[b].blk of instance work[b].item (FP64 only).
+template <typename T, int K> void launch_batch(const BatchItem<T>* items, const BatchWork* work, int nwork, hipStream_t s);
+// workgroups of kernel class k for an instance of size n: count = columns (k = 1, 3) or ny (k = 2)
+int batch_blocks(int k, int n, int count);
+bool batch_size_supported(int n);
+
+}  // namespace gf

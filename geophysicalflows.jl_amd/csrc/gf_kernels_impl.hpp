@@ -36,6 +36,18 @@ template <typename T> __device__ __forceinline__ T krwave(const Geo<T>& g, int k
 __device__ __forceinline__ long boff(int kr, int j, int ny) {
   return ((long)(kr >> 1) * ny + j) * 2 + (kr & 1);
 }
+// 32-bit element offset in the column-blocked layout (fields stay below 4 GiB)
+__device__ __forceinline__ uint32_t boff32(int kr, int j, int ny) {
+  return ((uint32_t)(kr >> 1) * (uint32_t)ny + (uint32_t)j) * 2u + (uint32_t)(kr & 1);
+}
+// Element i of an array whose base pointer is uniform across the workgroup, addressed
+// through a 32-bit byte offset (scalar base + vector offset memory instructions).
+template <typename X> __device__ __forceinline__ const X& at(const X* base, uint32_t i) {
+  return *reinterpret_cast<const X*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(X));
+}
+template <typename X> __device__ __forceinline__ X& at(X* base, uint32_t i) {
+  return *reinterpret_cast<X*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(X));
+}
 template <typename T> __device__ __forceinline__ cplx<T> czero() { return mk((T)0, (T)0); }
 
 // Philox-4x32-10 (Salmon et al. 2011); key (k0,k1), counter (c0..c3).
@@ -68,9 +80,9 @@ __device__ __noinline__ cplx<T> complex_normal(uint64_t seed, int64_t step, uint
 
 template <typename T> struct Coef {
   const void* p; int cplx_;
-  __device__ __forceinline__ cplx<T> mul(long i, cplx<T> x) const {
-    if (cplx_) return cmul(((const cplx<T>*)p)[i], x);
-    return rmul(((const T*)p)[i], x);
+  __device__ __forceinline__ cplx<T> mul(uint32_t i, cplx<T> x) const {
+    if (cplx_) return cmul(at((const cplx<T>*)p, i), x);
+    return rmul(at((const T*)p, i), x);
   }
 };
 
@@ -78,7 +90,7 @@ template <typename T> struct Coef {
 // K1: spectral state -> mixed fields (inverse y-transform), one column / group
 // ---------------------------------------------------------------------------
 #ifndef GF_K1_WAVES
-#define GF_K1_WAVES 2
+#define GF_K1_WAVES 3
 #endif
 #ifndef GF_K2_WAVES
 #define GF_K2_WAVES 2
@@ -97,98 +109,115 @@ template <int MODEL, int FS> struct Fields {
   }
 };
 
+// Thread layout of the y-direction kernels: a workgroup holds G column pairs;
+// lane bit 0 selects the column of a pair and the remaining bits the FFT thread,
+// so the two columns of a pair (interleaved in memory) are read and written by
+// one wave as one contiguous run.
+template <int TPT> struct YMap {
+  int c, t, slot;
+  __device__ __forceinline__ YMap(int x) : c(x & 1), t((x >> 1) % TPT), slot((x >> 1) / TPT) {}
+};
+
+// Body of K1 for block blk of column pairs; lds holds 2G padded N-point buffers and
+// tw the per-pass twiddle tables (already loaded).
 template <typename T, int NY, int P, int G, int NL, int MODEL, int FS>
-__global__ __launch_bounds__(G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K1_WAVES)))
-void k1_kernel(const K1Args<T> a) {
+__device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __restrict__ lds,
+                                        const cplx<T>* __restrict__ tw) {
   constexpr int TPT = NY / P;
   using FSET = Fields<MODEL, FS>;
-  __shared__ cplx<T> lds[G][NY];
-  const int grp = threadIdx.x / TPT, t = threadIdx.x % TPT;
-  const int kr = blockIdx.x * G + grp;          // column
+  const YMap<TPT> m(threadIdx.x);
+  const int t = m.t;
+  const int kr = 2 * (blk * G + m.slot) + m.c;
   const Geo<T>& g = a.g;
   const bool active = kr < a.ncols && kr < g.nkr;
   const bool masked = FS != FS_RAW;
   const bool krk = kr < g.kmax || !masked;
   const T kv = krwave(g, kr);
-  const cplx<T>* tw = a.tw;
-  cplx<T> s[NL][P];
-#pragma unroll
-  for (int j = 0; j < NL; ++j)
-#pragma unroll
-    for (int q = 0; q < P; ++q) {
-      const int l = t + TPT * q;
-      const bool lk = kept_l(g, l) || !masked;
-      cplx<T> val = czero<T>();
-      if (active && lk && krk) val = a.sol[j * g.lstride + boff(kr, l, NY)];
-      s[j][q] = val;
-    }
+  cplx<T>* L = lds + (2 * m.slot + m.c) * LdsN<NY>::value;
   const T nrm = g.inv_norm;
+  // masked state value of layer j at y-index l (re-read per field: an L2 hit
+  // after the first field, and it keeps the register footprint small)
+  auto sval = [&](int j, int l) -> cplx<T> {
+    const bool lk = kept_l(g, l) || !masked;
+    cplx<T> v = czero<T>();
+    if (active && lk && krk) v = at(a.sol, (uint32_t)(j * g.lstride) + boff32(kr, l, NY));
+    return v;
+  };
 #pragma unroll
   for (int jl = 0; jl < NL; ++jl) {
-    // MultiLayerQG: psi_jl = sum_k Sinv[jl][k] q_k (src/multilayerqg.jl:495-509, 563-583)
-    cplx<T> psi[P];
-    if constexpr (NL > 1 && FS != FS_RAW) {
+    // per-point inversion factor, computed once before the fields that use it:
+    // TwoDNS 1/|k|^2 (invKrsq, [FF] TwoDGrid), QG psih (src/singlelayerqg.jl:484-490,
+    // src/multilayerqg.jl:495-509)
+    T iK[P];
+    cplx<T> ps[P];
+    if constexpr (FS != FS_RAW) {
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         const int l = t + TPT * q;
-        cplx<T> acc = czero<T>();
-        if (active) {
-          const T* m = a.sinv + boff(kr, l, NY) * (NL * NL) + jl * NL;
+        const T lv = lwave(g, l);
+        const T K2 = kv * kv + lv * lv;
+        if constexpr (MODEL == M_TWODNS) {
+          iK[q] = (kr == 0 && l == 0) ? (T)0 : (T)1 / K2;
+        } else if constexpr (NL > 1) {
+          cplx<T> acc = czero<T>();
+          if (active) {
+            const T* mm = a.sinv + boff(kr, l, NY) * (NL * NL) + jl * NL;
 #pragma unroll
-          for (int k = 0; k < NL; ++k) acc = acc + rmul(m[k], s[k][q]);
+            for (int k = 0; k < NL; ++k) acc = acc + rmul(mm[k], sval(k, l));
+          }
+          ps[q] = acc;
+        } else {
+          const cplx<T> sv = sval(jl, l);
+          if (a.eb) ps[q] = mk(-sv.re / (K2 + a.ell2inv), -sv.im / (K2 + a.ell2inv));
+          else ps[q] = rmul(-((kr == 0 && l == 0) ? (T)0 : (T)1 / K2), sv);
         }
-        psi[q] = acc;
       }
     }
 #pragma unroll
     for (int f = 0; f < FSET::N; ++f) {
-      constexpr int dummy = 0; (void)dummy;
       const int code = FSET::code(f);
       cplx<T> w[1][P];
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         const int l = t + TPT * q;
         const T lv = lwave(g, l);
-        const cplx<T> sv = s[jl][q];
-        cplx<T> r = sv;
+        cplx<T> r;
         if (code == F_RAW || code == F_Q) {
-          r = sv;
+          r = sval(jl, l);
         } else if constexpr (MODEL == M_TWODNS) {
           // uh = i l invK sol, vh = -i kr invK sol (src/twodnavierstokes.jl:254-255)
-          const T K2 = kv * kv + lv * lv;
-          const T iK = (kr == 0 && l == 0) ? (T)0 : (T)1 / K2;
-          if (code == F_U) { const T aa = lv * iK; r = mk(-aa * sv.im, aa * sv.re); }
-          else if (code == F_V) { const T bb = kv * iK; r = mk(bb * sv.im, -bb * sv.re); }
-          else { r = rmul(-iK, sv); }
+          const cplx<T> sv = sval(jl, l);
+          if (code == F_U) { const T aa = lv * iK[q]; r = mk(-aa * sv.im, aa * sv.re); }
+          else if (code == F_V) { const T bb = kv * iK[q]; r = mk(bb * sv.im, -bb * sv.re); }
+          else { r = rmul(-iK[q], sv); }
         } else {
-          cplx<T> ps;
-          if constexpr (NL > 1) {
-            ps = psi[q];
-          } else {
-            // psih = -invK qh, or -qh/(K^2 + 1/ell^2) (src/singlelayerqg.jl:484-490)
-            const T K2 = kv * kv + lv * lv;
-            if (a.eb) ps = mk(-sv.re / (K2 + a.ell2inv), -sv.im / (K2 + a.ell2inv));
-            else {
-              const T iK = (kr == 0 && l == 0) ? (T)0 : (T)1 / K2;
-              ps = rmul(-iK, sv);
-            }
-          }
           // uh = -i l psih, vh = i kr psih (src/singlelayerqg.jl:360-361)
-          if (code == F_U) r = mk(lv * ps.im, -lv * ps.re);
-          else if (code == F_V) r = mk(-kv * ps.im, kv * ps.re);
-          else r = ps;
+          const cplx<T> p = ps[q];
+          if (code == F_U) r = mk(lv * p.im, -lv * p.re);
+          else if (code == F_V) r = mk(-kv * p.im, kv * p.re);
+          else r = p;
         }
         // normalised inverse = conj(F(conj(x)))/(nx ny); the scale is a power of two
         w[0][q] = mk(r.re * nrm, -(r.im * nrm));
       }
-      fft_forward<T, NY, P, 1>(w, &lds[grp][0], t, tw);
+      fft_forward<T, NY, P, 1>(w, L, t, tw);
       if (active) {
         cplx<T>* o = a.out + (long)(jl * FSET::N + f) * a.fstride;
 #pragma unroll
-        for (int q = 0; q < P; ++q) o[boff(kr, t + TPT * q, NY)] = conj(w[0][q]);
+        for (int q = 0; q < P; ++q) at(o, boff32(kr, t + TPT * q, NY)) = conj(w[0][q]);
       }
     }
   }
+}
+
+template <typename T, int NY, int P, int G, int NL, int MODEL, int FS>
+__global__ __launch_bounds__(2 * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K1_WAVES)))
+void k1_kernel(const K1Args<T> a) {
+  __shared__ cplx<T> lds[2 * G * LdsN<NY>::value];
+  __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
+  load_twiddles<T, NY, P>(tw, a.tw);
+  __syncthreads();
+  k1_body<T, NY, P, G, NL, MODEL, FS>(a, blockIdx.x, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -199,7 +228,7 @@ template <typename T, int NX>
 __device__ __forceinline__ cplx<T> herm_load(const cplx<T>* f, int k, int y, int ny, int kin, bool active) {
   const int kk = k <= NX / 2 ? k : NX - k;
   if (!active || kk >= kin) return czero<T>();
-  cplx<T> v = f[boff(kk, y, ny)];
+  cplx<T> v = at(f, boff32(kk, y, ny));
   if (kk == 0 || kk == NX / 2) v.im = (T)0;   // c2r ignores the imaginary DC/Nyquist parts
   else if (k > NX / 2) v.im = -v.im;
   return v;
@@ -213,20 +242,19 @@ __device__ __forceinline__ cplx<T> split_row(cplx<T> Wk, cplx<T> Wm, int row) {
                   : mk((Wk.im + Wm.im) / 2, (Wm.re - Wk.re) / 2);
 }
 
+// Body of K2 for block blk of row pairs; lds holds G padded N-point buffers.
 template <typename T, int NX, int P, int G, int MODE, int NL>
-__global__ __launch_bounds__(G * (NX / P)) __attribute__((amdgpu_waves_per_eu(GF_K2_WAVES)))
-void k2_kernel(const K2Args<T> a) {
+__device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __restrict__ lds,
+                                        const cplx<T>* __restrict__ tw) {
   constexpr int TPT = NX / P;
   constexpr int MAXI = (NX / 2 + 1 + TPT - 1) / TPT;
-  __shared__ cplx<T> lds[G][NX];
   const int grp = threadIdx.x / TPT, t = threadIdx.x % TPT;
   const Geo<T>& g = a.g;
   const int ny = g.ny;
-  const int rp = blockIdx.x * G + grp;
+  const int rp = blk * G + grp;
   const bool active = rp < ny / 2;
   const int y0 = 2 * rp;
-  const cplx<T>* tw = a.tw;
-  cplx<T>* L = &lds[grp][0];
+  cplx<T>* L = lds + grp * LdsN<NX>::value;
 
   if constexpr (MODE == K2_FORWARD_IN) {
     // physical rows (y0, y0+1) of each field -> half spectra along x
@@ -244,13 +272,13 @@ void k2_kernel(const K2Args<T> a) {
       }
       fft_forward<T, NX, P, 1>(w, L, t, tw);
 #pragma unroll
-      for (int q = 0; q < P; ++q) L[swz<T>(t + TPT * q)] = w[0][q];
+      for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = w[0][q];
       __syncthreads();
       if (active) {
         cplx<T>* o = a.out + f * a.ostride;
         for (int it = t; it < 2 * a.kout; it += TPT) {
           const int row = it & 1, kr = it >> 1;
-          o[boff(kr, y0 + row, ny)] = split_row(L[swz<T>(kr)], L[swz<T>((NX - kr) & (NX - 1))], row);
+          o[boff(kr, y0 + row, ny)] = split_row(L[pad(kr)], L[pad((NX - kr) & (NX - 1))], row);
         }
       }
       __syncthreads();
@@ -320,8 +348,8 @@ void k2_kernel(const K2Args<T> a) {
         if constexpr (!FULL) {
           if (a.eta != nullptr && active) {
             // u (q + eta), v (q + eta) (src/singlelayerqg.jl:368-370)
-            q0 = q0 + a.eta[(long)y0 * NX + x];
-            q1 = q1 + a.eta[(long)(y0 + 1) * NX + x];
+            q0 = q0 + at(a.eta, (uint32_t)(y0 * NX + x));
+            q1 = q1 + at(a.eta, (uint32_t)((y0 + 1) * NX + x));
           }
           z[0][q] = mk(u0 * q0, v0 * q0);
           z[1][q] = mk(u1 * q1, v1 * q1);
@@ -359,14 +387,14 @@ void k2_kernel(const K2Args<T> a) {
       cplx<T>* oB = a.out + (long)(2 * jl + 1) * a.ostride;
       for (int row = 0; row < 2; ++row) {
 #pragma unroll
-        for (int q = 0; q < P; ++q) L[swz<T>(t + TPT * q)] = z[row][q];
+        for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = z[row][q];
         __syncthreads();
         cplx<T> Arow[MAXI], Brow[MAXI];
 #pragma unroll
         for (int i = 0; i < MAXI; ++i) {
           const int kr = t + i * TPT;
           if (kr < a.kout) {
-            const cplx<T> Wk = L[swz<T>(kr)], Wm = L[swz<T>((NX - kr) & (NX - 1))];
+            const cplx<T> Wk = L[pad(kr)], Wm = L[pad((NX - kr) & (NX - 1))];
             Arow[i] = split_row(Wk, Wm, 0);
             Brow[i] = split_row(Wk, Wm, 1);
           }
@@ -375,13 +403,13 @@ void k2_kernel(const K2Args<T> a) {
         if constexpr (FULL) {
           // this row's C = u Qx + v Qy spectrum from the packed (C0 + i C1) transform
 #pragma unroll
-          for (int q = 0; q < P; ++q) L[swz<T>(t + TPT * q)] = z[2][q];
+          for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = z[2][q];
           __syncthreads();
 #pragma unroll
           for (int i = 0; i < MAXI; ++i) {
             const int kr = t + i * TPT;
             if (kr < a.kout) {
-              const cplx<T> Cv = split_row(L[swz<T>(kr)], L[swz<T>((NX - kr) & (NX - 1))], row);
+              const cplx<T> Cv = split_row(L[pad(kr)], L[pad((NX - kr) & (NX - 1))], row);
               const T kv = krwave(g, kr);
               // P' = -i kr (u q)^ - (u Qx + v Qy)^
               Arow[i] = mk(kv * Arow[i].im - Cv.re, -kv * Arow[i].re - Cv.im);
@@ -394,9 +422,9 @@ void k2_kernel(const K2Args<T> a) {
           for (int i = 0; i < MAXI; ++i) {
             const int kr = t + i * TPT;
             if (kr < a.kout) {
-              const long o = boff(kr, y0 + row, ny);
-              oA[o] = Arow[i];
-              oB[o] = Brow[i];
+              const uint32_t o = boff32(kr, y0 + row, ny);
+              at(oA, o) = Arow[i];
+              at(oB, o) = Brow[i];
             }
           }
         }
@@ -405,20 +433,30 @@ void k2_kernel(const K2Args<T> a) {
   }
 }
 
+template <typename T, int NX, int P, int G, int MODE, int NL>
+__global__ __launch_bounds__(G * (NX / P)) __attribute__((amdgpu_waves_per_eu(GF_K2_WAVES)))
+void k2_kernel(const K2Args<T> a) {
+  __shared__ cplx<T> lds[G * LdsN<NX>::value];
+  __shared__ cplx<T> tw[TwPlan<NX, P>::size()];
+  load_twiddles<T, NX, P>(tw, a.tw);
+  __syncthreads();
+  k2_body<T, NX, P, G, MODE, NL>(a, blockIdx.x, lds, tw);
+}
+
 // ---------------------------------------------------------------------------
 // K3: flux -> spectral N + stepper stage epilogue, one column / group
 // ---------------------------------------------------------------------------
+// Body of K3 for block blk of column pairs.
 template <typename T, int NY, int P, int G, int NL>
-__global__ __launch_bounds__(G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K3_WAVES)))
-void k3_kernel(const K3Args<T> a) {
+__device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __restrict__ lds,
+                                        const cplx<T>* __restrict__ tw) {
   constexpr int TPT = NY / P;
-  __shared__ cplx<T> lds[G][NY];
-  const int grp = threadIdx.x / TPT, t = threadIdx.x % TPT;
-  const int kr = blockIdx.x * G + grp;
+  const YMap<TPT> m(threadIdx.x);
+  const int t = m.t;
+  const int kr = 2 * (blk * G + m.slot) + m.c;
   const Geo<T>& g = a.g;
   const bool active = kr < a.ncols && kr < g.nkr;
-  const cplx<T>* tw = a.tw;
-  cplx<T>* L = &lds[grp][0];
+  cplx<T>* L = lds + (2 * m.slot + m.c) * LdsN<NY>::value;
   const T kv = krwave(g, kr);
 
   if (a.epi == E_STORE) {
@@ -467,8 +505,8 @@ void k3_kernel(const K3Args<T> a) {
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int y = t + TPT * q;
-      A[0][q] = active ? fa[boff(kr, y, NY)] : czero<T>();
-      B[0][q] = active ? fb[boff(kr, y, NY)] : czero<T>();
+      A[0][q] = active ? at(fa, boff32(kr, y, NY)) : czero<T>();
+      B[0][q] = active ? at(fb, boff32(kr, y, NY)) : czero<T>();
     }
     fft_forward<T, NY, P, 1>(A, L, t, tw);
     fft_forward<T, NY, P, 1>(B, L, t, tw);
@@ -480,8 +518,8 @@ void k3_kernel(const K3Args<T> a) {
       const bool kept = kept_l(g, l) && kr < g.kmax;
       if (!kept && !a.full) continue;
       const T lv = lwave(g, l);
-      const long idx = boff(kr, l, NY);
-      const long ix = lo + idx;
+      const uint32_t idx = boff32(kr, l, NY);
+      const uint32_t ix = (uint32_t)lo + idx;
       cplx<T> N;
       const cplx<T> Av = A[0][q], Bv = B[0][q];
       if (a.apply_kr) {
@@ -492,7 +530,7 @@ void k3_kernel(const K3Args<T> a) {
       }
       if (a.etah != nullptr) {
         // N -= i kr U etah (src/singlelayerqg.jl:379)
-        const cplx<T> e = a.etah[idx];
+        const cplx<T> e = at(a.etah, idx);
         const T cu = kv * a.Ueta;
         N = mk(N.re + cu * e.im, N.im - cu * e.re);
       }
@@ -503,85 +541,85 @@ void k3_kernel(const K3Args<T> a) {
           N = N + rmul(a.mu_drag * K2, psin[q]);
         }
       }
-      if (a.Fh != nullptr) N = N + a.Fh[ix];
+      if (a.Fh != nullptr) N = N + at(a.Fh, ix);
       if (a.sqrt_spec != nullptr) {
         if (kr <= a.fkx && (l <= a.fky || l >= NY - a.fky)) {
-          const T sp = a.sqrt_spec[idx];
+          const T sp = at(a.sqrt_spec, idx);
           if (sp != (T)0) {
             const cplx<T> xi = complex_normal<T>(a.seed, a.fstep, (uint32_t)(l * g.nkr + kr), (uint32_t)a.instance);
             N = N + mk((xi.re * sp) / a.sqrt_dt, (xi.im * sp) / a.sqrt_dt);
           }
         }
       }
-      const long ci = a.cl_layered ? ix : idx;
-      const cplx<T> sv = kept ? a.sol[ix] : czero<T>();
+      const uint32_t ci = a.cl_layered ? ix : idx;
+      const cplx<T> sv = kept ? at(a.sol, ix) : czero<T>();
       switch (a.epi) {
         case E_CALCN:
-          a.nout[ix] = N;
+          at(a.nout, ix) = N;
           break;
         case E_ETD1: {  // sol1 = E2 sol + zeta N1
           if (kept) {
             const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
-            a.s1[ix] = E2.mul(ci, sv) + Z.mul(ci, N);
+            at(a.s1, ix) = E2.mul(ci, sv) + Z.mul(ci, N);
           }
-          a.n1[ix] = N;
+          at(a.n1, ix) = N;
         } break;
         case E_ETD2: {  // sol2 = E2 sol + zeta N2
           if (kept) {
             const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
-            a.s2[ix] = E2.mul(ci, sv) + Z.mul(ci, N);
+            at(a.s2, ix) = E2.mul(ci, sv) + Z.mul(ci, N);
           }
-          a.n23[ix] = N;
+          at(a.n23, ix) = N;
         } break;
         case E_ETD3: {  // sol2 = E2 sol1 + zeta (2 N3 - N1)
           if (kept) {
             const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
-            const cplx<T> s1v = a.s1[ix];
-            const cplx<T> n1v = a.n1[ix];
-            a.s2[ix] = E2.mul(ci, s1v) + Z.mul(ci, mk((T)2 * N.re - n1v.re, (T)2 * N.im - n1v.im));
+            const cplx<T> s1v = at(a.s1, ix);
+            const cplx<T> n1v = at(a.n1, ix);
+            at(a.s2, ix) = E2.mul(ci, s1v) + Z.mul(ci, mk((T)2 * N.re - n1v.re, (T)2 * N.im - n1v.im));
           }
-          a.n23[ix] = a.n23[ix] + N;
+          at(a.n23, ix) = at(a.n23, ix) + N;
         } break;
         case E_ETD4: {  // sol = E sol + alpha N1 + 2 beta (N2 + N3) + Gamma N4
           const Coef<T> E{a.c2, a.ccplx}, Al{a.c3, a.ccplx}, Be{a.c4, a.ccplx}, Ga{a.c5, a.ccplx};
-          const cplx<T> n23v = a.n23[ix];
-          cplx<T> r = E.mul(ci, sv) + Al.mul(ci, a.n1[ix]);
+          const cplx<T> n23v = at(a.n23, ix);
+          cplx<T> r = E.mul(ci, sv) + Al.mul(ci, at(a.n1, ix));
           r = r + Be.mul(ci, mk((T)2 * n23v.re, (T)2 * n23v.im));
           r = r + Ga.mul(ci, N);
-          if (a.filter) r = rmul(a.filter[idx], r);
-          a.sol[ix] = r;
+          if (a.filter) r = rmul(at(a.filter, idx), r);
+          at(a.sol, ix) = r;
         } break;
         case E_RK1:
         case E_RK2:
         case E_RK3:
         case E_RK4: {
           const Coef<T> Lc{a.c0, a.ccplx};
-          const cplx<T> xin = (a.epi == E_RK1) ? sv : (kept ? a.s1[ix] : czero<T>());
+          const cplx<T> xin = (a.epi == E_RK1) ? sv : (kept ? at(a.s1, ix) : czero<T>());
           const cplx<T> R = N + Lc.mul(ci, xin);   // RHS = N + L sol_i
           if (a.epi == E_RK1) {
-            if (kept) a.s1[ix] = sv + rmul(a.dt / (T)2, R);
-            a.racc[ix] = mk(R.re / (T)6, R.im / (T)6);
+            if (kept) at(a.s1, ix) = sv + rmul(a.dt / (T)2, R);
+            at(a.racc, ix) = mk(R.re / (T)6, R.im / (T)6);
           } else if (a.epi == E_RK2) {
-            if (kept) a.s1[ix] = sv + rmul(a.dt / (T)2, R);
-            const cplx<T> ra = a.racc[ix];
-            a.racc[ix] = mk(ra.re + R.re / (T)3, ra.im + R.im / (T)3);
+            if (kept) at(a.s1, ix) = sv + rmul(a.dt / (T)2, R);
+            const cplx<T> ra = at(a.racc, ix);
+            at(a.racc, ix) = mk(ra.re + R.re / (T)3, ra.im + R.im / (T)3);
           } else if (a.epi == E_RK3) {
-            if (kept) a.s1[ix] = sv + rmul(a.dt, R);
-            const cplx<T> ra = a.racc[ix];
-            a.racc[ix] = mk(ra.re + R.re / (T)3, ra.im + R.im / (T)3);
+            if (kept) at(a.s1, ix) = sv + rmul(a.dt, R);
+            const cplx<T> ra = at(a.racc, ix);
+            at(a.racc, ix) = mk(ra.re + R.re / (T)3, ra.im + R.im / (T)3);
           } else {
-            const cplx<T> ra = a.racc[ix];
+            const cplx<T> ra = at(a.racc, ix);
             const cplx<T> acc = mk(ra.re + R.re / (T)6, ra.im + R.im / (T)6);
             cplx<T> r = sv + rmul(a.dt, acc);
-            if (a.filter) r = rmul(a.filter[idx], r);
-            a.sol[ix] = r;
+            if (a.filter) r = rmul(at(a.filter, idx), r);
+            at(a.sol, ix) = r;
           }
         } break;
         case E_FE: {
           const Coef<T> Lc{a.c0, a.ccplx};
           cplx<T> r = sv + rmul(a.dt, N + Lc.mul(ci, sv));
-          if (a.filter) r = rmul(a.filter[idx], r);
-          a.sol[ix] = r;
+          if (a.filter) r = rmul(at(a.filter, idx), r);
+          at(a.sol, ix) = r;
         } break;
         default: break;
       }
@@ -589,19 +627,35 @@ void k3_kernel(const K3Args<T> a) {
   }
 }
 
+template <typename T, int NY, int P, int G, int NL>
+__global__ __launch_bounds__(2 * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K3_WAVES)))
+void k3_kernel(const K3Args<T> a) {
+  __shared__ cplx<T> lds[2 * G * LdsN<NY>::value];
+  __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
+  load_twiddles<T, NY, P>(tw, a.tw);
+  __syncthreads();
+  k3_body<T, NY, P, G, NL>(a, blockIdx.x, lds, tw);
+}
+
 // ---------------------------------------------------------------------------
 // size dispatch helpers
 // ---------------------------------------------------------------------------
-template <int N> struct PY { static constexpr int P = N >= 4096 ? 16 : (N >= 32 ? 8 : 4); };
+#ifndef GF_P2048
+#define GF_P2048 8
+#endif
+template <int N> struct PY { static constexpr int P = N >= 4096 ? 16 : (N == 2048 ? GF_P2048 : (N >= 32 ? 8 : 4)); };
+// K2: G thread groups of N/P threads (one row pair each), at least one wave per workgroup
 template <int N, int P> struct GW { static constexpr int G = (N / P) >= 64 ? 1 : 64 / (N / P); };
+// K1/K3: G column pairs of 2 N/P threads each, at least one wave per workgroup
+template <int N, int P> struct GY { static constexpr int G = 2 * (N / P) >= 64 ? 1 : 64 / (2 * (N / P)); };
 
 template <typename T, int N, int NL>
 void k1_n(const K1Args<T>& a, hipStream_t s) {
   constexpr int P = (NL > 1 && N >= 32) ? 4 : PY<N>::P;
-  constexpr int G = GW<N, P>::G;
-  const int nblk = (a.ncols + G - 1) / G;
+  constexpr int G = GY<N, P>::G;
+  const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
   if (nblk <= 0) return;
-  const dim3 grid(nblk), blk(G * (N / P));
+  const dim3 grid(nblk), blk(2 * G * (N / P));
   if constexpr (NL > 1) {
     if (a.fieldset == FS_CALCN) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_CALCN>), grid, blk, 0, s, a);
     else if (a.fieldset == FS_OUT) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_OUT>), grid, blk, 0, s, a);
@@ -634,9 +688,9 @@ void k2_n(const K2Args<T>& a, hipStream_t s) {
 template <typename T, int N, int NL>
 void k3_n(const K3Args<T>& a, hipStream_t s) {
   constexpr int P = (NL > 1 && N >= 32) ? 4 : PY<N>::P;
-  constexpr int G = GW<N, P>::G;
-  const int nblk = (a.ncols + G - 1) / G;
-  if (nblk > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, G, NL>), dim3(nblk), dim3(G * (N / P)), 0, s, a);
+  constexpr int G = GY<N, P>::G;
+  const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
+  if (nblk > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, G, NL>), dim3(nblk), dim3(2 * G * (N / P)), 0, s, a);
 }
 
 }  // namespace gf

@@ -66,7 +66,7 @@ def Problem(nlayers, dev=None, *, nx=128, ny=None, Lx=2 * math.pi, Ly=None, f0=1
     p.nlayers, p.f0, p.β, p.H, p.b, p.U, p.μ, p.ν, p.nν, p.stepper = n, f0, β, Ha, ba, Ua, μ, ν, nν, stepper
     gp = ba[:-1] - ba[1:]
     p.gprime = gp
-    prob = MLQGProblem(d, grid, p, nlayers=n)
+    prob = MLQGProblem(d, grid, p, nlayers=n, keep=keep)
     if Fh is not None:
         prob.set_forcing(Fh)
     return prob

@@ -11,6 +11,7 @@ is a NumPy array of shape ``(ny, nx)`` here -- the same bytes.
 """
 import ctypes
 import math
+import threading
 
 import numpy as np
 
@@ -93,14 +94,20 @@ class Vars:
     """Physical fields written by ``updatevars`` (names as in the reference's Vars)."""
 
 
+_DEFER = threading.local()
+
+
 class Problem:
     """Device-resident problem; see ``TwoDNavierStokes.Problem`` etc. for the constructors."""
 
     _names = ()
 
-    def __init__(self, desc, grid, params, nlayers=1):
+    def __init__(self, desc, grid, params, nlayers=1, keep=()):
         self._desc = desc
         self._dt = desc.dt
+        self._keep = list(keep)          # arrays the descriptor points into
+        self._owner = None
+        self._pending_Fh = None
         self.grid = grid
         self.params = params
         self.nlayers = nlayers
@@ -108,16 +115,26 @@ class Problem:
         self.stepper = params.stepper
         self._ctype = np.complex128 if desc.dtype == _lib.GF_F64 else np.complex64
         self._rtype = np.float64 if desc.dtype == _lib.GF_F64 else np.float32
+        self.clock = Clock(self)
+        self._ctx = None
+        if getattr(_DEFER, "on", False):
+            return                       # an Ensemble binds this problem to one of its members
         ctx = ctypes.c_void_p()
         check(lib.gf_ctx_create(ctypes.byref(ctx), ctypes.byref(desc)))
         self._ctx = ctx
-        self.clock = Clock(self)
+
+    def _bind(self, member, owner):
+        self._ctx = member
+        self._owner = owner              # keeps the ensemble alive while this view is used
+        if self._pending_Fh is not None:
+            Fh, self._pending_Fh = self._pending_Fh, None
+            self.set_forcing(Fh)
 
     def __del__(self):
         ctx = getattr(self, "_ctx", None)
-        if ctx is not None and ctx.value:
+        if ctx is not None and ctx.value and getattr(self, "_owner", None) is None:
             lib.gf_ctx_destroy(ctx)
-            self._ctx = None
+        self._ctx = None
 
     # ---- state ----
     @property
@@ -167,6 +184,9 @@ class Problem:
 
     def set_forcing(self, Fh):
         a = np.ascontiguousarray(np.asarray(Fh, dtype=self._ctype).reshape(self._sshape))
+        if self._ctx is None:
+            self._pending_Fh = a
+            return
         check(lib.gf_set_forcing(self._ctx, a.ctypes.data), self._ctx)
 
     def synchronize(self):
@@ -177,8 +197,66 @@ class Problem:
 
 
 def stepforward(prob, nsteps=1):
-    """[FF] ``stepforward!(prob, nsteps)``: advance ``nsteps`` on the device."""
+    """[FF] ``stepforward!(prob, nsteps)``: advance ``nsteps`` on the device.
+
+    ``prob`` may be an ``Ensemble`` (every member advances, batched) or one of its
+    members (that member alone advances)."""
     check(lib.gf_step(prob._ctx, int(nsteps)), prob._ctx)
+
+
+class Ensemble:
+    """Independent problems advanced together on the device.
+
+    ``members`` is a list of ``(constructor, kwargs)`` pairs, for example
+    ``(SingleLayerQG.Problem, dict(nx=64, β=10.0, stepper="FilteredRK4", dt=0.01))``;
+    sizes may differ.  ``ens.problems[i]`` is an ordinary ``Problem`` bound to member
+    ``i`` (``set_q``, ``updatevars``, ``sol`` and diagnostics work on it);
+    ``stepforward(ens, n)`` advances every member with one launch per kernel class
+    and stage, ``stepforward(ens.problems[i], n)`` advances member ``i`` alone.
+    The reference has no such object: an ensemble there is a loop over Problems.
+    """
+
+    def __init__(self, members):
+        probs = []
+        _DEFER.on = True
+        try:
+            for ctor, kw in members:
+                probs.append(ctor(GPU(), **kw))
+        finally:
+            _DEFER.on = False
+        n = len(probs)
+        descs = (_lib.ProblemDesc * n)(*[p._desc for p in probs])
+        ctx = ctypes.c_void_p()
+        check(lib.gf_ctx_create_batch(ctypes.byref(ctx), descs, n))
+        self._ctx = ctx
+        self.problems = probs
+        for i, p in enumerate(probs):
+            m = ctypes.c_void_p()
+            check(lib.gf_ctx_instance(ctx, i, ctypes.byref(m)), ctx)
+            p._bind(m, self)
+
+    def __len__(self):
+        return len(self.problems)
+
+    def __getitem__(self, i):
+        return self.problems[i]
+
+    @property
+    def t(self):
+        t = ctypes.c_double()
+        check(lib.gf_clock(self._ctx, ctypes.byref(t), None), self._ctx)
+        return t.value
+
+    def synchronize(self):
+        check(lib.gf_synchronize(self._ctx), self._ctx)
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx is not None and ctx.value:
+            for p in getattr(self, "problems", ()):
+                p._ctx = None
+            lib.gf_ctx_destroy(ctx)
+        self._ctx = None
 
 
 def make_desc(model, nx, ny, Lx, Ly, T, stepper, dt, aliased_fraction):

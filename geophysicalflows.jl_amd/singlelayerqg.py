@@ -50,7 +50,7 @@ def Problem(dev=None, *, nx=256, ny=None, Lx=2 * math.pi, Ly=None, β=0.0, defor
     Fh = attach_forcing(d, calcF)
     p = Params()
     p.β, p.deformation_radius, p.U, p.eta, p.μ, p.ν, p.nν, p.calcF, p.stepper = β, ell, U, eta, μ, ν, nν, calcF, stepper
-    prob = SLQGProblem(d, grid, p)
+    prob = SLQGProblem(d, grid, p, keep=keep)
     p.etah = None
     if Fh is not None:
         prob.set_forcing(Fh)

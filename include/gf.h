@@ -142,6 +142,23 @@ int gf_timing_read(gf_ctx* ctx, int k, int64_t* launches, double* ms);
  * step phase: full = 1 for the last step of a gf_step call. */
 int gf_kernel_bytes(const gf_ctx* ctx, int k, int full, double* bytes);
 
+/* Ensembles of independent problems advanced together.
+ *
+ * No counterpart in the reference: there an ensemble is a Julia loop of
+ * stepforward!(prob, n) over separate Problems (one calcN! per member per stage).
+ * gf_ctx_create_batch builds n members from n descriptors (FP64; TwoDNS or
+ * SingleLayerQG; square grids with n in 32..512, sizes may differ; one stepper
+ * for all; no stochastic ring forcing).  gf_step on the ensemble advances every
+ * member with one launch per kernel class and stage covering all of them.
+ * gf_ctx_instance returns a borrowed handle for member i, valid until the
+ * ensemble is destroyed (gf_ctx_destroy on it does nothing); every
+ * single-problem entry point works on it, and gf_step on it advances that member
+ * alone (the sequential path).  For a plain context gf_batch_size is 1 and
+ * gf_ctx_instance(ctx, 0, &m) gives ctx itself. */
+int gf_ctx_create_batch(gf_ctx** out, const gf_problem_desc* descs, int n);
+int gf_batch_size(const gf_ctx* ctx);
+int gf_ctx_instance(gf_ctx* ensemble, int i, gf_ctx** member);
+
 /* Library version string. */
 const char* gf_version(void);
 

@@ -89,3 +89,54 @@ def test_python_mirror_rejects_unknown_stepper(gf):
         gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, stepper="AB3")
     with pytest.raises(NotImplementedError):
         gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, calcF=lambda *a: None)
+
+
+def _slqg(gf, n=64, **kw):
+    return _desc(gf, model=1, nx=n, ny=n, stepper=3, **kw)
+
+
+@pytest.mark.parametrize("bad,code", [
+    (dict(dtype=1), -2), (dict(model=2, nlayers=2), -2), (dict(ny=64), -2), (dict(nx=1024, ny=1024), -2),
+    (dict(nx=16, ny=16), -2), (dict(forcing=2, dkf=1.0, kf=3.0), -2), (dict(stepper=2), -1), (dict(dt=0.0), -1),
+])
+def test_invalid_ensembles_are_rejected(gf, bad, code):
+    lib = gf._lib.lib
+    descs = (gf._lib.ProblemDesc * 3)(_slqg(gf), _slqg(gf, 128), _slqg(gf))
+    for k, v in bad.items():
+        setattr(descs[1], k, v)
+    ctx = ctypes.c_void_p()
+    assert lib.gf_ctx_create_batch(ctypes.byref(ctx), descs, 3) == code
+    assert not ctx.value
+    assert b"ensemble member 1" in lib.gf_last_error(None)
+
+
+def test_empty_ensemble_and_null_handles(gf):
+    lib = gf._lib.lib
+    ctx = ctypes.c_void_p()
+    assert lib.gf_ctx_create_batch(ctypes.byref(ctx), None, 0) == -1
+    assert lib.gf_batch_size(None) == -1
+    assert lib.gf_ctx_instance(None, 0, ctypes.byref(ctx)) == -1
+    lib.gf_ctx_destroy(None)
+
+
+def test_peakedisotropicspectrum_shape_and_energy(gf):
+    # src/utils.jl:29-58 with allones = true: |psih| = (K^2 (1 + (K/k0)^4))^(-1/2) scaled to E0
+    import numpy as np
+    g = gf.problem.Grid(128, 128, 2 * np.pi, 2 * np.pi, 1 / 3, np.float64)
+    q = gf.peakedisotropicspectrum(g, 6, 0.5, allones=True)
+    qh = np.fft.rfft2(q)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        psih = np.where(g.Krsq > 0, -qh / g.Krsq, 0)
+    assert np.sum(g.Krsq * np.abs(psih) ** 2) / (128 * 128) ** 2 == pytest.approx(0.5, rel=1e-12)
+    K = np.sqrt(g.Krsq)
+    k0 = 6.0
+    sel = (K > 3) & (K < 40)
+    expect = (g.Krsq[sel] * (1 + (K[sel] / k0) ** 4)) ** -0.5
+    ratio = np.abs(psih[sel]) / expect
+    assert np.ptp(ratio) / ratio.mean() < 1e-10
+    # the reference's radial-spectrum check: |qh|/Krsq ~ K^-2 well above k0 (test/test_utils.jl:4-15)
+    hi = (K > 15) & (K <= 17.5)
+    r = (np.abs(qh[hi]) / g.Krsq[hi]) * K[hi] ** 2
+    assert np.ptp(r) / r.mean() < 0.2
+    with pytest.raises(ValueError):
+        gf.peakedisotropicspectrum(gf.problem.Grid(32, 34, 2 * np.pi, 3 * np.pi, 1 / 3, np.float64), 6, 0.5)
