@@ -102,17 +102,24 @@ template <int R, typename T> __device__ __forceinline__ void dft(cplx<T>* v) {
 
 // Twiddle tables.  Pass p of the Stockham plan (stride NS_p, radix R_p) needs
 // w^r with w = exp(-2 pi i k/(NS_p R_p)), k = j mod NS_p, r = 1..R_p-1.  The
-// workgroup keeps, for every pass with NS_p > 1, the NS_p values w(k) contiguous
-// in LDS (sum over passes of NS_p < N entries) so that the lanes of a wave read
-// consecutive slots; the powers w^r are formed by squaring and multiplication
-// (at most three roundings deep for R <= 16).
+// workgroup keeps, for every pass with NS_p > 1, the values w(k) contiguous in
+// LDS so that the lanes of a wave read consecutive slots; the powers w^r are
+// formed by squaring and multiplication (at most three roundings deep for
+// R <= 16).  A pass with NS_p > TW_SPLIT stores w(k) as two tables,
+// w(l) for l < TW_SPLIT and w(TW_SPLIT h) for h < NS_p/TW_SPLIT, and forms
+// w(k) = w(TW_SPLIT h) w(l) with one more rounding: at N = 2048 the tables take
+// 144 slots instead of 584, which lets two 512-thread workgroups share a CU.
+constexpr int TW_SPLIT = 64;
+
 template <int N, int P>
 struct TwPlan {
   using PL = Plan<N, P>;
   static constexpr int ns(int p) { return p == 0 ? 1 : ns(p - 1) * PL::radix(p - 1); }
+  static constexpr bool split(int p) { return ns(p) > TW_SPLIT; }
+  static constexpr int slots(int p) { return split(p) ? TW_SPLIT + ns(p) / TW_SPLIT : ns(p); }
   // offset of pass p's table (passes 1..NPASS-1)
-  static constexpr int off(int p) { return p <= 1 ? 0 : off(p - 1) + ns(p - 1); }
-  static constexpr int size() { return PL::NPASS <= 1 ? 1 : off(PL::NPASS - 1) + ns(PL::NPASS - 1); }
+  static constexpr int off(int p) { return p <= 1 ? 0 : off(p - 1) + slots(p - 1); }
+  static constexpr int size() { return PL::NPASS <= 1 ? 1 : off(PL::NPASS - 1) + slots(PL::NPASS - 1); }
 };
 
 // Fill the per-pass tables from the full table twg[i] = exp(-2 pi i i/N) in global memory.
@@ -122,7 +129,12 @@ __device__ __forceinline__ void load_twiddles(cplx<T>* __restrict__ dst, const c
   using PL = Plan<N, P>;
   if constexpr (PASS < PL::NPASS) {
     constexpr int ns = TP::ns(PASS), step = N / (ns * PL::radix(PASS)), o = TP::off(PASS);
-    for (int k = threadIdx.x; k < ns; k += blockDim.x) dst[o + k] = twg[k * step];
+    if constexpr (TP::split(PASS)) {
+      for (int k = threadIdx.x; k < TW_SPLIT; k += blockDim.x) dst[o + k] = twg[k * step];
+      for (int h = threadIdx.x; h < ns / TW_SPLIT; h += blockDim.x) dst[o + TW_SPLIT + h] = twg[h * TW_SPLIT * step];
+    } else {
+      for (int k = threadIdx.x; k < ns; k += blockDim.x) dst[o + k] = twg[k * step];
+    }
     load_twiddles<T, N, P, PASS + 1>(dst, twg);
   }
 }
@@ -168,7 +180,10 @@ __device__ __forceinline__ void stockham(cplx<T> (&v)[C][P], cplx<T>* __restrict
     cplx<T> w[R];
     if constexpr (NS > 1) {
       constexpr int TOFF = TwPlan<N, P>::off(PASS);
-      w[1] = tw[TOFF + k];
+      if constexpr (TwPlan<N, P>::split(PASS))
+        w[1] = cmul(tw[TOFF + TW_SPLIT + k / TW_SPLIT], tw[TOFF + k % TW_SPLIT]);
+      else
+        w[1] = tw[TOFF + k];
       powers<T, R>(w);
     }
 #pragma unroll

@@ -644,8 +644,17 @@ void k3_kernel(const K3Args<T> a) {
 #define GF_P2048 8
 #endif
 template <int N> struct PY { static constexpr int P = N >= 4096 ? 16 : (N == 2048 ? GF_P2048 : (N >= 32 ? 8 : 4)); };
-// K2: G thread groups of N/P threads (one row pair each), at least one wave per workgroup
-template <int N, int P> struct GW { static constexpr int G = (N / P) >= 64 ? 1 : 64 / (N / P); };
+// K2: G thread groups of N/P threads (one row pair each), at least one wave per workgroup and
+// at least GF_K2_MIN_G row pairs, so that with GF_K2_MIN_G = 2 one workgroup covers the four
+// rows of every 128-byte line of a column pair in the blocked layout (no line is shared
+// between workgroups, which may sit on different XCDs with separate L2s).
+#ifndef GF_K2_MIN_G
+#define GF_K2_MIN_G 2
+#endif
+template <int N, int P> struct GW {
+  static constexpr int G0 = (N / P) >= 64 ? 1 : 64 / (N / P);
+  static constexpr int G = G0 < GF_K2_MIN_G ? GF_K2_MIN_G : G0;
+};
 // K1/K3: G column pairs of 2 N/P threads each, at least one wave per workgroup
 template <int N, int P> struct GY { static constexpr int G = 2 * (N / P) >= 64 ? 1 : 64 / (2 * (N / P)); };
 
