@@ -134,6 +134,81 @@ def ensemble_run(gf, steps, seq_steps):
             "batched_over_sequential": batched / sequential, "kernels": kern, "state_finite": finite}
 
 
+def mlqg_run(gf, steps):
+    """MultiLayerQG, 3 layers, 512^2 FP64 ETDRK4 dt = 0.005: H = [0.2, 0.3, 0.5], densities
+    [4.0, 5.0, 5.375] with g = 1 mapped to buoyancy through the reduced gravities
+    g'_{j+1/2} = g (rho_{j+1} - rho_j) / rho_{j+1} (b = [0, -0.2, -0.2698]; only b_j - b_{j+1}
+    enters, src/multilayerqg.jl:297), f0 = 1, beta = 5, U = [1.0, 0.2, 0.0], mu = 0.005,
+    q = 1e-2 N(0, 1) per layer (seed 4)."""
+    rho = np.array([4.0, 5.0, 5.375])
+    gprime = (rho[1:] - rho[:-1]) / rho[1:]
+    b = np.concatenate([[0.0], -np.cumsum(gprime)])
+    n = 512
+    prob = gf.MultiLayerQG.Problem(3, gf.GPU(), nx=n, Lx=2 * np.pi, f0=1.0, β=5.0, U=[1.0, 0.2, 0.0],
+                                   H=[0.2, 0.3, 0.5], b=b, μ=0.005, dt=0.005, stepper="ETDRK4")
+    q0 = 1e-2 * np.random.default_rng(4).standard_normal((3, n, n))
+    gf.MultiLayerQG.set_q(prob, q0)
+    gf.stepforward(prob, 10)
+    prob.synchronize()
+    t0 = time.perf_counter()
+    gf.stepforward(prob, steps)
+    prob.synchronize()
+    el = time.perf_counter() - t0
+    return {"workload": "MultiLayerQG 3 layers 512^2 FP64 ETDRK4 dt=0.005, H=[0.2,0.3,0.5], rho=[4,5,5.375] "
+                        "(b=[0,-0.2,-0.2698]), f0=1 beta=5 U=[1,0.2,0] mu=0.005",
+            "value": n * n * 3 * steps / el, "unit": "grid-pt-steps/s (nx ny nlayers steps / s)", "steps": steps,
+            "ms_per_step": el / steps * 1e3, "state_finite": bool(np.isfinite(prob.sol).all())}
+
+
+def decaying_run(gf, steps=200):
+    """TwoDNavierStokes decaying turbulence, 128^2 FP64, nu = 0, FilteredRK4 dt = 0.005, 200 steps,
+    zeta0 from peakedisotropicspectrum (k_peak = 6, energy 1, seed 1)."""
+    n = 128
+    prob = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, Lx=2 * np.pi, dt=0.005, stepper="FilteredRK4")
+    grid = gf.problem.Grid(n, n, 2 * np.pi, 2 * np.pi, 1 / 3, np.float64)
+    gf.TwoDNavierStokes.set_ζ(prob, gf.peakedisotropicspectrum(grid, 6, 1.0, rng=np.random.default_rng(1)))
+    prob.synchronize()
+    t0 = time.perf_counter()
+    gf.stepforward(prob, steps)
+    prob.synchronize()
+    el = time.perf_counter() - t0
+    return {"workload": "TwoDNavierStokes decaying 128^2 FP64 nu=0 FilteredRK4 dt=0.005, 200 steps",
+            "value": n * n * steps / el, "unit": "grid-pt-steps/s", "steps": steps, "ms_per_step": el / steps * 1e3,
+            "state_finite": bool(np.isfinite(prob.sol).all())}
+
+
+def adjoint_run(gf, nsteps=2000, checkpoint=50):
+    """Adjoint TwoDNavierStokes 512^2 FP64 ETDRK4 dt = 0.005, nu = 1e-3 (n_nu = 1): gradient of
+    J = 1/2 |zeta(T) - target|^2 after 2000 steps w.r.t. zeta0 and nu, against 2000 forward steps.
+    zeta0 and the target are peakedisotropicspectrum fields (k_peak = 6, energy 0.5, seeds 5, 6)."""
+    n = 512
+    grid = gf.problem.Grid(n, n, 2 * np.pi, 2 * np.pi, 1 / 3, np.float64)
+    z0 = gf.peakedisotropicspectrum(grid, 6, 0.5, rng=np.random.default_rng(5))
+    target = gf.peakedisotropicspectrum(grid, 6, 0.5, rng=np.random.default_rng(6))
+    prob = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, ν=1e-3, nν=1, dt=0.005, stepper="ETDRK4")
+    gf.TwoDNavierStokes.set_ζ(prob, z0)
+    gf.stepforward(prob, 20)
+    prob.synchronize()
+    gf.TwoDNavierStokes.set_ζ(prob, z0)
+    t0 = time.perf_counter()
+    gf.stepforward(prob, nsteps)
+    prob.synchronize()
+    tf = time.perf_counter() - t0
+    gf.TwoDNavierStokes.set_ζ(prob, z0)
+    # untimed: allocates the 2000-step tape (8 spectral fields per step, 33.8 GB) and the tables
+    gf.TwoDNavierStokes.adjoint_gradient(prob, target, nsteps, nsteps, nu=False)
+    out = {"workload": "adjoint TwoDNavierStokes 512^2 FP64 ETDRK4 dt=0.005 nu=1e-3 nnu=1, 2000 steps, "
+                       "gradient of 1/2|zeta(T)-target|^2 w.r.t. zeta0 and nu",
+           "forward_s": tf}
+    for name, ck in (("checkpoint_50", checkpoint), ("full_tape", nsteps)):
+        t0 = time.perf_counter()
+        J, grad, dnu = gf.TwoDNavierStokes.adjoint_gradient(prob, target, nsteps, ck)
+        ta = time.perf_counter() - t0
+        out[name] = {"gradient_s": ta, "adjoint_over_forward": ta / tf, "J": J, "dJ_dnu": dnu,
+                     "grad_finite": bool(np.isfinite(grad).all())}
+    return out
+
+
 def cpu_baseline(nsteps=3):
     """The oracle (NumPy + SciPy pocketfft with one worker per core) on the same workload."""
     from oracle import fourierflows as off
@@ -184,9 +259,19 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--no-stable", action="store_true", help="skip the dt = 0.001 run")
     ap.add_argument("--copy-probe", action="store_true", help="only run the 1 GiB copy used to calibrate counters")
+    ap.add_argument("--only-ensemble", action="store_true", help="only run the batched-ensemble measurement")
+    ap.add_argument("--only-adjoint", action="store_true", help="only run the adjoint measurement")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.only_adjoint:
+        import __graft_entry__ as ge
+        print(json.dumps({"adjoint_512": adjoint_run(ge.load_package())}), flush=True)
+        return
+    if args.only_ensemble:
+        import __graft_entry__ as ge
+        print(json.dumps({"ensemble": ensemble_run(ge.load_package(), max(1, args.ensemble_steps), 3)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -233,6 +318,9 @@ def main():
         out["stable_dt_0.001"] = {"value": NX * NX * steps / es, "ms_per_step": es / steps * 1e3, "state_finite": fs}
     if not args.no_ensemble:
         out["ensemble"] = ensemble_run(gf, max(1, args.ensemble_steps), 3)
+        out["mlqg_512x3"] = mlqg_run(gf, 200)
+        out["decaying_128"] = decaying_run(gf)
+        out["adjoint_512"] = adjoint_run(gf)
     if args.fp32:
         e32, k32, f32 = gpu_run(gf, steps, warmup, np.float32)
         out["fp32"] = {"value": NX * NX * steps / e32, "ms_per_step": e32 / steps * 1e3, "state_finite": f32}

@@ -22,7 +22,7 @@ EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string"
            "gf_set_physical", "gf_updatevars", "gf_num_vars", "gf_calcN", "gf_set_forcing",
            "gf_step", "gf_clock", "gf_set_clock", "gf_synchronize", "gf_timing",
            "gf_timing_read", "gf_kernel_bytes", "gf_version", "gf_ctx_create_batch",
-           "gf_batch_size", "gf_ctx_instance")
+           "gf_batch_size", "gf_ctx_instance", "gf_adjoint_grad")
 
 
 class ProblemDesc(ctypes.Structure):
@@ -84,6 +84,7 @@ def _load():
     lib.gf_ctx_create_batch.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ProblemDesc), ip]
     lib.gf_batch_size.argtypes = [vp]
     lib.gf_ctx_instance.argtypes = [vp, ip, ctypes.POINTER(vp)]
+    lib.gf_adjoint_grad.argtypes = [vp, vp, ip, ip, vp, dp, dp]
     for name in EXPORTS:
         getattr(lib, name)
     return lib

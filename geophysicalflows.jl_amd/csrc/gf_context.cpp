@@ -15,6 +15,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "gf_kernels.hpp"
@@ -78,6 +79,24 @@ void etd_coeffs(cd dtL, double dt, cd& zeta, cd& alpha, cd& beta, cd& gamma) {
   alpha = dt * (sa / (double)M);
   beta = dt * (sb / (double)M);
   gamma = dt * (sg / (double)M);
+}
+
+// d/dL of the contour-mean coefficients (zeta, alpha, beta, Gamma): with z = dt L + e^{i theta},
+// d/dL <f(z)> = dt <f'(z)>, times the dt the coefficients carry.
+void etd_coeff_derivs(cd dtL, double dt, cd& dzeta, cd& dalpha, cd& dbeta, cd& dgamma) {
+  const int M = 32;
+  cd sz = 0, sa = 0, sb = 0, sg = 0;
+  for (int m = 0; m < M; ++m) {
+    const double th = 2 * M_PI / M * (m + 0.5);
+    const cd z = dtL + cd(std::cos(th), std::sin(th));
+    const cd ez = std::exp(z), ez2 = std::exp(z / 2.0), z2 = z * z, z4 = z2 * z2;
+    sz += (ez2 * (z / 2.0 - 1.0) + 1.0) / z2;
+    sa += (12.0 + 2.0 * z + ez * (-12.0 + 10.0 * z - 4.0 * z2 + z2 * z)) / z4;
+    sb += (-6.0 - 2.0 * z + ez * (6.0 - 4.0 * z + z2)) / z4;
+    sg += (12.0 + 6.0 * z + z2 + ez * (-12.0 + 6.0 * z - z2)) / z4;
+  }
+  const double f = dt * dt / M;
+  dzeta = f * sz; dalpha = f * sa; dbeta = f * sb; dgamma = f * sg;
 }
 
 struct DevBuf {
@@ -157,6 +176,9 @@ struct CtxBase {
   virtual int ninstances() const { return 1; }
   virtual CtxBase* instance(int i) { return i == 0 ? this : nullptr; }
   virtual void set_clock(double tt, int64_t s) { t = tt; stepno = s; }
+  virtual void adjoint_grad(const double*, int, int, double*, double*, double*) {
+    throw GfError(GF_EUNSUPPORTED, "the adjoint is provided for TwoDNavierStokes FP64 ETDRK4 problems");
+  }
   double t = 0;
   int64_t stepno = 0;
 };
@@ -701,6 +723,176 @@ struct Ctx final : CtxBase {
     return b + e * L;
   }
   void sync() override { HIPCHECK(hipStreamSynchronize(stream)); }
+
+  // ---------------- adjoint (TwoDNavierStokes, ETDRK4, FP64) ----------------
+  DevBuf adj_mixed, adj_flux, lamb, l0b, l1b, lN1b, lN2b, lN3b, nu_part, dtab[6], tape, ckpts;
+  long tape_steps = 0;
+  int ka3_blocks = 0;
+
+  void adjoint_setup() {
+    if (adj_mixed.p) return;
+    adj_mixed.alloc(sizeof(C) * fsz * 5);
+    adj_flux.alloc(sizeof(C) * fsz * 3);
+    for (DevBuf* b : {&lamb, &l0b, &l1b, &lN1b, &lN2b, &lN3b}) b->alloc(sizeof(C) * fsz);
+    ka3_blocks = adj_blocks<double>(d.ny, g.kmax);
+    nu_part.alloc(sizeof(double) * ka3_blocks);
+    // dJ/dnu tables: d(coefficient)/dL times dL/dnu = -Krsq^nnu (0 at k = 0)
+    const int ny = d.ny, nkr = g.nkr;
+    const size_t nm = (size_t)ny * nkr;
+    std::vector<double> t6[6];
+    for (auto& v : t6) v.assign(nm, 0.0);
+    const double dt = d.dt;
+    parallel_for((long)nm, [&](long i) {
+      const int l = (int)(i / nkr), kr = (int)(i % nkr);
+      const double L = Lop[i].real();
+      const double dLdnu = (kr == 0 && l == 0) ? 0.0 : -std::pow(krsq(kr, l), (double)d.nnu);
+      cd dz, da, db, dg;
+      etd_coeff_derivs(cd(dt * L, 0), dt, dz, da, db, dg);
+      t6[0][i] = dLdnu * dt * std::exp(dt * L);
+      t6[1][i] = dLdnu * dt / 2 * std::exp(dt * L / 2);
+      t6[2][i] = dLdnu * dz.real(); t6[3][i] = dLdnu * da.real();
+      t6[4][i] = dLdnu * db.real(); t6[5][i] = dLdnu * dg.real();
+    });
+    for (int k = 0; k < 6; ++k) upload_blocked(dtab[k], t6[k]);
+  }
+
+  C* slot(long i, int f) const { return tape.as<C>() + ((size_t)i * 8 + f) * fsz; }
+
+  // One ETDRK4 step from tape slot i (s0) writing s1, s2, s3 and N1..N4 of that slot and the
+  // new state to next (kept modes only: every later read masks)
+  void tape_step(long i, C* next) {
+    C* s0 = slot(i, 0);
+    C* s1 = slot(i, 1);
+    C* s2 = slot(i, 2);
+    C* s3 = slot(i, 3);
+    K1Args<T> a; K2Args<T> b; K3Args<T> k;
+    auto run = [&]() {
+      launch_k1<T>(d.ny, a, stream); launch_k2<T>(d.nx, b, stream); launch_k3<T>(d.ny, k, stream);
+    };
+    make_args(s0, E_ETD1, false, a, b, k);
+    k.sol = s0; k.s1 = s1; k.nsave = slot(i, 4); run();
+    make_args(s1, E_ETD2, false, a, b, k);
+    k.sol = s0; k.s2 = s2; k.nsave = slot(i, 5); run();
+    make_args(s2, E_ETD3, false, a, b, k);
+    k.s1 = s1; k.s2 = s3; k.nsave = slot(i, 6); run();
+    make_args(s3, E_ETD4, false, a, b, k);
+    k.sol = s0; k.sol_out = next; k.nsave = slot(i, 7); run();
+    HIPCHECK(hipGetLastError());
+  }
+
+  // adjoint of the step taped in slot i: lamb (lambda of the new state) -> lambda of s0
+  void adjoint_step(long i, bool want_nu) {
+    if constexpr (std::is_same<T, double>::value) {
+      for (int stage = 4; stage >= 1; --stage) {
+        AdjK1Args<T> a{};
+        a.g = g; a.s = slot(i, stage - 1);
+        a.x = stage == 4 ? lamb.as<C>() : stage == 3 ? lN3b.as<C>() : stage == 2 ? lN2b.as<C>() : lN1b.as<C>();
+        a.lamc = stage == 4 ? (const T*)coef[5].p : nullptr;
+        a.out = adj_mixed.as<C>(); a.fstride = fsz; a.ncols = g.kmax; a.tw = twy.as<C>();
+        AdjK2Args<T> b{};
+        b.g = g; b.in = adj_mixed.as<C>(); b.fstride = fsz; b.kin = g.kmax;
+        b.out = adj_flux.as<C>(); b.ostride = fsz; b.kout = g.kmax; b.tw = twx.as<C>();
+        AdjK3Args<T> k{};
+        k.g = g; k.in = adj_flux.as<C>(); k.fstride = fsz; k.ncols = g.kmax; k.stage = stage;
+        k.lam = lamb.as<C>(); k.l0 = l0b.as<C>(); k.l1 = l1b.as<C>();
+        k.lN1 = lN1b.as<C>(); k.lN2 = lN2b.as<C>(); k.lN3 = lN3b.as<C>();
+        k.E2 = (const T*)coef[0].p; k.Z = (const T*)coef[1].p; k.E = (const T*)coef[2].p;
+        k.Al = (const T*)coef[3].p; k.Be = (const T*)coef[4].p;
+        k.dE = dtab[0].as<T>(); k.dE2 = dtab[1].as<T>(); k.dZ = dtab[2].as<T>();
+        k.dA = dtab[3].as<T>(); k.dB = dtab[4].as<T>(); k.dG = dtab[5].as<T>();
+        k.s0 = slot(i, 0); k.s1 = slot(i, 1);
+        k.N1 = slot(i, 4); k.N2 = slot(i, 5); k.N3 = slot(i, 6); k.N4 = slot(i, 7);
+        k.nu_part = want_nu ? nu_part.as<T>() : nullptr;
+        k.tw = twy.as<C>();
+        rec(0, true); launch_adj<T>(1, d.ny, &a, stream); rec(0, false);
+        rec(1, true); launch_adj<T>(2, d.nx, &b, stream); rec(1, false);
+        rec(2, true); launch_adj<T>(3, d.ny, &k, stream); rec(2, false);
+      }
+      HIPCHECK(hipGetLastError());
+    }
+  }
+
+  // host-side Hermitian weight, alias mask and zero-mean handling of a blocked spectral field
+  void host_spectral(std::vector<C>& h, bool to_lambda, double scale) const {
+    for (int l = 0; l < d.ny; ++l)
+      for (int kr = 0; kr < g.nkr; ++kr) {
+        C& z = h[boff_h(kr, l, d.ny)];
+        const bool keep = kr < g.kmax && (l < g.la0 || l >= g.la1);
+        const double w = (kr == 0 || 2 * kr == d.nx) ? 1.0 : 2.0;
+        const double f = !keep ? 0.0 : (to_lambda ? w * scale : scale / w);
+        z.re = (T)(z.re * f); z.im = (T)(z.im * f);
+      }
+    if (!to_lambda) h[boff_h(0, 0, d.ny)] = C{};   // set_zeta!'s zero mean
+  }
+
+  void adjoint_grad(const double* target, int nsteps, int every, double* grad, double* dnu, double* Jout) override {
+    if constexpr (!std::is_same<T, double>::value) {
+      throw GfError(GF_EUNSUPPORTED, "the adjoint is provided for FP64 problems");
+    } else {
+      if (d.model != GF_MODEL_TWODNS || d.stepper != GF_ETDRK4 || d.forcing != GF_FORCING_NONE)
+        throw GfError(GF_EUNSUPPORTED, "the adjoint is provided for unforced TwoDNavierStokes ETDRK4 problems");
+      if (d.nx != d.ny) throw GfError(GF_EUNSUPPORTED, "the adjoint needs a square grid");
+      if (nsteps < 1 || every < 1) throw GfError(GF_EINVAL, "nsteps and checkpoint_every must be >= 1");
+      adjoint_setup();
+      const long seg = std::min(every, nsteps);
+      const long nseg = (nsteps + seg - 1) / seg;
+      if (tape_steps < seg) {
+        tape.alloc(sizeof(C) * fsz * (8 * seg + 1));
+        tape_steps = seg;
+      }
+      ckpts.alloc(sizeof(C) * fsz * nseg);
+      const size_t fb = sizeof(C) * fsz;
+      auto len = [&](long s) { return std::min(seg, (long)nsteps - s * seg); };
+      // forward: checkpoints at segment starts; the last segment's tape stays for the sweep
+      HIPCHECK(hipMemcpyAsync(slot(0, 0), sol.p, fb, hipMemcpyDeviceToDevice, stream));
+      for (long s = 0; s < nseg; ++s) {
+        HIPCHECK(hipMemcpyAsync(ckpts.as<C>() + s * fsz, slot(0, 0), fb, hipMemcpyDeviceToDevice, stream));
+        const long n = len(s);
+        for (long i = 0; i < n; ++i) tape_step(i, slot(i + 1, 0));
+        if (s + 1 < nseg) HIPCHECK(hipMemcpyAsync(slot(0, 0), slot(n, 0), fb, hipMemcpyDeviceToDevice, stream));
+      }
+      // loss and lambda_T = M (w/N) rfft2(zeta_T - target), zeta_T = irfft2(M sol_T)
+      const long nlast = len(nseg - 1);
+      const size_t np = (size_t)d.nx * d.ny;
+      HIPCHECK(hipMemcpyAsync(cinp.p, slot(nlast, 0), fb, hipMemcpyDeviceToDevice, stream));
+      launch_mask<T>(g, cinp.as<C>(), 1, stream);
+      inv_device(cinp.as<C>(), phys.as<T>(), 1);
+      std::vector<T> r(np);
+      HIPCHECK(hipMemcpyAsync(r.data(), phys.p, sizeof(T) * np, hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      double J = 0;
+      for (size_t i = 0; i < np; ++i) { r[i] -= (T)target[i]; J += 0.5 * (double)r[i] * (double)r[i]; }
+      HIPCHECK(hipMemcpyAsync(phys.p, r.data(), sizeof(T) * np, hipMemcpyHostToDevice, stream));
+      fwd_device(phys.as<T>(), lamb.as<C>(), 1, 0);
+      std::vector<C> h(fsz);
+      HIPCHECK(hipMemcpyAsync(h.data(), lamb.p, fb, hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      host_spectral(h, true, 1.0 / (double)np);
+      HIPCHECK(hipMemcpyAsync(lamb.p, h.data(), fb, hipMemcpyHostToDevice, stream));
+      HIPCHECK(hipMemsetAsync(nu_part.p, 0, sizeof(double) * ka3_blocks, stream));
+      // reverse sweep, recomputing every segment but the last
+      for (long s = nseg - 1; s >= 0; --s) {
+        const long n = len(s);
+        if (s != nseg - 1) {
+          HIPCHECK(hipMemcpyAsync(slot(0, 0), ckpts.as<C>() + s * fsz, fb, hipMemcpyDeviceToDevice, stream));
+          for (long i = 0; i < n; ++i) tape_step(i, slot(i + 1, 0));
+        }
+        for (long i = n - 1; i >= 0; --i) adjoint_step(i, dnu != nullptr);
+      }
+      // dJ/dzeta0 = N irfft2(Z M lambda_0 / w)
+      HIPCHECK(hipMemcpyAsync(h.data(), lamb.p, fb, hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      host_spectral(h, false, (double)np);
+      HIPCHECK(hipMemcpyAsync(cinp.p, h.data(), fb, hipMemcpyHostToDevice, stream));
+      inv_device(cinp.as<C>(), phys.as<T>(), 1);
+      if (grad) HIPCHECK(hipMemcpyAsync(grad, phys.p, sizeof(T) * np, hipMemcpyDeviceToHost, stream));
+      std::vector<double> parts(ka3_blocks);
+      HIPCHECK(hipMemcpyAsync(parts.data(), nu_part.p, sizeof(double) * ka3_blocks, hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      if (dnu) { double sum = 0; for (double v : parts) sum += v; *dnu = sum; }
+      if (Jout) *Jout = J;
+    }
+  }
 };
 
 // Ensemble of independent FP64 instances advanced together: per stage, one launch per
@@ -1037,6 +1229,12 @@ int gf_timing_read(gf_ctx* c, int k, int64_t* n, double* ms) {
   if (!n || !ms) return GF_EINVAL;
   return guarded(c, [&] { c->impl->timing_read(k, n, ms); });
 }
+int gf_adjoint_grad(gf_ctx* c, const double* target, int nsteps, int checkpoint_every, double* dJ_dzeta0,
+                    double* dJ_dnu, double* J) {
+  if (!target) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->adjoint_grad(target, nsteps, checkpoint_every, dJ_dzeta0, dJ_dnu, J); });
+}
+
 int gf_kernel_bytes(const gf_ctx* c, int k, int full, double* bytes) {
   if (!c || !bytes || k < 0 || k > 2) return GF_EINVAL;
   *bytes = c->impl->kernel_bytes(k, full);

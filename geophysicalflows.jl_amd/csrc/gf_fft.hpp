@@ -165,6 +165,17 @@ __device__ __forceinline__ void powers(cplx<T> (&w)[R]) {
 __device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
 template <int N> struct LdsN { static constexpr int value = N + N / 8; };
 
+// The y-direction kernels interleave the two columns of a pair across adjacent lanes, each
+// column with its own N-point buffer.  Buffers N + N/8 slots apart start on the same LDS bank,
+// so the two lanes of a pair would collide on every exchange; the second column's buffer is
+// shifted by 64 bytes (half of the 128 bytes the banks cover) to put them on disjoint banks.
+template <typename T> struct ColSkew { static constexpr int value = 64 / (int)sizeof(cplx<T>); };
+// buffer of column c of pair slot in a workgroup's LDS (2 G LdsN<N> + ColSkew slots)
+template <typename T, int N>
+__device__ __forceinline__ cplx<T>* col_buffer(cplx<T>* lds, int slot, int c) {
+  return lds + (2 * slot + c) * LdsN<N>::value + c * ColSkew<T>::value;
+}
+
 // One Stockham pass (index PASS, stride NS) on C transforms, then recurse.
 template <typename T, int N, int P, int C, int PASS, int NS>
 __device__ __forceinline__ void stockham(cplx<T> (&v)[C][P], cplx<T>* __restrict__ lds, int t,

@@ -113,8 +113,60 @@ struct K3Args {
   T dt;
   const T* filter;          // nullable
   int zero_mean;            // E_STORE: set mode (0,0) to zero
+  // adjoint tape (ETDRK4): nsave receives each stage's N, sol_out the new state of
+  // E_ETD4 (both nullable: no tape / update sol in place)
+  cplx<T>* nsave;
+  cplx<T>* sol_out;
   const cplx<T>* tw;
 };
+
+// Adjoint of the TwoDNavierStokes ETDRK4 step (gf_adjoint_impl.hpp).
+template <typename T>
+struct AdjK1Args {
+  Geo<T> g;
+  const cplx<T>* s;         // stage state (masked on read)
+  const cplx<T>* x;         // co-state source; lambda_N = lamc * x (masked on read)
+  const T* lamc;            // real coefficient table (blocked), nullable = 1
+  cplx<T>* out;             // 5 mixed fields: u, v, zeta, pA, pB
+  long fstride;
+  int ncols;                // kmax
+  const cplx<T>* tw;
+};
+
+template <typename T>
+struct AdjK2Args {
+  Geo<T> g;
+  const cplx<T>* in;        // the 5 mixed fields of AdjK1Args
+  long fstride;
+  int kin;
+  cplx<T>* out;             // 3 flux fields: x-spectra of zeta pA, zeta pB, u pA + v pB
+  long ostride;
+  int kout;
+  const cplx<T>* tw;
+};
+
+template <typename T>
+struct AdjK3Args {
+  Geo<T> g;
+  const cplx<T>* in;        // the 3 flux fields of AdjK2Args
+  long fstride;
+  int ncols;                // kmax
+  int stage;                // 4, 3, 2, 1: adjoint of the ETDRK4 stage with that calcN
+  // co-state buffers (blocked): lambda of sol' (in; out at stage 1), partial sums
+  cplx<T>* lam; cplx<T>* l0; cplx<T>* l1; cplx<T>* lN1; cplx<T>* lN2; cplx<T>* lN3;
+  const T* E; const T* E2; const T* Z; const T* Al; const T* Be;   // ETDRK4 coefficients (real)
+  // dJ/dnu: coefficient derivatives times dL/dnu, the step's tape, one slot per workgroup
+  const T* dE; const T* dE2; const T* dZ; const T* dA; const T* dB; const T* dG;
+  const cplx<T>* s0; const cplx<T>* s1;
+  const cplx<T>* N1; const cplx<T>* N2; const cplx<T>* N3; const cplx<T>* N4;
+  T* nu_part;               // nullable: no nu gradient
+  const cplx<T>* tw;
+};
+
+// k = 1, 2, 3: launch KA1/KA2/KA3 for grid size n (square grids) with args of that kernel
+template <typename T> void launch_adj(int k, int n, const void* args, hipStream_t s);
+// workgroups of KA3 (and KA1) for ncols columns at size n
+template <typename T> int adj_blocks(int n, int ncols);
 
 template <typename T> void launch_k1(int ny, const K1Args<T>& a, hipStream_t s);
 template <typename T> void launch_k2(int nx, const K2Args<T>& a, hipStream_t s);

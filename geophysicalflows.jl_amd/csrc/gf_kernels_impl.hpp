@@ -96,7 +96,7 @@ template <typename T> struct Coef {
 #define GF_K2_WAVES 2
 #endif
 #ifndef GF_K3_WAVES
-#define GF_K3_WAVES 3
+#define GF_K3_WAVES 4   // measured faster than 3 despite a small spill (two 512-thread workgroups per CU)
 #endif
 
 template <int MODEL, int FS> struct Fields {
@@ -133,7 +133,7 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
   const bool masked = FS != FS_RAW;
   const bool krk = kr < g.kmax || !masked;
   const T kv = krwave(g, kr);
-  cplx<T>* L = lds + (2 * m.slot + m.c) * LdsN<NY>::value;
+  cplx<T>* L = col_buffer<T, NY>(lds, m.slot, m.c);
   const T nrm = g.inv_norm;
   // masked state value of layer j at y-index l (re-read per field: an L2 hit
   // after the first field, and it keeps the register footprint small)
@@ -213,7 +213,7 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
 template <typename T, int NY, int P, int G, int NL, int MODEL, int FS>
 __global__ __launch_bounds__(2 * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K1_WAVES)))
 void k1_kernel(const K1Args<T> a) {
-  __shared__ cplx<T> lds[2 * G * LdsN<NY>::value];
+  __shared__ cplx<T> lds[2 * G * LdsN<NY>::value + ColSkew<T>::value];
   __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
   load_twiddles<T, NY, P>(tw, a.tw);
   __syncthreads();
@@ -309,8 +309,98 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
         }
       }
     }
+  } else if constexpr (MODE == K2_ADVECT) {
+    // ---- advection without background gradients: u, v, q -> u (q [+ eta]), v (q [+ eta]) ----
+    // (src/twodnavierstokes.jl:262-270, src/singlelayerqg.jl:368-375).  Each inverse transform
+    // packs the two rows of one field, (q0 + i q1), (u0 + i u1), (v0 + i v1), so a phase reads
+    // whole 64-byte pieces (two rows of a column pair) and, with the other row pair of the
+    // workgroup, whole 128-byte lines; products are formed as soon as their factors exist.  The
+    // forward transforms are per row, (u q + i v q); both rows are split before the flux is
+    // stored, so the two rows of a column pair are written together.
+    static_assert(NL == 1, "K2_ADVECT serves single-layer models");
+    const cplx<T>* fu = a.in;
+    const cplx<T>* fv = a.in + a.fstride;
+    const cplx<T>* fq = a.in + 2 * a.fstride;
+    auto inverse_rows = [&](const cplx<T>* f, cplx<T> (&w)[1][P]) {
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int k = t + TPT * q;
+        const cplx<T> A = herm_load<T, NX>(f, k, y0, ny, a.kin, active);
+        const cplx<T> B = herm_load<T, NX>(f, k, y0 + 1, ny, a.kin, active);
+        w[0][q] = mk(A.re - B.im, -(A.im + B.re));     // conj(A + i B)
+      }
+      fft_forward<T, NX, P, 1>(w, L, t, tw);             // row y0 = Re, row y0 + 1 = -Im
+    };
+    T q0[P], q1[P];
+    {
+      cplx<T> w[1][P];
+      inverse_rows(fq, w);
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int x = t + TPT * q;
+        q0[q] = w[0][q].re;
+        q1[q] = -w[0][q].im;
+        if (a.eta != nullptr && active) {
+          q0[q] = q0[q] + at(a.eta, (uint32_t)(y0 * NX + x));
+          q1[q] = q1[q] + at(a.eta, (uint32_t)((y0 + 1) * NX + x));
+        }
+      }
+    }
+    T uq0[P], uq1[P];
+    {
+      cplx<T> w[1][P];
+      inverse_rows(fu, w);
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        uq0[q] = w[0][q].re * q0[q];
+        uq1[q] = -w[0][q].im * q1[q];
+      }
+    }
+    cplx<T> z0[1][P], z1[1][P];
+    {
+      cplx<T> w[1][P];
+      inverse_rows(fv, w);
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        z0[0][q] = mk(uq0[q], w[0][q].re * q0[q]);      // (u q) + i (v q), row y0
+        z1[0][q] = mk(uq1[q], -w[0][q].im * q1[q]);     // row y0 + 1
+      }
+    }
+    // split both rows (kr <= NX/2; every read stays inside the N-point buffer), then store
+    cplx<T> A0[MAXI], B0[MAXI];
+    fft_forward<T, NX, P, 1>(z0, L, t, tw);
+#pragma unroll
+    for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = z0[0][q];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXI; ++i) {
+      const int kr = t + i * TPT;
+      const cplx<T> Wk = L[pad(kr & (NX - 1))], Wm = L[pad((NX - kr) & (NX - 1))];
+      A0[i] = split_row(Wk, Wm, 0);
+      B0[i] = split_row(Wk, Wm, 1);
+    }
+    __syncthreads();
+    fft_forward<T, NX, P, 1>(z1, L, t, tw);
+#pragma unroll
+    for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = z1[0][q];
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < MAXI; ++i) {
+        const int kr = t + i * TPT;
+        if (kr < a.kout) {
+          const cplx<T> Wk = L[pad(kr)], Wm = L[pad((NX - kr) & (NX - 1))];
+          const uint32_t o0 = boff32(kr, y0, ny), o1 = boff32(kr, y0 + 1, ny);
+          at(a.out, o0) = A0[i];
+          at(a.out, o1) = split_row(Wk, Wm, 0);
+          at(a.out + a.ostride, o0) = B0[i];
+          at(a.out + a.ostride, o1) = split_row(Wk, Wm, 1);
+        }
+      }
+    }
+    __syncthreads();
   } else {
-    // ---- advection: per layer, u, v, q -> (u q, v q [, u Qx + v Qy]) ----
+    // ---- advection with background gradients: per layer, u, v, q -> (u q, v q, u Qx + v Qy) ----
     constexpr bool FULL = MODE == K2_ADVECT_FULL;
     constexpr int NF = FULL ? 3 : 2;
 #pragma unroll
@@ -433,8 +523,12 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
   }
 }
 
+#ifndef GF_K2_ADVECT_WAVES
+#define GF_K2_ADVECT_WAVES 4
+#endif
 template <typename T, int NX, int P, int G, int MODE, int NL>
-__global__ __launch_bounds__(G * (NX / P)) __attribute__((amdgpu_waves_per_eu(GF_K2_WAVES)))
+__global__ __launch_bounds__(G * (NX / P))
+__attribute__((amdgpu_waves_per_eu(MODE == K2_ADVECT ? GF_K2_ADVECT_WAVES : GF_K2_WAVES)))
 void k2_kernel(const K2Args<T> a) {
   __shared__ cplx<T> lds[G * LdsN<NX>::value];
   __shared__ cplx<T> tw[TwPlan<NX, P>::size()];
@@ -456,7 +550,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
   const int kr = 2 * (blk * G + m.slot) + m.c;
   const Geo<T>& g = a.g;
   const bool active = kr < a.ncols && kr < g.nkr;
-  cplx<T>* L = lds + (2 * m.slot + m.c) * LdsN<NY>::value;
+  cplx<T>* L = col_buffer<T, NY>(lds, m.slot, m.c);
   const T kv = krwave(g, kr);
 
   if (a.epi == E_STORE) {
@@ -553,6 +647,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       }
       const uint32_t ci = a.cl_layered ? ix : idx;
       const cplx<T> sv = kept ? at(a.sol, ix) : czero<T>();
+      if (a.nsave != nullptr) at(a.nsave, ix) = N;
       switch (a.epi) {
         case E_CALCN:
           at(a.nout, ix) = N;
@@ -587,7 +682,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
           r = r + Be.mul(ci, mk((T)2 * n23v.re, (T)2 * n23v.im));
           r = r + Ga.mul(ci, N);
           if (a.filter) r = rmul(at(a.filter, idx), r);
-          at(a.sol, ix) = r;
+          at(a.sol_out != nullptr ? a.sol_out : a.sol, ix) = r;
         } break;
         case E_RK1:
         case E_RK2:
@@ -630,7 +725,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
 template <typename T, int NY, int P, int G, int NL>
 __global__ __launch_bounds__(2 * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K3_WAVES)))
 void k3_kernel(const K3Args<T> a) {
-  __shared__ cplx<T> lds[2 * G * LdsN<NY>::value];
+  __shared__ cplx<T> lds[2 * G * LdsN<NY>::value + ColSkew<T>::value];
   __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
   load_twiddles<T, NY, P>(tw, a.tw);
   __syncthreads();
@@ -687,7 +782,7 @@ void k2_n(const K2Args<T>& a, hipStream_t s) {
   constexpr int G = GW<N, P>::G;
   const dim3 grid((a.g.ny / 2 + G - 1) / G), blk(G * (N / P));
   switch (a.mode) {
-    case K2_ADVECT: hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT, NL>), grid, blk, 0, s, a); break;
+    case K2_ADVECT: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT, 1>), grid, blk, 0, s, a); break;
     case K2_ADVECT_FULL: hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT_FULL, NL>), grid, blk, 0, s, a); break;
     case K2_INVERSE_OUT: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_INVERSE_OUT, 1>), grid, blk, 0, s, a); break;
     case K2_FORWARD_IN: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_FORWARD_IN, 1>), grid, blk, 0, s, a); break;

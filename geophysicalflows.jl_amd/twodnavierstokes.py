@@ -68,3 +68,19 @@ def enstrophy(prob):
 def palinstrophy(prob):
     g = prob.grid
     return _norm(prob) * parsevalsum(0.5 * g.Krsq * np.abs(prob.sol) ** 2, g)
+
+
+def adjoint_gradient(prob, target, nsteps, checkpoint=50, nu=True):
+    """Loss ``J = 1/2 sum (zeta(T) - target)^2`` after ``nsteps`` steps from the current state and
+    its gradients with respect to the initial vorticity (physical, ``(ny, nx)``) and, if ``nu``,
+    the viscosity: ``(J, dJ/dzeta0, dJ/dnu)``.  The discrete adjoint of the device ETDRK4 step
+    (unforced, FP64); checkpoints every ``checkpoint`` steps (no reference counterpart)."""
+    import ctypes
+    from ._lib import check, lib
+    g = prob.grid
+    t = np.ascontiguousarray(np.asarray(target, dtype=np.float64).reshape(g.ny, g.nx))
+    grad = np.empty_like(t)
+    J, dnu = ctypes.c_double(), ctypes.c_double()
+    check(lib.gf_adjoint_grad(prob._ctx, t.ctypes.data, int(nsteps), int(checkpoint), grad.ctypes.data,
+                              ctypes.byref(dnu) if nu else None, ctypes.byref(J)), prob._ctx)
+    return J.value, grad, (dnu.value if nu else None)

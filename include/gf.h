@@ -159,6 +159,23 @@ int gf_ctx_create_batch(gf_ctx** out, const gf_problem_desc* descs, int n);
 int gf_batch_size(const gf_ctx* ctx);
 int gf_ctx_instance(gf_ctx* ensemble, int i, gf_ctx** member);
 
+/* Gradient of J = 1/2 sum_{x,y} (zeta(T) - target)^2 for the problem's current state as
+ * zeta0 (set with gf_set_physical), where zeta(T) = irfft2(M sol_T) after nsteps ETDRK4 steps.
+ *
+ * No counterpart in the reference, which has no adjoint. It is the discrete adjoint of this
+ * library's own ETDRK4 step, including calcN!'s in-place dealias!.
+ *  - Scope: unforced TwoDNavierStokes, FP64, square grids.
+ *  - The forward pass keeps a checkpoint every checkpoint_every steps. The backward sweep
+ *    recomputes each segment into a device tape of 8 spectral fields per step.
+ *    checkpoint_every >= nsteps tapes the whole run once.
+ *  - Outputs:
+ *      dJ_dzeta0: nx*ny doubles, physical layout, x fastest; nullable;
+ *      dJ_dnu: scalar; nullable, in which case it is skipped;
+ *      J: the loss; nullable.
+ *  - The context's state and clock are left unchanged. */
+int gf_adjoint_grad(gf_ctx* ctx, const double* target, int nsteps, int checkpoint_every,
+                    double* dJ_dzeta0, double* dJ_dnu, double* J);
+
 /* Library version string. */
 const char* gf_version(void);
 

@@ -1,0 +1,77 @@
+"""GPU adjoint of the TwoDNavierStokes ETDRK4 step against the oracle's adjoint
+(oracle/adjoint.py, pinned on the CPU by finite differences and dot-product identities),
+against central differences of the GPU's own loss, and for checkpoint invariance.
+
+Tolerances (FP64): gradient vs oracle 1e-10 relative L2, dJ/dnu 1e-9 relative, J 1e-12;
+central differences (h = 1e-6 on unit-variance directions) 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle import adjoint as oad
+from oracle import fourierflows as off
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(gf, n, nu, seed, scale=0.5):
+    rng = np.random.default_rng(seed)
+    grid = gf.problem.Grid(n, n, 2 * np.pi, 2 * np.pi, 1 / 3, np.float64)
+    z0 = gf.peakedisotropicspectrum(grid, min(6, n // 8), scale, rng=rng)
+    target = gf.peakedisotropicspectrum(grid, min(6, n // 8), scale, rng=rng)
+    prob = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, ν=nu, nν=1, dt=0.005, stepper="ETDRK4")
+    gf.TwoDNavierStokes.set_ζ(prob, z0)
+    return prob, z0, target
+
+
+@pytest.mark.parametrize("n,nsteps,ckpt", [(32, 12, 5), (64, 10, 10), (128, 6, 4)])
+def test_gradient_matches_oracle(gf, n, nsteps, ckpt):
+    prob, z0, target = _setup(gf, n, 1e-3, 40 + n)
+    J, grad, dnu = gf.TwoDNavierStokes.adjoint_gradient(prob, target, nsteps, ckpt)
+    g = off.TwoDGrid(n, 2 * np.pi)
+    Jr, gr, dnur = oad.loss_and_gradient(z0, target, g, 1e-3, 1, 0.005, nsteps, checkpoint=ckpt)
+    assert J == pytest.approx(Jr, rel=1e-12)
+    assert np.linalg.norm(grad - gr) <= 1e-10 * np.linalg.norm(gr)
+    assert dnu == pytest.approx(dnur, rel=1e-9)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(gf):
+    prob, z0, target = _setup(gf, 64, 1e-3, 5)
+    base = gf.TwoDNavierStokes.adjoint_gradient(prob, target, 13, 13)
+    for ckpt in (1, 4, 6):
+        other = gf.TwoDNavierStokes.adjoint_gradient(prob, target, 13, ckpt)
+        assert other[0] == base[0]
+        assert np.array_equal(other[1], base[1])
+        assert other[2] == base[2]
+
+
+def test_gradient_matches_central_differences(gf):
+    n, nsteps = 128, 20
+    prob, z0, target = _setup(gf, n, 2e-3, 6)
+    J, grad, dnu = gf.TwoDNavierStokes.adjoint_gradient(prob, target, nsteps, 8)
+    rng = np.random.default_rng(7)
+    h = 1e-6
+    for _ in range(3):
+        v = rng.standard_normal((n, n))
+        Js = []
+        for sgn in (1, -1):
+            gf.TwoDNavierStokes.set_ζ(prob, z0 + sgn * h * v)
+            Js.append(gf.TwoDNavierStokes.adjoint_gradient(prob, target, nsteps, nsteps, nu=False)[0])
+        fd = (Js[0] - Js[1]) / (2 * h)
+        assert fd == pytest.approx(float(np.sum(grad * v)), rel=1e-6)
+
+
+def test_state_unchanged_and_unsupported_problems(gf):
+    prob, z0, target = _setup(gf, 64, 1e-3, 8)
+    before = prob.sol
+    gf.TwoDNavierStokes.adjoint_gradient(prob, target, 5, 2)
+    assert np.array_equal(prob.sol, before)
+    assert prob.clock.step == 0
+    rk = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, ν=1e-3, dt=0.005, stepper="RK4")
+    with pytest.raises(gf._lib.GFError) as e:
+        gf.TwoDNavierStokes.adjoint_gradient(rk, target, 5)
+    assert e.value.code == gf._lib.GF_EUNSUPPORTED
+    q = gf.SingleLayerQG.Problem(gf.GPU(), nx=64, dt=0.005, stepper="ETDRK4")
+    with pytest.raises(gf._lib.GFError) as e:
+        gf.TwoDNavierStokes.adjoint_gradient(q, target, 5)
+    assert e.value.code == gf._lib.GF_EUNSUPPORTED
