@@ -28,6 +28,23 @@ HBM_PEAK_GBS = 8000.0
 NX = 2048
 CFG = dict(nx=NX, Lx=2 * np.pi, nu=1e-21, nnu=4, mu=0.1, nmu=0, dt=0.005, kf=14.0, dkf=1.5, eps=0.1, seed=2)
 KERNELS = ("y_inverse_K1", "x_pass_K2", "y_forward_stage_K3")
+# kernel symbols of the FP64 2048^2 headline path, as rocprofv3 names them
+SYMBOLS = {"y_inverse_K1": "k1_kernel<double, 2048, 8, 1, 1, 0, 0>",
+           "x_pass_K2": "k2_kernel<double, 2048, 8, 2, 0, 1>",
+           "y_forward_stage_K3": "k3_kernel<double, 2048, 8, 1, 1>"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "twodns_2048_fp64_etdrk4_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of each headline kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (scaled by the copy-kernel calibration); {} if it is absent."""
+    try:
+        txt = open(TRAFFIC_FILE).read()
+        block = txt.split("```json", 1)[1].split("```", 1)[0]
+        tb = json.loads(block)["traffic_bytes"]
+        return {k: tb[s] for k, s in SYMBOLS.items() if s in tb}
+    except (OSError, IndexError, KeyError, ValueError):
+        return {}
 
 
 def gpu_run(gf, steps, warmup, T, dt=CFG["dt"]):
@@ -285,6 +302,9 @@ def main():
     value = NX * NX * steps / elapsed
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
     dk = kern[dom]
+    traffic = measured_traffic()
+    for k in kern:
+        kern[k]["hbm_traffic_bytes"] = traffic.get(k)
     S = 16.0 * (NX // 2 + 1) * NX
     step_bytes = 64 * S
     out = {
@@ -305,7 +325,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["achieved_GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dk["achieved_GBps"] / HBM_PEAK_GBS,
-                     "bytes_per_launch": dk["bytes_per_launch"], "avg_us": dk["avg_us"], "traffic": None},
+                     "bytes_per_launch": dk["bytes_per_launch"], "avg_us": dk["avg_us"], "traffic": traffic.get(dom),
+                     "traffic_source": "profiles/twodns_2048_fp64_etdrk4_hbm_traffic.md" if traffic else None},
         "kernels": kern,
         "step_model": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
                        "frac_of_peak": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},

@@ -76,7 +76,7 @@ __device__ __forceinline__ void ka1_body(const AdjK1Args<T>& a, int blk, cplx<T>
 template <typename T, int NY, int P, int G>
 __global__ __launch_bounds__(2 * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K1_WAVES)))
 void ka1_kernel(const AdjK1Args<T> a) {
-  __shared__ cplx<T> lds[2 * G * LdsN<NY>::value + ColSkew<T>::value];
+  __shared__ cplx<T> lds[G * PairLds<T, NY>::value];
   __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
   load_twiddles<T, NY, P>(tw, a.tw);
   __syncthreads();
@@ -283,7 +283,7 @@ __device__ __forceinline__ void ka3_body(const AdjK3Args<T>& a, int blk, cplx<T>
 template <typename T, int NY, int P, int G>
 __global__ __launch_bounds__(2 * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(2)))
 void ka3_kernel(const AdjK3Args<T> a) {
-  __shared__ cplx<T> lds[2 * G * LdsN<NY>::value + ColSkew<T>::value];
+  __shared__ cplx<T> lds[G * PairLds<T, NY>::value];
   __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
   load_twiddles<T, NY, P>(tw, a.tw);
   __syncthreads();

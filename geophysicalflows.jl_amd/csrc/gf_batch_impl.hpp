@@ -16,8 +16,9 @@ namespace gf {
 
 constexpr int BATCH_THREADS = 256;
 constexpr int BATCH_P = 8;
-// 2 G1 LdsN<N> = G2 LdsN<N> for every N, plus the column skew of the y-direction bodies
-constexpr int BATCH_LDS = BATCH_THREADS * BATCH_P * 9 / 8 + 8;
+// G1 PairLds<N> (2 G1 LdsN<N> plus one column skew per pair) and G2 LdsN<N>: at most
+// 2304 + 32 skews of 4 slots (N = 32) for every N
+constexpr int BATCH_LDS = BATCH_THREADS * BATCH_P * 9 / 8 + 32 * 4;
 constexpr int BATCH_TW = 128;
 
 template <int N> struct BatchGeom {
@@ -25,7 +26,7 @@ template <int N> struct BatchGeom {
   static constexpr int G1 = BATCH_THREADS / (2 * TPT);   // column pairs per K1/K3 workgroup
   static constexpr int G2 = BATCH_THREADS / TPT;         // row pairs per K2 workgroup
   static_assert(PY<N>::P == BATCH_P, "batched sizes use 8 points per thread");
-  static_assert(2 * G1 * LdsN<N>::value + ColSkew<double>::value <= BATCH_LDS && G2 * LdsN<N>::value <= BATCH_LDS, "LDS");
+  static_assert(G1 * PairLds<double, N>::value <= BATCH_LDS && G2 * LdsN<N>::value <= BATCH_LDS, "LDS");
   static_assert(TwPlan<N, BATCH_P>::size() <= BATCH_TW, "twiddle table");
 };
 

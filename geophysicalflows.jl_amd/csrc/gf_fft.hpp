@@ -170,10 +170,12 @@ template <int N> struct LdsN { static constexpr int value = N + N / 8; };
 // so the two lanes of a pair would collide on every exchange; the second column's buffer is
 // shifted by 64 bytes (half of the 128 bytes the banks cover) to put them on disjoint banks.
 template <typename T> struct ColSkew { static constexpr int value = 64 / (int)sizeof(cplx<T>); };
-// buffer of column c of pair slot in a workgroup's LDS (2 G LdsN<N> + ColSkew slots)
+// Slots one column pair occupies (two buffers and the skew; pairs never overlap), and the
+// buffer of column c of pair slot in a workgroup's LDS of G PairLds<T, N> slots.
+template <typename T, int N> struct PairLds { static constexpr int value = 2 * LdsN<N>::value + ColSkew<T>::value; };
 template <typename T, int N>
 __device__ __forceinline__ cplx<T>* col_buffer(cplx<T>* lds, int slot, int c) {
-  return lds + (2 * slot + c) * LdsN<N>::value + c * ColSkew<T>::value;
+  return lds + slot * PairLds<T, N>::value + c * (LdsN<N>::value + ColSkew<T>::value);
 }
 
 // One Stockham pass (index PASS, stride NS) on C transforms, then recurse.

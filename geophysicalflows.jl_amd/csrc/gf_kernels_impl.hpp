@@ -213,7 +213,7 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
 template <typename T, int NY, int P, int G, int NL, int MODEL, int FS>
 __global__ __launch_bounds__(2 * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K1_WAVES)))
 void k1_kernel(const K1Args<T> a) {
-  __shared__ cplx<T> lds[2 * G * LdsN<NY>::value + ColSkew<T>::value];
+  __shared__ cplx<T> lds[G * PairLds<T, NY>::value];
   __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
   load_twiddles<T, NY, P>(tw, a.tw);
   __syncthreads();
@@ -524,7 +524,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
 }
 
 #ifndef GF_K2_ADVECT_WAVES
-#define GF_K2_ADVECT_WAVES 4
+#define GF_K2_ADVECT_WAVES 2   // 181 VGPRs, no spill: 4 waves spills ~300 B and doubles K2's HBM traffic
 #endif
 template <typename T, int NX, int P, int G, int MODE, int NL>
 __global__ __launch_bounds__(G * (NX / P))
@@ -725,7 +725,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
 template <typename T, int NY, int P, int G, int NL>
 __global__ __launch_bounds__(2 * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K3_WAVES)))
 void k3_kernel(const K3Args<T> a) {
-  __shared__ cplx<T> lds[2 * G * LdsN<NY>::value + ColSkew<T>::value];
+  __shared__ cplx<T> lds[G * PairLds<T, NY>::value];
   __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
   load_twiddles<T, NY, P>(tw, a.tw);
   __syncthreads();
