@@ -180,7 +180,7 @@ void ka2_kernel(const AdjK2Args<T> a) {
   __shared__ cplx<T> tw[TwPlan<NX, P>::size()];
   load_twiddles<T, NX, P>(tw, a.tw);
   __syncthreads();
-  ka2_body<T, NX, P, G>(a, blockIdx.x, lds, tw);
+  ka2_body<T, NX, P, G>(a, xcd_block(blockIdx.x, gridDim.x), lds, tw);
 }
 
 // ---------------------------------------------------------------------------

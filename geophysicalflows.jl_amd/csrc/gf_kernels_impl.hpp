@@ -242,6 +242,14 @@ __device__ __forceinline__ cplx<T> split_row(cplx<T> Wk, cplx<T> Wm, int row) {
                   : mk((Wk.im + Wm.im) / 2, (Wm.re - Wk.re) / 2);
 }
 
+// Workgroups are handed to the 8 XCDs in turn (workgroup b to XCD b mod 8), each XCD with its
+// own L2.  Mapping workgroup b to block (b mod 8) nb/8 + b/8 gives every XCD a contiguous range
+// of row pairs, so the row pairs that share the 128-byte lines of the blocked layout (four rows
+// of a column pair) run together on one L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+}
+
 // Body of K2 for block blk of row pairs; lds holds G padded N-point buffers.
 template <typename T, int NX, int P, int G, int MODE, int NL>
 __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __restrict__ lds,
@@ -311,12 +319,11 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
     }
   } else if constexpr (MODE == K2_ADVECT) {
     // ---- advection without background gradients: u, v, q -> u (q [+ eta]), v (q [+ eta]) ----
-    // (src/twodnavierstokes.jl:262-270, src/singlelayerqg.jl:368-375).  Each inverse transform
-    // packs the two rows of one field, (q0 + i q1), (u0 + i u1), (v0 + i v1), so a phase reads
-    // whole 64-byte pieces (two rows of a column pair) and, with the other row pair of the
-    // workgroup, whole 128-byte lines; products are formed as soon as their factors exist.  The
-    // forward transforms are per row, (u q + i v q); both rows are split before the flux is
-    // stored, so the two rows of a column pair are written together.
+    // (src/twodnavierstokes.jl:262-270, src/singlelayerqg.jl:368-375).  Every transform packs
+    // the two rows of one field: inverse (q0 + i q1), (u0 + i u1), (v0 + i v1) and forward
+    // (u0 q0 + i u1 q1), (v0 q0 + i v1 q1).  A phase thus reads and writes whole 64-byte pieces
+    // (two rows of a column pair; with the workgroup's other row pair, whole 128-byte lines), and
+    // only q stays live across phases.
     static_assert(NL == 1, "K2_ADVECT serves single-layer models");
     const cplx<T>* fu = a.in;
     const cplx<T>* fv = a.in + a.fstride;
@@ -330,6 +337,24 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
         w[0][q] = mk(A.re - B.im, -(A.im + B.re));     // conj(A + i B)
       }
       fft_forward<T, NX, P, 1>(w, L, t, tw);             // row y0 = Re, row y0 + 1 = -Im
+    };
+    auto forward_rows = [&](cplx<T>* o, cplx<T> (&w)[1][P]) {
+      fft_forward<T, NX, P, 1>(w, L, t, tw);             // of (row y0 + i row y0 + 1)
+#pragma unroll
+      for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = w[0][q];
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int i = 0; i < MAXI; ++i) {
+          const int kr = t + i * TPT;
+          if (kr < a.kout) {
+            const cplx<T> Wk = L[pad(kr)], Wm = L[pad((NX - kr) & (NX - 1))];
+            at(o, boff32(kr, y0, ny)) = split_row(Wk, Wm, 0);
+            at(o, boff32(kr, y0 + 1, ny)) = split_row(Wk, Wm, 1);
+          }
+        }
+      }
+      __syncthreads();
     };
     T q0[P], q1[P];
     {
@@ -346,59 +371,20 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
         }
       }
     }
-    T uq0[P], uq1[P];
     {
       cplx<T> w[1][P];
       inverse_rows(fu, w);
 #pragma unroll
-      for (int q = 0; q < P; ++q) {
-        uq0[q] = w[0][q].re * q0[q];
-        uq1[q] = -w[0][q].im * q1[q];
-      }
+      for (int q = 0; q < P; ++q) w[0][q] = mk(w[0][q].re * q0[q], -w[0][q].im * q1[q]);   // u q, both rows
+      forward_rows(a.out, w);
     }
-    cplx<T> z0[1][P], z1[1][P];
     {
       cplx<T> w[1][P];
       inverse_rows(fv, w);
 #pragma unroll
-      for (int q = 0; q < P; ++q) {
-        z0[0][q] = mk(uq0[q], w[0][q].re * q0[q]);      // (u q) + i (v q), row y0
-        z1[0][q] = mk(uq1[q], -w[0][q].im * q1[q]);     // row y0 + 1
-      }
+      for (int q = 0; q < P; ++q) w[0][q] = mk(w[0][q].re * q0[q], -w[0][q].im * q1[q]);   // v q, both rows
+      forward_rows(a.out + a.ostride, w);
     }
-    // split both rows (kr <= NX/2; every read stays inside the N-point buffer), then store
-    cplx<T> A0[MAXI], B0[MAXI];
-    fft_forward<T, NX, P, 1>(z0, L, t, tw);
-#pragma unroll
-    for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = z0[0][q];
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < MAXI; ++i) {
-      const int kr = t + i * TPT;
-      const cplx<T> Wk = L[pad(kr & (NX - 1))], Wm = L[pad((NX - kr) & (NX - 1))];
-      A0[i] = split_row(Wk, Wm, 0);
-      B0[i] = split_row(Wk, Wm, 1);
-    }
-    __syncthreads();
-    fft_forward<T, NX, P, 1>(z1, L, t, tw);
-#pragma unroll
-    for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = z1[0][q];
-    __syncthreads();
-    if (active) {
-#pragma unroll
-      for (int i = 0; i < MAXI; ++i) {
-        const int kr = t + i * TPT;
-        if (kr < a.kout) {
-          const cplx<T> Wk = L[pad(kr)], Wm = L[pad((NX - kr) & (NX - 1))];
-          const uint32_t o0 = boff32(kr, y0, ny), o1 = boff32(kr, y0 + 1, ny);
-          at(a.out, o0) = A0[i];
-          at(a.out, o1) = split_row(Wk, Wm, 0);
-          at(a.out + a.ostride, o0) = B0[i];
-          at(a.out + a.ostride, o1) = split_row(Wk, Wm, 1);
-        }
-      }
-    }
-    __syncthreads();
   } else {
     // ---- advection with background gradients: per layer, u, v, q -> (u q, v q, u Qx + v Qy) ----
     constexpr bool FULL = MODE == K2_ADVECT_FULL;
@@ -524,7 +510,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
 }
 
 #ifndef GF_K2_ADVECT_WAVES
-#define GF_K2_ADVECT_WAVES 2   // 181 VGPRs, no spill: 4 waves spills ~300 B and doubles K2's HBM traffic
+#define GF_K2_ADVECT_WAVES 3   // 147 VGPRs, no spill; 4 waves spills and adds scratch traffic
 #endif
 template <typename T, int NX, int P, int G, int MODE, int NL>
 __global__ __launch_bounds__(G * (NX / P))
@@ -534,7 +520,7 @@ void k2_kernel(const K2Args<T> a) {
   __shared__ cplx<T> tw[TwPlan<NX, P>::size()];
   load_twiddles<T, NX, P>(tw, a.tw);
   __syncthreads();
-  k2_body<T, NX, P, G, MODE, NL>(a, blockIdx.x, lds, tw);
+  k2_body<T, NX, P, G, MODE, NL>(a, xcd_block(blockIdx.x, gridDim.x), lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -718,6 +704,9 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
         } break;
         default: break;
       }
+      // one point's loads and stores at a time: keeps the epilogue within 128 VGPRs (two
+      // 512-thread workgroups per CU) with less spilling than eight points in flight
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
@@ -739,12 +728,12 @@ void k3_kernel(const K3Args<T> a) {
 #define GF_P2048 8
 #endif
 template <int N> struct PY { static constexpr int P = N >= 4096 ? 16 : (N == 2048 ? GF_P2048 : (N >= 32 ? 8 : 4)); };
-// K2: G thread groups of N/P threads (one row pair each), at least one wave per workgroup and
-// at least GF_K2_MIN_G row pairs, so that with GF_K2_MIN_G = 2 one workgroup covers the four
-// rows of every 128-byte line of a column pair in the blocked layout (no line is shared
-// between workgroups, which may sit on different XCDs with separate L2s).
+// K2: G thread groups of N/P threads (one row pair each), at least one wave per workgroup.  The
+// two row pairs that share a 128-byte line of the blocked layout run in neighbouring workgroups
+// on one XCD (xcd_block), so one row pair per workgroup (three 256-thread workgroups per CU at
+// 2048) reads each line from HBM once.
 #ifndef GF_K2_MIN_G
-#define GF_K2_MIN_G 2
+#define GF_K2_MIN_G 1
 #endif
 template <int N, int P> struct GW {
   static constexpr int G0 = (N / P) >= 64 ? 1 : 64 / (N / P);
