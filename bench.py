@@ -226,6 +226,61 @@ def adjoint_run(gf, nsteps=2000, checkpoint=50):
     return out
 
 
+def adjoint_batch_run(gf, nb=16, nsteps=2000, checkpoint=50, fd_dirs=8):
+    """Config 5 as a batch: 16 independent 512^2 problems (own contexts and streams), their
+    2000-step gradients computed concurrently from 16 host threads, against the same 16 forward
+    runs done concurrently.  Then a central-difference check of one gradient on fd_dirs random
+    directions (h = 1e-4; the loss is O(1e7), so smaller steps drown in rounding)."""
+    from concurrent.futures import ThreadPoolExecutor
+    n = 512
+    grid = gf.problem.Grid(n, n, 2 * np.pi, 2 * np.pi, 1 / 3, np.float64)
+    probs, z0s, targets = [], [], []
+    for b in range(nb):
+        rng = np.random.default_rng(500 + b)
+        z0s.append(gf.peakedisotropicspectrum(grid, 6, 0.5, rng=rng))
+        targets.append(gf.peakedisotropicspectrum(grid, 6, 0.5, rng=rng))
+        probs.append(gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, ν=1e-3, nν=1, dt=0.005, stepper="ETDRK4"))
+    pool = ThreadPoolExecutor(max_workers=nb)
+
+    def fwd(b):
+        gf.TwoDNavierStokes.set_ζ(probs[b], z0s[b])
+        gf.stepforward(probs[b], nsteps)
+        probs[b].synchronize()
+
+    def grad(b):
+        gf.TwoDNavierStokes.set_ζ(probs[b], z0s[b])
+        return gf.TwoDNavierStokes.adjoint_gradient(probs[b], targets[b], nsteps, checkpoint)
+
+    list(pool.map(lambda b: gf.TwoDNavierStokes.adjoint_gradient(probs[b], targets[b], 20, 20), range(nb)))
+    t0 = time.perf_counter()
+    list(pool.map(fwd, range(nb)))
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = list(pool.map(grad, range(nb)))
+    tg = time.perf_counter() - t0
+    pool.shutdown()
+    # central differences on one member of the batch
+    p, z0, tgt = probs[0], z0s[0], targets[0]
+    J, g0, _ = res[0]
+    rng = np.random.default_rng(7)
+    h, worst = 1e-4, 0.0
+    for _ in range(fd_dirs):
+        v = rng.standard_normal((n, n))
+        Js = []
+        for sgn in (1, -1):
+            gf.TwoDNavierStokes.set_ζ(p, z0 + sgn * h * v)
+            gf.stepforward(p, nsteps)
+            gf.TwoDNavierStokes.updatevars(p)
+            Js.append(0.5 * float(np.sum((p.vars.ζ - tgt) ** 2)))
+        fd = (Js[0] - Js[1]) / (2 * h)
+        worst = max(worst, abs(fd - float(np.sum(g0 * v))) / abs(fd))
+    return {"workload": f"{nb} independent adjoint TwoDNavierStokes 512^2 FP64 ETDRK4 problems, 2000 steps, "
+                        f"checkpoints every {checkpoint}, gradients run concurrently (one context and stream each)",
+            "forward_batch_s": tf, "gradient_batch_s": tg, "adjoint_over_forward": tg / tf,
+            "fd_check": {"directions": fd_dirs, "h": h, "max_rel_error": worst},
+            "grads_finite": all(bool(np.isfinite(r[1]).all()) for r in res)}
+
+
 def cpu_baseline(nsteps=3):
     """The oracle (NumPy + SciPy pocketfft with one worker per core) on the same workload."""
     from oracle import fourierflows as off
@@ -284,7 +339,8 @@ def main():
         return
     if args.only_adjoint:
         import __graft_entry__ as ge
-        print(json.dumps({"adjoint_512": adjoint_run(ge.load_package())}), flush=True)
+        gf = ge.load_package()
+        print(json.dumps({"adjoint_512": adjoint_run(gf), "adjoint_512_batch16": adjoint_batch_run(gf)}), flush=True)
         return
     if args.only_ensemble:
         import __graft_entry__ as ge
@@ -342,6 +398,7 @@ def main():
         out["mlqg_512x3"] = mlqg_run(gf, 200)
         out["decaying_128"] = decaying_run(gf)
         out["adjoint_512"] = adjoint_run(gf)
+        out["adjoint_512_batch16"] = adjoint_batch_run(gf)
     if not args.no_fp32:
         e32, k32, f32 = gpu_run(gf, steps, warmup, np.float32, dt=0.001)
         s32 = 8.0 * (NX // 2 + 1) * NX

@@ -16,13 +16,16 @@ GF_F64, GF_F32 = 0, 1
 STEPPERS = {"ETDRK4": 0, "FilteredETDRK4": 1, "RK4": 2, "FilteredRK4": 3,
             "ForwardEuler": 4, "FilteredForwardEuler": 5}
 GF_FORCING_NONE, GF_FORCING_FIXED, GF_FORCING_RING = 0, 1, 2
+(GF_DIAG_ENERGY, GF_DIAG_KINETIC_ENERGY, GF_DIAG_POTENTIAL_ENERGY, GF_DIAG_ENSTROPHY, GF_DIAG_PALINSTROPHY,
+ GF_DIAG_ENERGY_DISSIPATION, GF_DIAG_ENSTROPHY_DISSIPATION, GF_DIAG_ENERGY_DRAG, GF_DIAG_ENSTROPHY_DRAG,
+ GF_DIAG_ENERGY_WORK, GF_DIAG_ENSTROPHY_WORK, GF_DIAG_LAYER_ENERGIES) = range(12)
 
 EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string",
            "gf_set_state", "gf_get_state", "gf_set_state_dev", "gf_get_state_dev",
            "gf_set_physical", "gf_updatevars", "gf_num_vars", "gf_calcN", "gf_set_forcing",
            "gf_step", "gf_clock", "gf_set_clock", "gf_synchronize", "gf_timing",
            "gf_timing_read", "gf_kernel_bytes", "gf_version", "gf_ctx_create_batch",
-           "gf_batch_size", "gf_ctx_instance", "gf_adjoint_grad")
+           "gf_batch_size", "gf_ctx_instance", "gf_adjoint_grad", "gf_diagnostic")
 
 
 class ProblemDesc(ctypes.Structure):
@@ -85,6 +88,7 @@ def _load():
     lib.gf_batch_size.argtypes = [vp]
     lib.gf_ctx_instance.argtypes = [vp, ip, ctypes.POINTER(vp)]
     lib.gf_adjoint_grad.argtypes = [vp, vp, ip, ip, vp, dp, dp]
+    lib.gf_diagnostic.argtypes = [vp, ip, ctypes.c_double, ip, dp, ip]
     for name in EXPORTS:
         getattr(lib, name)
     return lib

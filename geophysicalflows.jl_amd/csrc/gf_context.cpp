@@ -176,6 +176,9 @@ struct CtxBase {
   virtual int ninstances() const { return 1; }
   virtual CtxBase* instance(int i) { return i == 0 ? this : nullptr; }
   virtual void set_clock(double tt, int64_t s) { t = tt; stepno = s; }
+  virtual int diagnostic(int, double, int, double*, int) {
+    throw GfError(GF_EINVAL, "diagnostics are computed per problem; address one member through gf_ctx_instance");
+  }
   virtual void adjoint_grad(const double*, int, int, double*, double*, double*) {
     throw GfError(GF_EUNSUPPORTED, "the adjoint is provided for TwoDNavierStokes FP64 ETDRK4 problems");
   }
@@ -200,6 +203,9 @@ struct Ctx final : CtxBase {
   // device buffers
   DevBuf sol, s1, s2, n1, n23, racc, nout, mixed, flux, twx, twy, phys, jl, cinp;
   DevBuf coef[6], filter, eta, etah, etax, etay, Uy, Qy1, sinv, Fh, sqrt_spec;
+  DevBuf prevsol, diag_part;
+  bool prevsol_valid = false;
+  std::vector<double> Hlay, gprime;   // MLQG layer depths and reduced gravities
   KernelTimer tm;
 
   bool own_stream = true;
@@ -374,19 +380,17 @@ struct Ctx final : CtxBase {
           Lop[(size_t)l * nkr + kr] = L;
         }
       Lop[0] = 0;
+      if (has_eta) {
+        // etah = rfft(eta): the -i kr U etah term and the enstrophy (src/singlelayerqg.jl:190)
+        std::vector<T> ft(np);
+        for (size_t i = 0; i < np; ++i) ft[i] = (T)etav[i];
+        HIPCHECK(hipMemcpy(phys.p, ft.data(), sizeof(T) * np, hipMemcpyHostToDevice));
+        etah.alloc(sizeof(C) * fsz);
+        fwd_device(phys.as<T>(), etah.as<C>(), 1, 0);
+        HIPCHECK(hipStreamSynchronize(stream));
+      }
       if (!varyingU) {
-        if (has_eta) {
-          upload_real(eta, etav);
-          if (d.U != 0) {
-            // etah = rfft(eta) for the -i kr U etah term
-            std::vector<T> ft(np);
-            for (size_t i = 0; i < np; ++i) ft[i] = (T)etav[i];
-            HIPCHECK(hipMemcpy(phys.p, ft.data(), sizeof(T) * np, hipMemcpyHostToDevice));
-            etah.alloc(sizeof(C) * fsz);
-            fwd_device(phys.as<T>(), etah.as<C>(), 1, 0);
-            HIPCHECK(hipStreamSynchronize(stream));
-          }
-        }
+        if (has_eta) upload_real(eta, etav);
       } else {
         // varying U(y): Qx = d eta/dx, Qy = d eta/dy - U_yy (src/singlelayerqg.jl:182-189)
         advect_full = true;
@@ -408,11 +412,13 @@ struct Ctx final : CtxBase {
       for (int kr = 0; kr < nkr; ++kr) Lop[(size_t)l * nkr + kr] = cd(-d.nu * ipow(krsq(kr, l), d.nnu), 0);
     Lop[0] = 0;
     std::vector<double> H(d.H, d.H + n), b(d.b, d.b + n), gp(n - 1), Fm(n - 1), Fp(n - 1);
+    Hlay = H;
     for (int j = 0; j < n - 1; ++j) {
       gp[j] = b[j] - b[j + 1];
       Fm[j] = d.f0 * d.f0 / (gp[j] * H[j + 1]);
       Fp[j] = d.f0 * d.f0 / (gp[j] * H[j]);
     }
+    gprime = gp;
     std::vector<double> F((size_t)n * n, 0.0);
     for (int j = 0; j < n; ++j) {
       F[j * n + j] = -((j < n - 1 ? Fp[j] : 0.0) + (j > 0 ? Fm[j - 1] : 0.0));
@@ -567,7 +573,8 @@ struct Ctx final : CtxBase {
     c = K3Args<T>{};
     c.g = g; c.model = d.model; c.in = flux.as<C>(); c.fstride = fsz; c.ncols = full ? g.nkr : g.kmax;
     c.full = full; c.epi = epi; c.apply_kr = !advect_full;
-    c.etah = etah.as<C>(); c.Ueta = (T)d.U;
+    c.etah = (d.model == GF_MODEL_SLQG && !advect_full && d.U != 0) ? etah.as<C>() : nullptr;
+    c.Ueta = (T)d.U;
     c.Fh = (d.forcing == GF_FORCING_FIXED) ? Fh.as<C>() : nullptr;
     c.sqrt_spec = (d.forcing == GF_FORCING_RING && fkx >= 0) ? sqrt_spec.as<T>() : nullptr;
     c.seed = d.seed; c.fstep = stepno; c.sqrt_dt = (T)std::sqrt(d.dt); c.fkx = fkx; c.fky = fky;
@@ -611,6 +618,13 @@ struct Ctx final : CtxBase {
     const int ns = stages(ins, epis);
     for (int s = 0; s < n; ++s) {
       const bool full = s == n - 1;
+      // the work diagnostics of stochastic forcing use the state at the start of the step
+      // (prevsol, src/twodnavierstokes.jl:311-319); kept for the last step of the call
+      if (full && d.forcing == GF_FORCING_RING) {
+        if (!prevsol.p) prevsol.alloc(sizeof(C) * fsz * nl);
+        HIPCHECK(hipMemcpyAsync(prevsol.p, sol.p, sizeof(C) * fsz * nl, hipMemcpyDeviceToDevice, stream));
+        prevsol_valid = true;
+      }
       for (int k = 0; k < ns; ++k) calcN_launch(ins[k], epis[k], full);
       t += d.dt;
       stepno += 1;
@@ -724,9 +738,72 @@ struct Ctx final : CtxBase {
   }
   void sync() override { HIPCHECK(hipStreamSynchronize(stream)); }
 
+  // ---------------- diagnostics ----------------
+  int diagnostic(int kind, double xi, int nxi, double* out, int nout) override {
+    DiagArgs<T> a{};
+    a.g = g; a.kind = kind; a.model = d.model; a.nl = nl; a.sol = sol.as<C>();
+    a.nout = 1;
+    const double np2 = (double)d.nx * d.ny * (double)d.nx * d.ny;
+    double scale = 1.0 / np2;                               // 1/(Lx Ly) times parsevalsum's Lx Ly/(nx ny)^2
+    const bool slqg = d.model == GF_MODEL_SLQG, twodns = d.model == GF_MODEL_TWODNS;
+    auto bad = [&](const char* m) { throw GfError(GF_EUNSUPPORTED, m); };
+    if (d.model == GF_MODEL_MLQG) {
+      if (kind != DIAG_LAYER_ENERGIES) bad("MultiLayerQG provides GF_DIAG_LAYER_ENERGIES");
+      a.nout = 2 * nl - 1;
+      double Hs = 0;
+      for (double h : Hlay) Hs += h;
+      const double V = d.Lx * d.Ly * Hs;
+      for (int j = 0; j < nl; ++j) a.lay_ke[j] = Hlay[j] / (2 * V);
+      for (int j = 0; j + 1 < nl; ++j) a.lay_pe[j] = d.f0 * d.f0 / (2 * V * gprime[j]);
+      a.sinv = sinv.as<T>();
+      scale = d.Lx * d.Ly / np2;
+    } else {
+      switch (kind) {
+        case DIAG_ENERGY: case DIAG_ENSTROPHY: break;
+        case DIAG_KINETIC_ENERGY: case DIAG_POTENTIAL_ENERGY: if (!slqg) bad("kinetic/potential energy are SingleLayerQG diagnostics"); break;
+        case DIAG_PALINSTROPHY: if (!twodns) bad("palinstrophy is a TwoDNavierStokes diagnostic"); break;
+        case DIAG_ENERGY_DISSIPATION: case DIAG_ENSTROPHY_DISSIPATION:
+        case DIAG_ENERGY_DRAG: case DIAG_ENSTROPHY_DRAG:
+          a.xi = (T)xi; a.nxi = nxi;
+          if (slqg && eb) bad("not implemented for finite deformation radius (src/singlelayerqg.jl)");
+          break;
+        case DIAG_ENERGY_WORK: case DIAG_ENSTROPHY_WORK:
+          if (slqg && eb) bad("not implemented for finite deformation radius (src/singlelayerqg.jl)");
+          if (d.forcing == GF_FORCING_NONE) bad("the problem has no forcing");
+          if (d.forcing == GF_FORCING_FIXED) a.Fh = Fh.as<C>();
+          if (d.forcing == GF_FORCING_RING) {
+            if (!prevsol_valid) throw GfError(GF_EINVAL, "work of stochastic forcing needs a step taken first");
+            a.prevsol = prevsol.as<C>();
+            a.sqrt_spec = fkx >= 0 ? sqrt_spec.as<T>() : nullptr;
+            a.seed = d.seed; a.fstep = stepno - 1; a.sqrt_dt = (T)std::sqrt(d.dt);
+            a.fkx = fkx; a.fky = fky; a.instance = d.instance;
+          }
+          break;
+        default: throw GfError(GF_EINVAL, "unknown or inapplicable diagnostic kind");
+      }
+      a.etah = (slqg && has_eta) ? etah.as<C>() : nullptr;
+      a.eb = eb; a.ell2inv = (T)ell2inv;
+    }
+    if (nout < a.nout) throw GfError(GF_EINVAL, "output array too small for this diagnostic");
+    const int nblocks = (int)std::min<long>(1024, (fsz + 255) / 256);
+    if (!diag_part.p) diag_part.alloc(sizeof(double) * 1024 * DIAG_MAXOUT);
+    launch_diag<T>(a, diag_part.as<double>(), nblocks, stream);
+    HIPCHECK(hipGetLastError());
+    std::vector<double> part((size_t)nblocks * a.nout);
+    HIPCHECK(hipMemcpyAsync(part.data(), diag_part.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (int k = 0; k < a.nout; ++k) {
+      double s = 0;
+      for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * a.nout + k];
+      out[k] = s * scale;
+    }
+    return a.nout;
+  }
+
   // ---------------- adjoint (TwoDNavierStokes, ETDRK4, FP64) ----------------
   DevBuf adj_mixed, adj_flux, lamb, l0b, l1b, lN1b, lN2b, lN3b, nu_part, dtab[6], tape, ckpts;
-  long tape_steps = 0;
+  long tape_steps = 0;       // steps the tape holds (grown on demand, never shrunk:
+  long ckpt_slots = 0;       // freeing device memory synchronises the whole device)
   int ka3_blocks = 0;
 
   void adjoint_setup() {
@@ -840,7 +917,10 @@ struct Ctx final : CtxBase {
         tape.alloc(sizeof(C) * fsz * (8 * seg + 1));
         tape_steps = seg;
       }
-      ckpts.alloc(sizeof(C) * fsz * nseg);
+      if (ckpt_slots < nseg) {
+        ckpts.alloc(sizeof(C) * fsz * nseg);
+        ckpt_slots = nseg;
+      }
       const size_t fb = sizeof(C) * fsz;
       auto len = [&](long s) { return std::min(seg, (long)nsteps - s * seg); };
       // forward: checkpoints at segment starts; the last segment's tape stays for the sweep
@@ -1229,6 +1309,13 @@ int gf_timing_read(gf_ctx* c, int k, int64_t* n, double* ms) {
   if (!n || !ms) return GF_EINVAL;
   return guarded(c, [&] { c->impl->timing_read(k, n, ms); });
 }
+int gf_diagnostic(gf_ctx* c, int kind, double xi, int nxi, double* out, int nout) {
+  if (!out || nout < 1) return GF_EINVAL;
+  int n = 0;
+  const int rc = guarded(c, [&] { n = c->impl->diagnostic(kind, xi, nxi, out, nout); });
+  return rc == GF_OK ? n : rc;
+}
+
 int gf_adjoint_grad(gf_ctx* c, const double* target, int nsteps, int checkpoint_every, double* dJ_dzeta0,
                     double* dJ_dnu, double* J) {
   if (!target) return GF_EINVAL;

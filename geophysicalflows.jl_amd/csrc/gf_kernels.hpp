@@ -168,6 +168,32 @@ template <typename T> void launch_adj(int k, int n, const void* args, hipStream_
 // workgroups of KA3 (and KA1) for ncols columns at size n
 template <typename T> int adj_blocks(int n, int ncols);
 
+// Domain-averaged diagnostics (gf_diag.hip); kinds as GF_DIAG_* in include/gf.h.
+enum DiagKind {
+  DIAG_ENERGY = 0, DIAG_KINETIC_ENERGY = 1, DIAG_POTENTIAL_ENERGY = 2, DIAG_ENSTROPHY = 3,
+  DIAG_PALINSTROPHY = 4, DIAG_ENERGY_DISSIPATION = 5, DIAG_ENSTROPHY_DISSIPATION = 6,
+  DIAG_ENERGY_DRAG = 7, DIAG_ENSTROPHY_DRAG = 8, DIAG_ENERGY_WORK = 9, DIAG_ENSTROPHY_WORK = 10,
+  DIAG_LAYER_ENERGIES = 11
+};
+constexpr int DIAG_MAXOUT = 2 * MAXL - 1;
+
+template <typename T>
+struct DiagArgs {
+  Geo<T> g;
+  int kind, model, nl, nout;
+  const cplx<T>* sol;       // raw state (not dealiased: the reference's diagnostics read sol as is)
+  const cplx<T>* prevsol;   // stochastic forcing: state at the start of the last step, nullable
+  const cplx<T>* Fh;        // fixed forcing table, nullable
+  const T* sqrt_spec;       // ring forcing, nullable
+  uint64_t seed; int64_t fstep; T sqrt_dt; int fkx, fky, instance;
+  const cplx<T>* etah;      // SLQG topographic PV spectrum, nullable
+  int eb; T ell2inv;
+  T xi; int nxi;            // dissipation coefficient and exponent
+  const T* sinv;            // MLQG S^-1 per mode
+  double lay_ke[MAXL], lay_pe[MAXL];   // MLQG: H_j/(2V), f0^2/(2 V g'_j)
+};
+template <typename T> void launch_diag(const DiagArgs<T>& a, double* part, int nblocks, hipStream_t s);
+
 template <typename T> void launch_k1(int ny, const K1Args<T>& a, hipStream_t s);
 template <typename T> void launch_k2(int nx, const K2Args<T>& a, hipStream_t s);
 template <typename T> void launch_k3(int ny, const K3Args<T>& a, hipStream_t s);

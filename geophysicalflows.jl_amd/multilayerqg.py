@@ -80,3 +80,11 @@ def set_q(prob, q):
 
 def updatevars(prob):
     prob._updatevars()
+
+
+def energies(prob):
+    """Layer kinetic energies KE_j and interface potential energies PE_{j+1/2}
+    (src/multilayerqg.jl:918-951), computed on the device: returns (KE, PE) arrays."""
+    n = prob.nlayers
+    v = prob._diagnostic(_lib.GF_DIAG_LAYER_ENERGIES, nout=2 * n - 1)
+    return np.array(v[:n]), np.array(v[n:])

@@ -192,6 +192,12 @@ class Problem:
     def synchronize(self):
         check(lib.gf_synchronize(self._ctx), self._ctx)
 
+    def _diagnostic(self, kind, xi=0.0, nxi=0, nout=1):
+        out = (ctypes.c_double * nout)()
+        n = lib.gf_diagnostic(self._ctx, int(kind), float(xi), int(nxi), out, nout)
+        check(min(n, 0), self._ctx)
+        return list(out)[:n]
+
     def set_clock(self, t, step):
         check(lib.gf_set_clock(self._ctx, float(t), int(step)), self._ctx)
 
@@ -249,6 +255,12 @@ class Ensemble:
 
     def synchronize(self):
         check(lib.gf_synchronize(self._ctx), self._ctx)
+
+    def _diagnostic(self, kind, xi=0.0, nxi=0, nout=1):
+        out = (ctypes.c_double * nout)()
+        n = lib.gf_diagnostic(self._ctx, int(kind), float(xi), int(nxi), out, nout)
+        check(min(n, 0), self._ctx)
+        return list(out)[:n]
 
     def __del__(self):
         ctx = getattr(self, "_ctx", None)

@@ -11,7 +11,6 @@ import numpy as np
 
 from . import _lib
 from .problem import GPU, Problem as _Problem, attach_forcing, dptr, make_desc, stepforward  # noqa: F401
-from .spectral import parsevalsum, parsevalsum2
 
 
 class Params:
@@ -67,25 +66,42 @@ def updatevars(prob):
     prob._updatevars()
 
 
-def _psih(prob, sol):
-    g, p = prob.grid, prob.params
-    if p.deformation_radius is None or p.deformation_radius == math.inf:
-        iK = np.where(g.Krsq > 0, 1.0 / np.where(g.Krsq > 0, g.Krsq, 1.0), 0.0)
-        return -iK * sol
-    return -sol / (g.Krsq + 1 / p.deformation_radius ** 2)
-
-
+# Diagnostics (src/singlelayerqg.jl:546-715), computed on the device from the raw state.
 def kinetic_energy(prob):
-    g = prob.grid
-    return parsevalsum2(np.sqrt(g.Krsq) * _psih(prob, prob.sol), g) / (2 * g.Lx * g.Ly)
+    return prob._diagnostic(_lib.GF_DIAG_KINETIC_ENERGY)[0]
 
 
 def potential_energy(prob):
-    g, p = prob.grid, prob.params
-    if p.deformation_radius is None or p.deformation_radius == math.inf:
-        return 0.0
-    return 1 / p.deformation_radius ** 2 * parsevalsum2(_psih(prob, prob.sol), g) / (2 * g.Lx * g.Ly)
+    return prob._diagnostic(_lib.GF_DIAG_POTENTIAL_ENERGY)[0]
 
 
 def energy(prob):
-    return kinetic_energy(prob) + potential_energy(prob)
+    return prob._diagnostic(_lib.GF_DIAG_ENERGY)[0]
+
+
+def enstrophy(prob):
+    return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY)[0]
+
+
+def energy_dissipation(prob):
+    return prob._diagnostic(_lib.GF_DIAG_ENERGY_DISSIPATION, prob.params.ν, prob.params.nν)[0]
+
+
+def enstrophy_dissipation(prob):
+    return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY_DISSIPATION, prob.params.ν, prob.params.nν)[0]
+
+
+def energy_drag(prob):
+    return prob._diagnostic(_lib.GF_DIAG_ENERGY_DRAG, prob.params.μ, 0)[0]
+
+
+def enstrophy_drag(prob):
+    return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY_DRAG, prob.params.μ, 0)[0]
+
+
+def energy_work(prob):
+    return prob._diagnostic(_lib.GF_DIAG_ENERGY_WORK)[0]
+
+
+def enstrophy_work(prob):
+    return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY_WORK)[0]

@@ -12,7 +12,6 @@ import numpy as np
 
 from . import _lib
 from .problem import GPU, Problem as _Problem, attach_forcing, make_desc, stepforward  # noqa: F401
-from .spectral import parsevalsum
 
 
 class Params:
@@ -48,26 +47,49 @@ def updatevars(prob):
     prob._updatevars()
 
 
-def _norm(prob):
-    g = prob.grid
-    return 1 / (g.Lx * g.Ly)
-
-
+# Diagnostics (src/twodnavierstokes.jl:374-535), computed on the device from the raw state.
 def energy(prob):
-    g = prob.grid
-    with np.errstate(divide="ignore"):
-        iK = np.where(g.Krsq > 0, 1.0 / np.where(g.Krsq > 0, g.Krsq, 1.0), 0.0)
-    return _norm(prob) * parsevalsum(0.5 * iK * np.abs(prob.sol) ** 2, g)
+    return prob._diagnostic(_lib.GF_DIAG_ENERGY)[0]
 
 
 def enstrophy(prob):
-    g = prob.grid
-    return 1 / (2 * g.Lx * g.Ly) * parsevalsum(np.abs(prob.sol) ** 2, g)
+    return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY)[0]
 
 
 def palinstrophy(prob):
-    g = prob.grid
-    return _norm(prob) * parsevalsum(0.5 * g.Krsq * np.abs(prob.sol) ** 2, g)
+    return prob._diagnostic(_lib.GF_DIAG_PALINSTROPHY)[0]
+
+
+def energy_dissipation(prob, ξ, nξ):
+    return prob._diagnostic(_lib.GF_DIAG_ENERGY_DISSIPATION, ξ, nξ)[0]
+
+
+def enstrophy_dissipation(prob, ξ, nξ):
+    return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY_DISSIPATION, ξ, nξ)[0]
+
+
+def energy_dissipation_hyperviscosity(prob):
+    return energy_dissipation(prob, prob.params.ν, prob.params.nν)
+
+
+def energy_dissipation_hypoviscosity(prob):
+    return energy_dissipation(prob, prob.params.μ, prob.params.nμ)
+
+
+def enstrophy_dissipation_hyperviscosity(prob):
+    return enstrophy_dissipation(prob, prob.params.ν, prob.params.nν)
+
+
+def enstrophy_dissipation_hypoviscosity(prob):
+    return enstrophy_dissipation(prob, prob.params.μ, prob.params.nμ)
+
+
+def energy_work(prob):
+    return prob._diagnostic(_lib.GF_DIAG_ENERGY_WORK)[0]
+
+
+def enstrophy_work(prob):
+    return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY_WORK)[0]
 
 
 def adjoint_gradient(prob, target, nsteps, checkpoint=50, nu=True):

@@ -159,6 +159,31 @@ int gf_ctx_create_batch(gf_ctx** out, const gf_problem_desc* descs, int n);
 int gf_batch_size(const gf_ctx* ctx);
 int gf_ctx_instance(gf_ctx* ensemble, int i, gf_ctx** member);
 
+/* Domain-averaged diagnostics, computed on the device from the raw state as the reference
+ * computes them. Each is a Parseval sum over the rfft half plane.
+ *  - TwoDNavierStokes (src/twodnavierstokes.jl:374-535):
+ *      energy, enstrophy, palinstrophy;
+ *      energy_dissipation(prob, xi, nxi) and enstrophy_dissipation (GF_DIAG_*_DISSIPATION with
+ *        xi, nxi; *_hyperviscosity passes nu, nnu and *_hypoviscosity passes mu, nmu);
+ *      energy_work, enstrophy_work.
+ *  - SingleLayerQG (src/singlelayerqg.jl:546-715):
+ *      kinetic_energy, potential_energy, energy, enstrophy (of q + eta);
+ *      energy/enstrophy_dissipation (xi = nu, nxi = nnu);
+ *      energy/enstrophy_drag (xi = mu);
+ *      energy/enstrophy_work.
+ *  - MultiLayerQG energies (src/multilayerqg.jl:918-951): GF_DIAG_LAYER_ENERGIES writes
+ *    KE_1..KE_n, then PE_{3/2}..PE_{n-1/2}.
+ * For stochastic forcing the work uses (prevsol + sol)/2 and the last step's forcing.
+ * Returns the number of values written (>= 1) or a negative GF_E* code; out holds nout
+ * doubles. */
+enum {
+  GF_DIAG_ENERGY = 0, GF_DIAG_KINETIC_ENERGY = 1, GF_DIAG_POTENTIAL_ENERGY = 2, GF_DIAG_ENSTROPHY = 3,
+  GF_DIAG_PALINSTROPHY = 4, GF_DIAG_ENERGY_DISSIPATION = 5, GF_DIAG_ENSTROPHY_DISSIPATION = 6,
+  GF_DIAG_ENERGY_DRAG = 7, GF_DIAG_ENSTROPHY_DRAG = 8, GF_DIAG_ENERGY_WORK = 9, GF_DIAG_ENSTROPHY_WORK = 10,
+  GF_DIAG_LAYER_ENERGIES = 11
+};
+int gf_diagnostic(gf_ctx* ctx, int kind, double xi, int nxi, double* out, int nout);
+
 /* Gradient of J = 1/2 sum_{x,y} (zeta(T) - target)^2 for the problem's current state as
  * zeta0 (set with gf_set_physical), where zeta(T) = irfft2(M sol_T) after nsteps ETDRK4 steps.
  *
