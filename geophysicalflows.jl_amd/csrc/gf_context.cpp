@@ -585,6 +585,12 @@ struct Ctx final : CtxBase {
     c.c0 = coef[0].p; c.c1 = coef[1].p; c.c2 = coef[2].p; c.c3 = coef[3].p; c.c4 = coef[4].p; c.c5 = coef[5].p;
     c.ccplx = ccplx; c.cl_layered = 0; c.dt = (T)d.dt; c.filter = filtered() ? filter.as<T>() : nullptr;
     c.tw = twy.as<C>();
+    if (d.model == GF_MODEL_MLQG && d.linear) {
+      // calcNlinear! works on the whole spectrum: no dealiasing of its input, every mode stepped
+      a.ncols = g.nkr; a.nomask = 1;
+      b.kin = g.nkr; b.kout = g.nkr; b.linear = 1;
+      c.ncols = g.nkr; c.full = 1; c.nomask = 1;
+    }
   }
 
   void calcN_launch(const C* in, int epi, bool full) {
@@ -682,7 +688,7 @@ struct Ctx final : CtxBase {
     calcN_launch(cinp.as<C>(), E_CALCN, true);
     launch_to_julia<T>(g, nout.as<C>(), jl.as<C>(), nl, stream);
     HIPCHECK(hipMemcpyAsync(N, jl.p, bytes, hipMemcpyDeviceToHost, stream));
-    launch_mask<T>(g, cinp.as<C>(), nl, stream);
+    if (!(d.model == GF_MODEL_MLQG && d.linear)) launch_mask<T>(g, cinp.as<C>(), nl, stream);   // calcN!'s dealias!
     launch_to_julia<T>(g, cinp.as<C>(), jl.as<C>(), nl, stream);
     HIPCHECK(hipMemcpyAsync(solh, jl.p, bytes, hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
@@ -1108,7 +1114,6 @@ int validate(const gf_problem_desc* d, std::string& msg) {
   if (d->model == GF_MODEL_MLQG) {
     if (d->nlayers < 2 || d->nlayers > 4) { msg = "MultiLayerQG: nlayers must be 2..4 in this build"; return GF_EUNSUPPORTED; }
     if (!d->H || !d->b) { msg = "MultiLayerQG: H and b are required"; return GF_EINVAL; }
-    if (d->linear) { msg = "MultiLayerQG: linear=true is not supported by this build"; return GF_EUNSUPPORTED; }
     for (int j = 0; j + 1 < d->nlayers; ++j)
       if (d->b[j] - d->b[j + 1] == 0) { msg = "MultiLayerQG: equal buoyancy in adjacent layers"; return GF_EINVAL; }
   } else if (d->nlayers != 1 && d->nlayers != 0) {

@@ -61,6 +61,7 @@ struct K1Args {
   int eb;                   // SLQG equivalent-barotropic: psi = -q/(K^2 + ell2inv)
   T ell2inv;
   const T* sinv;            // MLQG: S^-1 per mode, n*n reals, blocked layout (n*n per mode)
+  int nomask;               // MLQG linearised equations: the state is read without dealiasing
   const cplx<T>* tw;
 };
 
@@ -78,6 +79,7 @@ struct K2Args {
   const T* Qy1;             // nlayers*ny
   const T* etax;            // ny*nx, bottom layer, nullable
   const T* etay;            // ny*nx, bottom layer, nullable
+  int linear;               // MLQG linearised equations: advect q by U only
   int has_bg;               // any nonzero Qx/Qy/U term
   cplx<T>* out;             // flux (2 fields per layer) or mixed (forward-in)
   long ostride;
@@ -113,6 +115,7 @@ struct K3Args {
   T dt;
   const T* filter;          // nullable
   int zero_mean;            // E_STORE: set mode (0,0) to zero
+  int nomask;               // MLQG linearised equations: no dealiasing of the state
   // adjoint tape (ETDRK4): nsave receives each stage's N, sol_out the new state of
   // E_ETD4 (both nullable: no tape / update sol in place)
   cplx<T>* nsave;

@@ -130,7 +130,8 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
   const int kr = 2 * (blk * G + m.slot) + m.c;
   const Geo<T>& g = a.g;
   const bool active = kr < a.ncols && kr < g.nkr;
-  const bool masked = FS != FS_RAW;
+  // the linearised MultiLayerQG calcN! does not dealias its input (src/multilayerqg.jl:675-683)
+  const bool masked = FS != FS_RAW && !(NL > 1 && FS == FS_CALCN && a.nomask);
   const bool krk = kr < g.kmax || !masked;
   const T kv = krwave(g, kr);
   cplx<T>* L = col_buffer<T, NY>(lds, m.slot, m.c);
@@ -430,11 +431,14 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
           z[0][q] = mk(u0 * q0, v0 * q0);
           z[1][q] = mk(u1 * q1, v1 * q1);
         } else {
-          // u += U; N = -(u Qx)^ - (v Qy)^ - i kr (u q)^ - i l (v q)^ (src/multilayerqg.jl:695-731)
-          T c0 = 0, c1 = 0;
+          // u += U; N = -(u Qx)^ - (v Qy)^ - i kr (u q)^ - i l (v q)^ (src/multilayerqg.jl:695-731);
+          // linearised: N = -(u Qx)^ - (v Qy)^ - i kr (U q)^ (src/multilayerqg.jl:744-779)
+          T c0 = 0, c1 = 0, U0 = 0, U1 = 0;
           if (active && a.has_bg) {
-            u0 = u0 + a.Uy[jl * ny + y0];
-            u1 = u1 + a.Uy[jl * ny + y0 + 1];
+            U0 = a.Uy[jl * ny + y0];
+            U1 = a.Uy[jl * ny + y0 + 1];
+            u0 = u0 + U0;
+            u1 = u1 + U1;
             T qy0 = a.Qy1[jl * ny + y0], qy1 = a.Qy1[jl * ny + y0 + 1];
             T qx0 = 0, qx1 = 0;
             if (jl == NL - 1) {
@@ -444,8 +448,13 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
             c0 = u0 * qx0 + v0 * qy0;
             c1 = u1 * qx1 + v1 * qy1;
           }
-          z[0][q] = mk(u0 * q0, v0 * q0);
-          z[1][q] = mk(u1 * q1, v1 * q1);
+          if (a.linear) {
+            z[0][q] = mk(U0 * q0, (T)0);
+            z[1][q] = mk(U1 * q1, (T)0);
+          } else {
+            z[0][q] = mk(u0 * q0, v0 * q0);
+            z[1][q] = mk(u1 * q1, v1 * q1);
+          }
           z[2][q] = mk(c0, c1);
         }
       }
@@ -569,7 +578,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
     for (int q = 0; q < P; ++q) {
       const int l = t + TPT * q;
       cplx<T> ps = czero<T>();
-      if (active && a.drag_src != nullptr && kept_l(g, l) && kr < g.kmax) {
+      if (active && a.drag_src != nullptr && ((kept_l(g, l) && kr < g.kmax) || a.nomask)) {
         const long idx = boff(kr, l, NY);
         const T* m = a.sinv + idx * (NL * NL) + (NL - 1) * NL;
 #pragma unroll
@@ -595,7 +604,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int l = t + TPT * q;
-      const bool kept = kept_l(g, l) && kr < g.kmax;
+      const bool kept = (kept_l(g, l) && kr < g.kmax) || (NL > 1 && a.nomask);
       if (!kept && !a.full) continue;
       const T lv = lwave(g, l);
       const uint32_t idx = boff32(kr, l, NY);

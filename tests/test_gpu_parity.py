@@ -278,6 +278,45 @@ def test_mlqg_calcN_steps_updatevars(gf, nlayers):
         assert rel(a, bb) < 1e-13
 
 
+@pytest.mark.parametrize("nlayers", [2, 3])
+def test_mlqg_linear_calcN_and_steps(gf, nlayers):
+    # calcNlinear! (src/multilayerqg.jl:675-683, 744-779): no dealiasing of the input
+    n = 64
+    H = {2: [0.2, 0.8], 3: [0.2, 0.3, 0.5]}[nlayers]
+    b = {2: [-1.0, -1.2], 3: [-1.0, -1.2, -1.3]}[nlayers]
+    g = off.TwoDGrid(n, TWO_PI)
+    X, Y = g.gridpoints()
+    eta = 0.3 * np.exp(-((X + 1) ** 2 + (Y - 0.5) ** 2) / 0.3)
+    Uy = np.stack([0.3 * np.cos(g.y) * (j + 1) / nlayers for j in range(nlayers)], axis=1)   # (ny, n)
+    s = rand_spec(np.random.default_rng(70 + nlayers), (nlayers, n, n // 2 + 1)) * 0.05
+    kw = dict(nx=n, Lx=TWO_PI, H=H, b=b, U=Uy, eta=eta, dt=0.005, stepper="ETDRK4", linear=True)
+    p = gf.MultiLayerQG.Problem(nlayers, gf.GPU(), f0=1.0, β=5.0, μ=0.01, ν=1e-3, **kw)
+    r = oml.Problem(nlayers, f0=1.0, beta=5.0, mu=0.01, nu=1e-3, **kw)
+    N, s_out = p.calcN(s)
+    Nr, sr = np.zeros_like(s), s.copy()
+    oml.calcNlinear(Nr, sr, 0.0, r.clock, r.vars, r.params, r.grid)
+    assert rel(N, Nr) < 1e-13
+    assert np.array_equal(s_out, s)           # the input is not dealiased
+    p.sol = s
+    r.sol[...] = s
+    gf.stepforward(p, 3)
+    off.stepforward(r, 3)
+    assert rel(p.sol, r.sol) < 1e-12
+
+
+def test_mlqg_linear_manufactured(gf):
+    # test/test_multilayerqg.jl:413-494 restated on 64 x 64: the steady solution stays put
+    from tests.test_oracle_qg import mlqg_manufactured_setup
+    kw, Ff, qf = mlqg_manufactured_setup(2, 64, linear=True)
+    p = gf.MultiLayerQG.Problem(2, gf.GPU(), nx=64, ny=64, Lx=kw["Lx"], Ly=kw["Ly"], f0=1.0, H=kw["H"], b=kw["b"],
+                                U=kw["U"], eta=kw["eta"], β=kw["beta"], μ=kw["mu"], ν=kw["nu"], nν=1,
+                                linear=True, dt=0.005, stepper="ForwardEuler", calcFq=Ff)
+    gf.MultiLayerQG.set_q(p, qf)
+    gf.stepforward(p, 100)
+    gf.MultiLayerQG.updatevars(p)
+    assert rel(p.vars.q, qf) < 1e-12
+
+
 @pytest.mark.parametrize("stepper", ["RK4", "FilteredRK4", "ForwardEuler"])
 def test_mlqg_other_steppers(gf, stepper):
     n = 64

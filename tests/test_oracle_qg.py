@@ -182,8 +182,11 @@ def test_mlqg_pv_inversion_3layer():
         assert np.linalg.norm(psi[j] - psis[j]) / np.linalg.norm(psis[j]) < 1e-13
 
 
-def _mlqg_manufactured(nlayers, n=64, dt=0.005, nt=40):
-    """test/test_multilayerqg.jl:225-305 (2 layers) and 312-400 (3 layers), on n x n."""
+def mlqg_manufactured_setup(nlayers, n=64, linear=False):
+    """Problem data and forcing of the reference's steady manufactured MultiLayerQG solutions on
+    n x n: test/test_multilayerqg.jl:225-305 (2 layers), 312-400 (3 layers) and, with linear,
+    the linearised equations of 413-494 (no J(psi_j, q_j) term; U1, U2 = 0.1, 0.05 and
+    u1 = 0.5 sech^2).  Returns (keyword arguments of Problem, forcing spectra, steady q)."""
     Lx = 2 * np.pi
     Ly = 2 * np.pi if nlayers == 2 else 5 * np.pi
     g = ff.TwoDGrid(n, Lx, n, Ly)
@@ -192,8 +195,8 @@ def _mlqg_manufactured(nlayers, n=64, dt=0.005, nt=40):
     beta = 0.35
     if nlayers == 2:
         H, b = [0.2, 0.8], [-1.0, -1.2]
-        Uc = [0.02, 0.025]
-        us = [0.05 / np.cosh(g.y / (Ly / 15)) ** 2, 0.02 * np.cos(3 * l0 * g.y)]
+        Uc = [0.1, 0.05] if linear else [0.02, 0.025]
+        us = [(0.5 if linear else 0.05) / np.cosh(g.y / (Ly / 15)) ** 2, 0.02 * np.cos(3 * l0 * g.y)]
         psi1, psi2, q1, q2 = _fields_2layer(g)
         psis, qs = (psi1, psi2), (q1, q2)
         F = np.array([[-25, 25], [25 / 4, -25 / 4]])
@@ -223,17 +226,29 @@ def _mlqg_manufactured(nlayers, n=64, dt=0.005, nt=40):
     for j in range(nlayers):
         Uj = Uy[j][:, None]
         Qyj = beta - uyy[j][:, None] - sum(F[j, k] * (Uy[k] - Uy[j])[:, None] for k in range(nlayers) if k != j)
-        fj = jac(psis[j], qs[j]) + Qyj * dx(psis[j]) + Uj * dx(qs[j]) - nu * lap(qs[j])
+        fj = Qyj * dx(psis[j]) + Uj * dx(qs[j]) - nu * lap(qs[j])
+        if not linear:
+            fj = fj + jac(psis[j], qs[j])
         if j == nlayers - 1:
             fj = fj + jac(psis[j], eta) + Uj * etax + mu * lap(psis[j])
         Ff.append(g.rfft(fj))
-    prob = ML.Problem(nlayers, nx=n, ny=n, Lx=Lx, Ly=Ly, f0=1, H=H, b=b, U=Uy.T, eta=eta, beta=beta,
-                      mu=mu, nu=nu, nnu=1, dt=dt, stepper="ForwardEuler", calcFq=np.array(Ff))
-    ML.set_q(prob, np.array(qs))
+    kw = dict(nx=n, ny=n, Lx=Lx, Ly=Ly, f0=1, H=H, b=b, U=Uy.T, eta=eta, beta=beta, mu=mu, nu=nu, nnu=1,
+              linear=linear)
+    return kw, np.array(Ff), np.array(qs)
+
+
+def _mlqg_manufactured(nlayers, n=64, dt=0.005, nt=40, linear=False):
+    kw, Ff, qf = mlqg_manufactured_setup(nlayers, n, linear)
+    prob = ML.Problem(nlayers, dt=dt, stepper="ForwardEuler", calcFq=Ff, **kw)
+    ML.set_q(prob, qf)
     ff.stepforward(prob, nt)
     ML.updatevars(prob)
-    qf = np.array(qs)
     return np.linalg.norm(prob.vars.q - qf) / np.linalg.norm(qf)
+
+
+def test_mlqg_linear_manufactured():
+    # test/test_multilayerqg.jl:413-494 (runtests.jl:135), restated on 64 x 64
+    assert _mlqg_manufactured(2, linear=True) < 1e-12
 
 
 @pytest.mark.parametrize("nlayers", [2, 3])
