@@ -14,7 +14,7 @@ GF_OK, GF_EINVAL, GF_EUNSUPPORTED, GF_EHIP, GF_ENOTFINITE, GF_ENODEVICE = 0, -1,
 GF_MODEL_TWODNS, GF_MODEL_SLQG, GF_MODEL_MLQG = 0, 1, 2
 GF_F64, GF_F32 = 0, 1
 STEPPERS = {"ETDRK4": 0, "FilteredETDRK4": 1, "RK4": 2, "FilteredRK4": 3,
-            "ForwardEuler": 4, "FilteredForwardEuler": 5}
+            "ForwardEuler": 4, "FilteredForwardEuler": 5, "AB3": 6, "FilteredAB3": 7}
 GF_FORCING_NONE, GF_FORCING_FIXED, GF_FORCING_RING = 0, 1, 2
 (GF_DIAG_ENERGY, GF_DIAG_KINETIC_ENERGY, GF_DIAG_POTENTIAL_ENERGY, GF_DIAG_ENSTROPHY, GF_DIAG_PALINSTROPHY,
  GF_DIAG_ENERGY_DISSIPATION, GF_DIAG_ENSTROPHY_DISSIPATION, GF_DIAG_ENERGY_DRAG, GF_DIAG_ENSTROPHY_DRAG,

@@ -108,7 +108,10 @@ struct DevBuf {
     if (bytes) {
       hipError_t e = hipMalloc(&p, bytes);
       if (e != hipSuccess) throw GfError(GF_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+      // the contexts' streams are non-blocking: the null-stream memset is not ordered before
+      // their work, so wait for it here (a late memset would clear what a kernel wrote)
       e = hipMemset(p, 0, bytes);
+      if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
       if (e != hipSuccess) throw GfError(GF_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
     }
   }
@@ -480,8 +483,10 @@ struct Ctx final : CtxBase {
   }
 
   bool filtered() const {
-    return d.stepper == GF_FILTERED_ETDRK4 || d.stepper == GF_FILTERED_RK4 || d.stepper == GF_FILTERED_FORWARD_EULER;
+    return d.stepper == GF_FILTERED_ETDRK4 || d.stepper == GF_FILTERED_RK4 || d.stepper == GF_FILTERED_FORWARD_EULER ||
+           d.stepper == GF_FILTERED_AB3;
   }
+  bool ab3() const { return d.stepper == GF_AB3 || d.stepper == GF_FILTERED_AB3; }
   bool etd() const { return d.stepper == GF_ETDRK4 || d.stepper == GF_FILTERED_ETDRK4; }
 
   void build_stepper() {
@@ -585,6 +590,11 @@ struct Ctx final : CtxBase {
     c.c0 = coef[0].p; c.c1 = coef[1].p; c.c2 = coef[2].p; c.c3 = coef[3].p; c.c4 = coef[4].p; c.c5 = coef[5].p;
     c.ccplx = ccplx; c.cl_layered = 0; c.dt = (T)d.dt; c.filter = filtered() ? filter.as<T>() : nullptr;
     c.tw = twy.as<C>();
+    if (ab3()) {
+      // the RHS history of the aliased band enters later steps: every AB3 step is computed in full
+      c.ab3_start = stepno < 3;
+      b.kout = g.nkr; c.ncols = g.nkr; c.full = 1;
+    }
     if (d.model == GF_MODEL_MLQG && d.linear) {
       // calcNlinear! works on the whole spectrum: no dealiasing of its input, every mode stepped
       a.ncols = g.nkr; a.nomask = 1;
@@ -614,7 +624,7 @@ struct Ctx final : CtxBase {
       epis[0] = E_RK1; epis[1] = E_RK2; epis[2] = E_RK3; epis[3] = E_RK4;
       return 4;
     }
-    ins[0] = sol.as<C>(); epis[0] = E_FE;
+    ins[0] = sol.as<C>(); epis[0] = ab3() ? E_AB3 : E_FE;
     return 1;
   }
 
@@ -1109,7 +1119,7 @@ int validate(const gf_problem_desc* d, std::string& msg) {
   }
   if (!(d->Lx > 0) || !(d->Ly > 0)) { msg = "Lx and Ly must be positive"; return GF_EINVAL; }
   if (!(d->dt > 0)) { msg = "dt must be positive"; return GF_EINVAL; }
-  if (d->stepper < 0 || d->stepper > 5) { msg = "unknown stepper"; return GF_EINVAL; }
+  if (d->stepper < 0 || d->stepper > 7) { msg = "unknown stepper"; return GF_EINVAL; }
   if (d->aliased_fraction < 0 || d->aliased_fraction >= 1) { msg = "aliased_fraction must be in [0, 1)"; return GF_EINVAL; }
   if (d->model == GF_MODEL_MLQG) {
     if (d->nlayers < 2 || d->nlayers > 4) { msg = "MultiLayerQG: nlayers must be 2..4 in this build"; return GF_EUNSUPPORTED; }
@@ -1140,6 +1150,7 @@ int validate_batch(const gf_problem_desc* ds, int n, std::string& msg) {
     }
     if (d.forcing == GF_FORCING_RING) { msg = who + "stochastic ring forcing is not supported in ensembles"; return GF_EUNSUPPORTED; }
     if (d.stepper != ds[0].stepper) { msg = who + "all members must use the same stepper"; return GF_EINVAL; }
+    if (d.stepper == GF_AB3 || d.stepper == GF_FILTERED_AB3) { msg = who + "AB3 is not supported in ensembles"; return GF_EUNSUPPORTED; }
   }
   return GF_OK;
 }

@@ -33,7 +33,7 @@ enum Epi {
   E_CALCN = 0, E_STORE = 1,
   E_ETD1 = 2, E_ETD2 = 3, E_ETD3 = 4, E_ETD4 = 5,
   E_RK1 = 6, E_RK2 = 7, E_RK3 = 8, E_RK4 = 9,
-  E_FE = 10
+  E_FE = 10, E_AB3 = 11
 };
 
 constexpr int MAXL = 8;  // max layers
@@ -116,6 +116,7 @@ struct K3Args {
   const T* filter;          // nullable
   int zero_mean;            // E_STORE: set mode (0,0) to zero
   int nomask;               // MLQG linearised equations: no dealiasing of the state
+  int ab3_start;            // AB3: forward Euler start-up step
   // adjoint tape (ETDRK4): nsave receives each stage's N, sol_out the new state of
   // E_ETD4 (both nullable: no tape / update sol in place)
   cplx<T>* nsave;

@@ -711,6 +711,25 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
           if (a.filter) r = rmul(at(a.filter, idx), r);
           at(a.sol, ix) = r;
         } break;
+        case E_AB3: {
+          // RHS = N + L sol; forward Euler start-up, then 23/12 RHS - 16/12 RHS_1 + 5/12 RHS_2;
+          // the history (racc = RHS_1, n1 = RHS_2) shifts every step ([FF] AB3TimeStepper)
+          const Coef<T> Lc{a.c0, a.ccplx};
+          const cplx<T> R = N + Lc.mul(ci, sv);
+          const cplx<T> R1 = at(a.racc, ix);
+          cplx<T> r;
+          if (a.ab3_start) {
+            r = sv + rmul(a.dt, R);
+          } else {
+            const cplx<T> R2 = at(a.n1, ix);
+            const cplx<T> acc = (rmul((T)(23.0 / 12.0), R) - rmul((T)(16.0 / 12.0), R1)) + rmul((T)(5.0 / 12.0), R2);
+            r = sv + rmul(a.dt, acc);
+          }
+          if (a.filter) r = rmul(at(a.filter, idx), r);
+          at(a.sol, ix) = r;
+          at(a.n1, ix) = R1;
+          at(a.racc, ix) = R;
+        } break;
         default: break;
       }
       // one point's loads and stores at a time: keeps the epilogue within 128 VGPRs (two

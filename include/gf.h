@@ -42,7 +42,8 @@ enum { GF_MODEL_TWODNS = 0, GF_MODEL_SLQG = 1, GF_MODEL_MLQG = 2 };
 enum { GF_F64 = 0, GF_F32 = 1 };
 enum {
   GF_ETDRK4 = 0, GF_FILTERED_ETDRK4 = 1, GF_RK4 = 2, GF_FILTERED_RK4 = 3,
-  GF_FORWARD_EULER = 4, GF_FILTERED_FORWARD_EULER = 5
+  GF_FORWARD_EULER = 4, GF_FILTERED_FORWARD_EULER = 5,
+  GF_AB3 = 6, GF_FILTERED_AB3 = 7   /* [FF] AB3TimeStepper (test/runtests.jl:58); not in ensembles */
 };
 enum { GF_FORCING_NONE = 0, GF_FORCING_FIXED = 1, GF_FORCING_RING = 2 };
 

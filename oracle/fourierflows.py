@@ -271,10 +271,36 @@ class ForwardEulerStepper:
             sol *= self.filter
 
 
+class AB3Stepper:
+    """[FF] ``AB3TimeStepper`` ("AB3", "FilteredAB3" at ``test/runtests.jl:58``): with
+    ``RHS = N + L sol``, forward Euler ``sol += dt RHS`` while ``clock.step < 3``, then
+    ``sol += dt (23/12 RHS - 16/12 RHS_1 + 5/12 RHS_2)``; the RHS history shifts every step.
+    Parity: the linear part is pinned by the reference's Rossby-wave tests; the number of
+    forward Euler start-up steps (3, recalled from FourierFlows 0.10) is parity unpinned."""
+
+    def __init__(self, eq, dt, filtered=False):
+        shape = eq.L.shape
+        self.RHS, self.RHS1, self.RHS2 = (np.zeros(shape, complex) for _ in range(3))
+        self.filter = makefilter(eq.grid) if filtered else None
+
+    def step(self, sol, clock, eq, vars, params, grid):
+        eq.calcN(self.RHS, sol, clock.t, clock, vars, params, grid)
+        self.RHS += eq.L * sol
+        if clock.step < 3:
+            sol += clock.dt * self.RHS
+        else:
+            sol += clock.dt * (23 / 12 * self.RHS - 16 / 12 * self.RHS1 + 5 / 12 * self.RHS2)
+        if self.filter is not None:
+            sol *= self.filter
+        self.RHS2[...] = self.RHS1
+        self.RHS1[...] = self.RHS
+
+
 STEPPERS = {
     "ETDRK4": (ETDRK4Stepper, False), "FilteredETDRK4": (ETDRK4Stepper, True),
     "RK4": (RK4Stepper, False), "FilteredRK4": (RK4Stepper, True),
     "ForwardEuler": (ForwardEulerStepper, False), "FilteredForwardEuler": (ForwardEulerStepper, True),
+    "AB3": (AB3Stepper, False), "FilteredAB3": (AB3Stepper, True),
 }
 
 

@@ -86,7 +86,7 @@ def test_null_arguments(gf):
 
 def test_python_mirror_rejects_unknown_stepper(gf):
     with pytest.raises(ValueError):
-        gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, stepper="AB3")
+        gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, stepper="LSRK54")
     with pytest.raises(NotImplementedError):
         gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, calcF=lambda *a: None)
 

@@ -278,6 +278,32 @@ def test_mlqg_calcN_steps_updatevars(gf, nlayers):
         assert rel(a, bb) < 1e-13
 
 
+@pytest.mark.parametrize("stepper", ["AB3", "FilteredAB3"])
+@pytest.mark.parametrize("model", ["twodns", "slqg"])
+def test_ab3_matches_oracle(gf, stepper, model):
+    # [FF] AB3 (test/runtests.jl:58): forward Euler start-up, then 3rd-order Adams-Bashforth;
+    # 7 steps cross the start-up, split over calls to exercise the carried RHS history
+    n = 64
+    rng = np.random.default_rng(80)
+    s = rand_spec(rng, (n, n // 2 + 1)) * 0.05
+    if model == "twodns":
+        p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, ν=1e-3, nν=1, μ=0.05, dt=0.002, stepper=stepper)
+        r = o2d.Problem(nx=n, nu=1e-3, nnu=1, mu=0.05, dt=0.002, stepper=stepper)
+    else:
+        g = off.TwoDGrid(n, TWO_PI)
+        X, Y = g.gridpoints()
+        eta = 0.3 * np.cos(2 * X) * np.sin(Y)
+        p = gf.SingleLayerQG.Problem(gf.GPU(), nx=n, β=2.0, U=0.2, eta=eta, ν=1e-3, dt=0.002, stepper=stepper)
+        r = osl.Problem(nx=n, beta=2.0, U=0.2, eta=eta, nu=1e-3, dt=0.002, stepper=stepper)
+    p.sol = s
+    r.sol[...] = s
+    for k in (2, 1, 4):
+        gf.stepforward(p, k)
+        off.stepforward(r, k)
+        assert rel(p.sol, r.sol) < 1e-12, k
+    assert p.clock.step == 7
+
+
 @pytest.mark.parametrize("nlayers", [2, 3])
 def test_mlqg_linear_calcN_and_steps(gf, nlayers):
     # calcNlinear! (src/multilayerqg.jl:675-683, 744-779): no dealiasing of the input

@@ -95,6 +95,7 @@ def test_slqg_energies_closed_form():
 
 @pytest.mark.parametrize("stepper,dt,nsteps", [("ETDRK4", 1e-2, 20), ("FilteredETDRK4", 1e-2, 20),
                                                ("RK4", 1e-2, 20), ("FilteredRK4", 1e-2, 20),
+                                               ("AB3", 1e-3, 200), ("FilteredAB3", 1e-3, 200),
                                                ("ForwardEuler", 1e-4, 2000)])
 @pytest.mark.parametrize("ell,U0", [(np.inf, 0.0), (1.23, 0.3)])
 def test_slqg_rossby_wave(stepper, dt, nsteps, ell, U0):
