@@ -10,9 +10,11 @@ from . import _lib  # noqa: F401  (loads libgfhip.so or raises ImportError)
 from . import multilayerqg as MultiLayerQG  # noqa: F401
 from . import singlelayerqg as SingleLayerQG  # noqa: F401
 from . import twodnavierstokes as TwoDNavierStokes  # noqa: F401
+from . import output  # noqa: F401
 from . import utils  # noqa: F401
-from .problem import CPU, GPU, Ensemble, RingForcing, stepforward  # noqa: F401
+from .output import Output, get_sol, savediagnostic, saveoutput, saveproblem  # noqa: F401
+from .problem import CPU, GPU, Diagnostic, Ensemble, RingForcing, stepforward  # noqa: F401
 from .utils import peakedisotropicspectrum  # noqa: F401
 
-__all__ = ["TwoDNavierStokes", "SingleLayerQG", "MultiLayerQG", "GPU", "CPU", "Ensemble", "RingForcing", "stepforward",
-           "peakedisotropicspectrum"]
+__all__ = ["TwoDNavierStokes", "SingleLayerQG", "MultiLayerQG", "GPU", "CPU", "Ensemble", "RingForcing", "Diagnostic", "stepforward",
+           "peakedisotropicspectrum", "Output", "get_sol", "saveproblem", "saveoutput", "savediagnostic"]

@@ -25,7 +25,9 @@ EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string"
            "gf_set_physical", "gf_updatevars", "gf_num_vars", "gf_calcN", "gf_set_forcing",
            "gf_step", "gf_clock", "gf_set_clock", "gf_synchronize", "gf_timing",
            "gf_timing_read", "gf_kernel_bytes", "gf_version", "gf_ctx_create_batch",
-           "gf_batch_size", "gf_ctx_instance", "gf_adjoint_grad", "gf_diagnostic")
+           "gf_batch_size", "gf_ctx_instance", "gf_adjoint_grad", "gf_diagnostic",
+           "gf_diagnostic_async", "gf_snapshot", "gf_snapshot_wait", "gf_host_alloc", "gf_host_free",
+           "gf_device_alloc", "gf_device_free", "gf_device_copy")
 
 
 class ProblemDesc(ctypes.Structure):
@@ -89,6 +91,19 @@ def _load():
     lib.gf_ctx_instance.argtypes = [vp, ip, ctypes.POINTER(vp)]
     lib.gf_adjoint_grad.argtypes = [vp, vp, ip, ip, vp, dp, dp]
     lib.gf_diagnostic.argtypes = [vp, ip, ctypes.c_double, ip, dp, ip]
+    lib.gf_diagnostic_async.argtypes = [vp, ip, ctypes.c_double, ip, vp, ip]
+    lib.gf_snapshot.argtypes = [vp, vp]
+    lib.gf_snapshot.restype = ctypes.c_int64
+    lib.gf_snapshot_wait.argtypes = [vp, ctypes.c_int64]
+    lib.gf_host_alloc.argtypes = [ctypes.c_size_t]
+    lib.gf_host_alloc.restype = ctypes.c_void_p
+    lib.gf_host_free.argtypes = [vp]
+    lib.gf_host_free.restype = None
+    lib.gf_device_alloc.argtypes = [ctypes.c_size_t]
+    lib.gf_device_alloc.restype = ctypes.c_void_p
+    lib.gf_device_free.argtypes = [vp]
+    lib.gf_device_free.restype = None
+    lib.gf_device_copy.argtypes = [vp, vp, ctypes.c_size_t]
     for name in EXPORTS:
         getattr(lib, name)
     return lib

@@ -175,11 +175,16 @@ struct CtxBase {
   virtual void timing_read(int k, int64_t* n, double* ms) = 0;
   virtual double kernel_bytes(int k, int full) const = 0;
   virtual void sync() = 0;
+  virtual int64_t snapshot(void*) { throw GfError(GF_EINVAL, "snapshots are taken per problem; address one member through gf_ctx_instance"); }
+  virtual void snapshot_wait(int64_t) {}
   // ensembles (a plain context is an ensemble of one)
   virtual int ninstances() const { return 1; }
   virtual CtxBase* instance(int i) { return i == 0 ? this : nullptr; }
   virtual void set_clock(double tt, int64_t s) { t = tt; stepno = s; }
   virtual int diagnostic(int, double, int, double*, int) {
+    throw GfError(GF_EINVAL, "diagnostics are computed per problem; address one member through gf_ctx_instance");
+  }
+  virtual int diagnostic_async(int, double, int, double*, int) {
     throw GfError(GF_EINVAL, "diagnostics are computed per problem; address one member through gf_ctx_instance");
   }
   virtual void adjoint_grad(const double*, int, int, double*, double*, double*) {
@@ -247,6 +252,50 @@ struct Ctx final : CtxBase {
   }
   ~Ctx() override {
     if (stream) { (void)hipStreamSynchronize(stream); if (own_stream) (void)hipStreamDestroy(stream); }
+    if (copy_stream) {
+      (void)hipStreamSynchronize(copy_stream);
+      for (int k = 0; k < 2; ++k) { (void)hipEventDestroy(snap_ready[k]); (void)hipEventDestroy(snap_done[k]); }
+      (void)hipStreamDestroy(copy_stream);
+    }
+  }
+
+  // ---------------- snapshots ----------------
+  // The state is converted to the Julia layout on the compute stream into one of two staging
+  // buffers; the device-to-host copy runs on a copy stream, so the steps queued after it
+  // overlap the copy. Before a staging buffer is reused the compute stream waits for its
+  // previous copy.
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t snap_ready[2] = {}, snap_done[2] = {};
+  DevBuf snap[2];
+  bool snap_used[2] = {false, false};
+  int64_t snap_count = 0;
+
+  int64_t snapshot(void* dst) override {
+    const size_t bytes = sizeof(C) * (size_t)g.nkr * d.ny * nl;
+    if (!copy_stream) {
+      HIPCHECK(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+      for (int k = 0; k < 2; ++k) {
+        HIPCHECK(hipEventCreateWithFlags(&snap_ready[k], hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&snap_done[k], hipEventDisableTiming));
+        snap[k].alloc(bytes);
+      }
+    }
+    const int k = (int)(snap_count & 1);
+    if (snap_used[k]) HIPCHECK(hipStreamWaitEvent(stream, snap_done[k], 0));
+    launch_to_julia<T>(g, sol.as<C>(), snap[k].as<C>(), nl, stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(snap_ready[k], stream));
+    HIPCHECK(hipStreamWaitEvent(copy_stream, snap_ready[k], 0));
+    HIPCHECK(hipMemcpyAsync(dst, snap[k].p, bytes, hipMemcpyDeviceToHost, copy_stream));
+    HIPCHECK(hipEventRecord(snap_done[k], copy_stream));
+    snap_used[k] = true;
+    return snap_count++;
+  }
+  // the copy stream runs in order: the event of the ticket's slot is this snapshot's or a later one's
+  void snapshot_wait(int64_t ticket) override {
+    if (!copy_stream) return;
+    if (ticket < 0 || ticket >= snap_count) { HIPCHECK(hipStreamSynchronize(copy_stream)); return; }
+    HIPCHECK(hipEventSynchronize(snap_done[ticket & 1]));
   }
 
   // ---------------- setup helpers ----------------
@@ -755,12 +804,13 @@ struct Ctx final : CtxBase {
   void sync() override { HIPCHECK(hipStreamSynchronize(stream)); }
 
   // ---------------- diagnostics ----------------
-  int diagnostic(int kind, double xi, int nxi, double* out, int nout) override {
-    DiagArgs<T> a{};
+  // fills the diagnostic's kernel arguments and the host-side scale; throws for a kind the
+  // problem does not provide
+  void diag_setup(int kind, double xi, int nxi, DiagArgs<T>& a, double& scale) {
     a.g = g; a.kind = kind; a.model = d.model; a.nl = nl; a.sol = sol.as<C>();
     a.nout = 1;
     const double np2 = (double)d.nx * d.ny * (double)d.nx * d.ny;
-    double scale = 1.0 / np2;                               // 1/(Lx Ly) times parsevalsum's Lx Ly/(nx ny)^2
+    scale = 1.0 / np2;                                      // 1/(Lx Ly) times parsevalsum's Lx Ly/(nx ny)^2
     const bool slqg = d.model == GF_MODEL_SLQG, twodns = d.model == GF_MODEL_TWODNS;
     auto bad = [&](const char* m) { throw GfError(GF_EUNSUPPORTED, m); };
     if (d.model == GF_MODEL_MLQG) {
@@ -787,8 +837,9 @@ struct Ctx final : CtxBase {
           if (slqg && eb) bad("not implemented for finite deformation radius (src/singlelayerqg.jl)");
           if (d.forcing == GF_FORCING_NONE) bad("the problem has no forcing");
           if (d.forcing == GF_FORCING_FIXED) a.Fh = Fh.as<C>();
-          if (d.forcing == GF_FORCING_RING) {
-            if (!prevsol_valid) throw GfError(GF_EINVAL, "work of stochastic forcing needs a step taken first");
+          if (d.forcing == GF_FORCING_RING && prevsol_valid) {
+            // before the first step Fh is still zero, and so is the work (as the reference's
+            // zero-initialised vars.Fh gives)
             a.prevsol = prevsol.as<C>();
             a.sqrt_spec = fkx >= 0 ? sqrt_spec.as<T>() : nullptr;
             a.seed = d.seed; a.fstep = stepno - 1; a.sqrt_dt = (T)std::sqrt(d.dt);
@@ -800,11 +851,31 @@ struct Ctx final : CtxBase {
       a.etah = (slqg && has_eta) ? etah.as<C>() : nullptr;
       a.eb = eb; a.ell2inv = (T)ell2inv;
     }
-    if (nout < a.nout) throw GfError(GF_EINVAL, "output array too small for this diagnostic");
+  }
+  // per-block partial sums into diag_part; returns the number of blocks
+  int diag_launch(const DiagArgs<T>& a) {
     const int nblocks = (int)std::min<long>(1024, (fsz + 255) / 256);
     if (!diag_part.p) diag_part.alloc(sizeof(double) * 1024 * DIAG_MAXOUT);
     launch_diag<T>(a, diag_part.as<double>(), nblocks, stream);
     HIPCHECK(hipGetLastError());
+    return nblocks;
+  }
+  int diagnostic_async(int kind, double xi, int nxi, double* out_dev, int nout) override {
+    DiagArgs<T> a{};
+    double scale = 1;
+    diag_setup(kind, xi, nxi, a, scale);
+    if (nout < a.nout) throw GfError(GF_EINVAL, "output array too small for this diagnostic");
+    const int nblocks = diag_launch(a);
+    launch_diag_finish(diag_part.as<double>(), nblocks, a.nout, scale, out_dev, stream);
+    HIPCHECK(hipGetLastError());
+    return a.nout;
+  }
+  int diagnostic(int kind, double xi, int nxi, double* out, int nout) override {
+    DiagArgs<T> a{};
+    double scale = 1;
+    diag_setup(kind, xi, nxi, a, scale);
+    if (nout < a.nout) throw GfError(GF_EINVAL, "output array too small for this diagnostic");
+    const int nblocks = diag_launch(a);
     std::vector<double> part((size_t)nblocks * a.nout);
     HIPCHECK(hipMemcpyAsync(part.data(), diag_part.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
@@ -1329,6 +1400,47 @@ int gf_diagnostic(gf_ctx* c, int kind, double xi, int nxi, double* out, int nout
   if (!out || nout < 1) return GF_EINVAL;
   int n = 0;
   const int rc = guarded(c, [&] { n = c->impl->diagnostic(kind, xi, nxi, out, nout); });
+  return rc == GF_OK ? n : rc;
+}
+
+int64_t gf_snapshot(gf_ctx* c, void* dst) {
+  if (!dst) return GF_EINVAL;
+  int64_t ticket = 0;
+  const int rc = guarded(c, [&] { ticket = c->impl->snapshot(dst); });
+  return rc == GF_OK ? ticket : rc;
+}
+int gf_snapshot_wait(gf_ctx* c, int64_t ticket) {
+  return guarded(c, [&] { c->impl->snapshot_wait(ticket); });
+}
+void* gf_host_alloc(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0 || hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+void gf_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+void* gf_device_alloc(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0 || hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  if (hipMemset(p, 0, bytes) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
+    (void)hipFree(p);
+    return nullptr;
+  }
+  return p;
+}
+void gf_device_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+int gf_device_copy(void* dst, const void* src, size_t bytes) {
+  if (!dst || !src) return GF_EINVAL;
+  return hipMemcpy(dst, src, bytes, hipMemcpyDefault) == hipSuccess ? GF_OK : GF_EHIP;
+}
+
+int gf_diagnostic_async(gf_ctx* c, int kind, double xi, int nxi, double* out_dev, int nout) {
+  if (!out_dev || nout < 1) return GF_EINVAL;
+  int n = 0;
+  const int rc = guarded(c, [&] { n = c->impl->diagnostic_async(kind, xi, nxi, out_dev, nout); });
   return rc == GF_OK ? n : rc;
 }
 

@@ -135,6 +135,21 @@ void launch_diag(const DiagArgs<T>& a, double* part, int nblocks, hipStream_t s)
   hipLaunchKernelGGL(diag_kernel<T>, dim3(nblocks), dim3(DIAG_THREADS), 0, s, a, part);
 }
 
+// Adds the per-block partials in block order and scales them, exactly as the host does for
+// gf_diagnostic, so a value recorded on the device equals the synchronous one bit for bit.
+__global__ __launch_bounds__(64) void diag_finish_kernel(const double* __restrict__ part, int nblocks, int nout,
+                                                         double scale, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= nout) return;
+  double s = 0;
+  for (int b = 0; b < nblocks; ++b) s += part[(long)b * nout + k];
+  out[k] = s * scale;
+}
+
+void launch_diag_finish(const double* part, int nblocks, int nout, double scale, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(diag_finish_kernel, dim3(1), dim3(64), 0, s, part, nblocks, nout, scale, out);
+}
+
 template void launch_diag<double>(const DiagArgs<double>&, double*, int, hipStream_t);
 template void launch_diag<float>(const DiagArgs<float>&, double*, int, hipStream_t);
 

@@ -197,6 +197,7 @@ struct DiagArgs {
   double lay_ke[MAXL], lay_pe[MAXL];   // MLQG: H_j/(2V), f0^2/(2 V g'_j)
 };
 template <typename T> void launch_diag(const DiagArgs<T>& a, double* part, int nblocks, hipStream_t s);
+void launch_diag_finish(const double* part, int nblocks, int nout, double scale, double* out, hipStream_t s);
 
 template <typename T> void launch_k1(int ny, const K1Args<T>& a, hipStream_t s);
 template <typename T> void launch_k2(int nx, const K2Args<T>& a, hipStream_t s);

@@ -11,7 +11,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .problem import GPU, Problem as _Problem, attach_forcing, dptr, make_desc, stepforward  # noqa: F401
+from .problem import GPU, Problem as _Problem, attach_forcing, device_diagnostic, dptr, make_desc, stepforward  # noqa: F401
 from .spectral import parsevalsum
 
 
@@ -82,6 +82,8 @@ def updatevars(prob):
     prob._updatevars()
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_LAYER_ENERGIES, 0.0, 0, 2 * p.nlayers - 1),
+                   unpack=lambda r: (np.array(r[: (len(r) + 1) // 2]), np.array(r[(len(r) + 1) // 2:])))
 def energies(prob):
     """Layer kinetic energies KE_j and interface potential energies PE_{j+1/2}
     (src/multilayerqg.jl:918-951), computed on the device: returns (KE, PE) arrays."""

@@ -142,6 +142,9 @@ class Problem:
         g = self.grid
         return (g.nl, g.nkr) if self.nlayers == 1 else (self.nlayers, g.nl, g.nkr)
 
+    def _sshape_bytes(self):
+        return int(np.prod(self._sshape)) * np.dtype(self._ctype).itemsize
+
     @property
     def _pshape(self):
         g = self.grid
@@ -202,12 +205,140 @@ class Problem:
         check(lib.gf_set_clock(self._ctx, float(t), int(step)), self._ctx)
 
 
-def stepforward(prob, nsteps=1):
-    """[FF] ``stepforward!(prob, nsteps)``: advance ``nsteps`` on the device.
+def stepforward(prob, nsteps=1, diags=None):
+    """[FF] ``stepforward!(prob, nsteps)`` and ``stepforward!(prob, diags, nsteps)``.
 
-    ``prob`` may be an ``Ensemble`` (every member advances, batched) or one of its
-    members (that member alone advances)."""
-    check(lib.gf_step(prob._ctx, int(nsteps)), prob._ctx)
+    Advances ``nsteps`` on the device.  ``prob`` may be an ``Ensemble`` (every member
+    advances, batched) or one of its members (that member alone advances).  With ``diags``
+    (a ``Diagnostic`` or a list of them; the reference's argument order
+    ``stepforward(prob, diags, nsteps)`` is accepted too) the problem advances one step at
+    a time and every diagnostic is incremented after each step, as [FF] does."""
+    if not isinstance(nsteps, (int, np.integer)):
+        diags, nsteps = nsteps, (1 if diags is None else diags)
+    if diags is None:
+        check(lib.gf_step(prob._ctx, int(nsteps)), prob._ctx)
+        return
+    if isinstance(diags, Diagnostic):
+        diags = [diags]
+    for _ in range(int(nsteps)):
+        check(lib.gf_step(prob._ctx, 1), prob._ctx)
+        for d in diags:
+            d.increment()
+
+
+def device_diagnostic(spec, unpack=None):
+    """Marks a diagnostic function as computable on the device: ``spec(prob)`` returns
+    ``(kind, xi, nxi, nout)`` for ``gf_diagnostic_async``; ``unpack`` turns one recorded row of
+    ``nout`` values into what the function itself returns."""
+    def wrap(fn):
+        fn._gf_device = spec
+        fn._gf_unpack = unpack
+        return fn
+    return wrap
+
+
+class Diagnostic:
+    """[FF] ``Diagnostic(calc, prob; freq=1, nsteps=100, ndata=ceil((nsteps+1)/freq))``.
+
+    ``calc(prob)`` is recorded at construction and then by ``increment()`` whenever
+    ``prob.clock.step`` is a multiple of ``freq``; ``d.i`` counts the records and ``d.t``,
+    ``d.steps`` hold their times and step numbers.  Indexing is 0-based (``d[0]`` is the
+    reference's ``d[1]``).
+
+    For this package's own diagnostics (``energy``, ``enstrophy``, ``energy_work``, ...)
+    each value is reduced on the device straight into a device-resident series
+    (``gf_diagnostic_async`` into ``gf_device_alloc`` memory): stepping with diagnostics copies
+    nothing to the host until ``data`` is read.  Any other callable is evaluated as ``calc(prob)`` after each step.
+    The series grows past ``ndata`` when needed, as the reference's does.
+    """
+
+    def __init__(self, calc, prob, freq=1, nsteps=100, ndata=None):
+        if freq < 1:
+            raise ValueError("freq must be >= 1")
+        self.calc, self.prob, self.freq = calc, prob, int(freq)
+        n = int(math.ceil((nsteps + 1) / freq)) if ndata is None else int(ndata)
+        n = max(n, 1)
+        self.i = 0
+        self.t = np.zeros(n)
+        self.steps = np.zeros(n, dtype=np.int64)
+        spec = getattr(calc, "_gf_device", None)
+        self._spec = spec(prob) if spec is not None and prob._ctx is not None else None
+        self._dev = None
+        if self._spec is not None:
+            self._dev = self._alloc(n)
+        else:
+            self._host = [None] * n
+        self.increment(force=True)
+
+    def _alloc(self, n):
+        p = lib.gf_device_alloc(n * self._spec[3] * 8)
+        if not p:
+            raise MemoryError("gf_device_alloc: device memory for a diagnostic series")
+        return p
+
+    def __del__(self):
+        dev = getattr(self, "_dev", None)
+        if dev:
+            if getattr(self.prob, "_ctx", None) is not None:
+                self.prob.synchronize()        # no record may still be in flight
+            lib.gf_device_free(ctypes.c_void_p(dev))
+            self._dev = None
+
+    @property
+    def ndata(self):
+        return len(self.t)
+
+    def _grow(self):
+        n = 2 * self.ndata
+        self.t = np.concatenate([self.t, np.zeros(n - len(self.t))])
+        self.steps = np.concatenate([self.steps, np.zeros(n - len(self.steps), dtype=np.int64)])
+        if self._spec is not None:
+            self.prob.synchronize()
+            dev = self._alloc(n)
+            check(lib.gf_device_copy(ctypes.c_void_p(dev), ctypes.c_void_p(self._dev), self.i * self._spec[3] * 8))
+            lib.gf_device_free(ctypes.c_void_p(self._dev))
+            self._dev = dev
+        else:
+            self._host += [None] * (n - len(self._host))
+
+    def increment(self, force=False):
+        """[FF] ``increment!(diag)``: record ``calc(prob)`` if the step is a multiple of ``freq``."""
+        step = self.prob.clock.step
+        if not force and step % self.freq != 0:
+            return
+        if self.i == self.ndata:
+            self._grow()
+        if self._spec is not None:
+            kind, xi, nxi, nout = self._spec
+            ptr = self._dev + self.i * nout * 8
+            n = lib.gf_diagnostic_async(self.prob._ctx, int(kind), float(xi), int(nxi), ctypes.c_void_p(ptr), nout)
+            check(min(n, 0), self.prob._ctx)
+        else:
+            self._host[self.i] = self.calc(self.prob)
+        self.t[self.i] = self.prob.clock.t
+        self.steps[self.i] = step
+        self.i += 1
+
+    @property
+    def data(self):
+        """The recorded values, ``data[:i]`` (device series copied out once per read)."""
+        if self._spec is None:
+            return list(self._host[: self.i])
+        self.prob.synchronize()
+        rows = np.empty((self.i, self._spec[3]))
+        check(lib.gf_device_copy(rows.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self._dev), rows.nbytes))
+        unpack = getattr(self.calc, "_gf_unpack", None)
+        if unpack is not None:
+            return [unpack(r) for r in rows]
+        return rows[:, 0].copy() if self._spec[3] == 1 else rows.copy()
+
+    def __getitem__(self, idx):
+        return self.data[idx]
+
+    @property
+    def value(self):
+        """The latest recorded value (the reference's ``diag.value``)."""
+        return self.data[self.i - 1]
 
 
 class Ensemble:

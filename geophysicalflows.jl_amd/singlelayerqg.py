@@ -10,7 +10,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .problem import GPU, Problem as _Problem, attach_forcing, dptr, make_desc, stepforward  # noqa: F401
+from .problem import GPU, Problem as _Problem, attach_forcing, device_diagnostic, dptr, make_desc, stepforward  # noqa: F401
 
 
 class Params:
@@ -67,41 +67,51 @@ def updatevars(prob):
 
 
 # Diagnostics (src/singlelayerqg.jl:546-715), computed on the device from the raw state.
+@device_diagnostic(lambda p: (_lib.GF_DIAG_KINETIC_ENERGY, 0.0, 0, 1))
 def kinetic_energy(prob):
     return prob._diagnostic(_lib.GF_DIAG_KINETIC_ENERGY)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_POTENTIAL_ENERGY, 0.0, 0, 1))
 def potential_energy(prob):
     return prob._diagnostic(_lib.GF_DIAG_POTENTIAL_ENERGY)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENERGY, 0.0, 0, 1))
 def energy(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENERGY)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENSTROPHY, 0.0, 0, 1))
 def enstrophy(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENERGY_DISSIPATION, p.params.ν, p.params.nν, 1))
 def energy_dissipation(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENERGY_DISSIPATION, prob.params.ν, prob.params.nν)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENSTROPHY_DISSIPATION, p.params.ν, p.params.nν, 1))
 def enstrophy_dissipation(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY_DISSIPATION, prob.params.ν, prob.params.nν)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENERGY_DRAG, p.params.μ, 0, 1))
 def energy_drag(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENERGY_DRAG, prob.params.μ, 0)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENSTROPHY_DRAG, p.params.μ, 0, 1))
 def enstrophy_drag(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY_DRAG, prob.params.μ, 0)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENERGY_WORK, 0.0, 0, 1))
 def energy_work(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENERGY_WORK)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENSTROPHY_WORK, 0.0, 0, 1))
 def enstrophy_work(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY_WORK)[0]

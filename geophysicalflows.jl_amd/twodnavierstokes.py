@@ -11,7 +11,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .problem import GPU, Problem as _Problem, attach_forcing, make_desc, stepforward  # noqa: F401
+from .problem import GPU, Problem as _Problem, attach_forcing, device_diagnostic, make_desc, stepforward  # noqa: F401
 
 
 class Params:
@@ -48,14 +48,17 @@ def updatevars(prob):
 
 
 # Diagnostics (src/twodnavierstokes.jl:374-535), computed on the device from the raw state.
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENERGY, 0.0, 0, 1))
 def energy(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENERGY)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENSTROPHY, 0.0, 0, 1))
 def enstrophy(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_PALINSTROPHY, 0.0, 0, 1))
 def palinstrophy(prob):
     return prob._diagnostic(_lib.GF_DIAG_PALINSTROPHY)[0]
 
@@ -68,26 +71,32 @@ def enstrophy_dissipation(prob, ξ, nξ):
     return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY_DISSIPATION, ξ, nξ)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENERGY_DISSIPATION, p.params.ν, p.params.nν, 1))
 def energy_dissipation_hyperviscosity(prob):
     return energy_dissipation(prob, prob.params.ν, prob.params.nν)
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENERGY_DISSIPATION, p.params.μ, p.params.nμ, 1))
 def energy_dissipation_hypoviscosity(prob):
     return energy_dissipation(prob, prob.params.μ, prob.params.nμ)
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENSTROPHY_DISSIPATION, p.params.ν, p.params.nν, 1))
 def enstrophy_dissipation_hyperviscosity(prob):
     return enstrophy_dissipation(prob, prob.params.ν, prob.params.nν)
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENSTROPHY_DISSIPATION, p.params.μ, p.params.nμ, 1))
 def enstrophy_dissipation_hypoviscosity(prob):
     return enstrophy_dissipation(prob, prob.params.μ, prob.params.nμ)
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENERGY_WORK, 0.0, 0, 1))
 def energy_work(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENERGY_WORK)[0]
 
 
+@device_diagnostic(lambda p: (_lib.GF_DIAG_ENSTROPHY_WORK, 0.0, 0, 1))
 def enstrophy_work(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY_WORK)[0]
 

@@ -185,6 +185,31 @@ enum {
 };
 int gf_diagnostic(gf_ctx* ctx, int kind, double xi, int nxi, double* out, int nout);
 
+/* The same diagnostic, written to device memory: out_dev holds nout doubles on the context's
+ * device. The reduction is queued on the context's stream and the call returns without waiting,
+ * so a time series ([FF] Diagnostic, increment!) stays on the device until it is read. The value
+ * equals gf_diagnostic's bit for bit. Returns the number of values written or a GF_E* code. */
+int gf_diagnostic_async(gf_ctx* ctx, int kind, double xi, int nxi, double* out_dev, int nout);
+
+/* Snapshot of the state for output ([FF] Output / saveoutput, which store prob.sol; e.g.
+ * examples/twodnavierstokes_decaying.jl:111-115). Queues a copy of sol, in the Julia layout, to
+ * the host buffer dst and returns a ticket >= 0 without waiting (or a negative GF_E* code).
+ * The copy runs on a second stream: gf_step calls made after gf_snapshot overlap it. dst
+ * must stay valid until gf_snapshot_wait(ctx, ticket) returns. The overlap needs pinned memory
+ * (gf_host_alloc); with pageable memory the copy is synchronous. */
+int64_t gf_snapshot(gf_ctx* ctx, void* dst);
+/* Wait until snapshot `ticket` is in host memory; ticket < 0 waits for every snapshot. */
+int gf_snapshot_wait(gf_ctx* ctx, int64_t ticket);
+/* Pinned host memory for snapshot buffers (hipHostMalloc); NULL on failure. */
+void* gf_host_alloc(size_t bytes);
+void gf_host_free(void* p);
+/* Device memory for callers without a device allocator of their own (the Python mirror's
+ * diagnostic series): zeroed hipMalloc, hipFree, and a synchronous copy in any direction. The
+ * copy does not wait for a context's stream: call gf_synchronize first. */
+void* gf_device_alloc(size_t bytes);
+void gf_device_free(void* p);
+int gf_device_copy(void* dst, const void* src, size_t bytes);
+
 /* Gradient of J = 1/2 sum_{x,y} (zeta(T) - target)^2 for the problem's current state as
  * zeta0 (set with gf_set_physical), where zeta(T) = irfft2(M sol_T) after nsteps ETDRK4 steps.
  *

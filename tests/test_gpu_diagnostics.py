@@ -146,3 +146,94 @@ def test_mlqg_energies_closed_form_and_oracle(gf):
     KEr, PEr = oml.energies(r3)
     for a, b in zip(list(KE) + list(PE), list(KEr) + list(PEr)):
         assert close(a, b), (a, b)
+
+
+def test_diagnostic_series_on_device_equals_synchronous_values(gf):
+    # [FF] Diagnostic / increment!: a series recorded on the device (gf_diagnostic_async) holds
+    # the values gf_diagnostic returns after each step, bit for bit
+    n, nsteps = 64, 12
+    T = gf.TwoDNavierStokes
+    kw = dict(nx=n, ν=1e-4, nν=2, μ=0.1, dt=0.005, stepper="RK4")
+    s = _rand(np.random.default_rng(64), (n, n // 2 + 1), 0.5)
+    s[0, 0] = 0
+    a = T.Problem(gf.GPU(), calcF=gf.RingForcing(6.0, 1.5, 0.1, seed=9), **kw)
+    b = T.Problem(gf.GPU(), calcF=gf.RingForcing(6.0, 1.5, 0.1, seed=9), **kw)
+    a.sol = s
+    b.sol = s
+    fns = [T.energy, T.enstrophy, T.energy_dissipation_hyperviscosity, T.energy_dissipation_hypoviscosity]
+    diags = [gf.Diagnostic(f, a, nsteps=nsteps) for f in fns]
+    work = gf.Diagnostic(T.energy_work, a, freq=3, nsteps=nsteps)
+    assert work.data[0] == 0          # no forcing realised before the first step (vars.Fh = 0)
+    gf.stepforward(a, diags + [work], nsteps)
+    want = [[f(b)] for f in fns]
+    wwork = []
+    for k in range(1, nsteps + 1):
+        gf.stepforward(b, 1)
+        for w, f in zip(want, fns):
+            w.append(f(b))
+        if k % 3 == 0:
+            wwork.append(T.energy_work(b))
+    for d, w in zip(diags, want):
+        assert d.i == nsteps + 1
+        assert np.array_equal(d.data, np.array(w))
+        assert np.allclose(d.t, 0.005 * np.arange(nsteps + 1), rtol=0, atol=1e-15)
+    assert np.array_equal(work.data[1:], np.array(wwork))
+    assert list(work.steps[:work.i]) == [0, 3, 6, 9, 12]
+
+
+def test_diagnostic_series_grows_and_unpacks_layer_energies(gf):
+    n = 64
+    p = gf.MultiLayerQG.Problem(3, gf.GPU(), nx=n, Lx=2 * np.pi, f0=1, H=[0.2, 0.3, 0.5], b=[0, -0.2, -0.27],
+                                dt=0.005, stepper="ETDRK4", U=[0.1, 0.02, 0.0], μ=0.01)
+    s = _rand(np.random.default_rng(65), (3, n, n // 2 + 1), 1e-3)
+    p.sol = s
+    d = gf.Diagnostic(gf.MultiLayerQG.energies, p, nsteps=2)        # ndata 3, grown twice below
+    gf.stepforward(p, d, 9)
+    assert d.i == 10 and d.ndata >= 10
+    KE, PE = d.data[-1]
+    KEn, PEn = gf.MultiLayerQG.energies(p)
+    assert np.array_equal(KE, KEn) and np.array_equal(PE, PEn)
+    assert len(KE) == 3 and len(PE) == 2
+
+
+def test_stochastic_energy_budget_on_device(gf):
+    """test/test_twodnavierstokes.jl:25-72 on the device: n = 256, dt = 0.005, nu = 1e-7 (nnu = 2),
+    mu = 0.1, kf = 12, dkf = 2, eps = 0.1, 200 steps of RK4 from rest; dE/dt = W + D + R to 1e-3
+    (Stratonovich work with prevsol).  The forcing is this library's ring forcing with complex
+    Gaussian amplitudes (the reference draws unit-modulus phases); the closure holds for both."""
+    n, dt, nt = 256, 0.005, 200
+    T = gf.TwoDNavierStokes
+    p = T.Problem(gf.GPU(), nx=n, Lx=2 * math.pi, ν=1e-7, nν=2, μ=0.1, nμ=0, dt=dt, stepper="RK4",
+                  calcF=gf.RingForcing(12.0, 2.0, 0.1, seed=1234))
+    T.set_ζ(p, np.zeros((n, n)))
+    E = gf.Diagnostic(T.energy, p, nsteps=nt)
+    D = gf.Diagnostic(T.energy_dissipation_hyperviscosity, p, nsteps=nt)
+    R = gf.Diagnostic(T.energy_dissipation_hypoviscosity, p, nsteps=nt)
+    W = gf.Diagnostic(T.energy_work, p, nsteps=nt)
+    gf.stepforward(p, [E, D, W, R], nt)
+    Ev, Dv, Rv, Wv = (np.asarray(x.data) for x in (E, D, R, W))
+    dEdt = (Ev[1:] - Ev[:-1]) / dt
+    computed = Wv[1:] + Dv[:-1] + Rv[:-1]
+    assert np.all(np.isfinite(Ev)) and Ev[-1] > 0
+    assert np.linalg.norm(dEdt - computed) / np.linalg.norm(dEdt) < 1e-3
+
+
+def test_energy_budget_series_matches_oracle(gf):
+    # the same budget series at 64^2 against the oracle's (60 steps, 1e-10 relative)
+    n, dt, nt = 64, 0.005, 60
+    T = gf.TwoDNavierStokes
+    p = T.Problem(gf.GPU(), nx=n, ν=1e-5, nν=2, μ=0.1, nμ=0, dt=dt, stepper="RK4",
+                  calcF=gf.RingForcing(6.0, 1.5, 0.1, seed=1234))
+    r = o2d.Problem(nx=n, nu=1e-5, nnu=2, mu=0.1, nmu=0, dt=dt, stepper="RK4", calcF=("ring", 6.0, 1.5, 0.1, 1234))
+    s = _rand(np.random.default_rng(66), (n, n // 2 + 1), 0.05)
+    s[0, 0] = 0
+    p.sol = s
+    r.sol[...] = s
+    fns = [(T.energy, o2d.energy), (T.energy_dissipation_hyperviscosity, o2d.energy_dissipation_hyperviscosity),
+           (T.energy_dissipation_hypoviscosity, o2d.energy_dissipation_hypoviscosity)]
+    dg = [gf.Diagnostic(f, p, nsteps=nt) for f, _ in fns]
+    do = [off.Diagnostic(g, r) for _, g in fns]
+    gf.stepforward(p, dg, nt)
+    off.stepforward(r, nt, diags=do)
+    for a, b in zip(dg, do):
+        assert np.linalg.norm(np.asarray(a.data) - np.asarray(b.data)) <= 1e-10 * np.linalg.norm(b.data)
