@@ -27,7 +27,7 @@ EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string"
            "gf_timing_read", "gf_kernel_bytes", "gf_version", "gf_ctx_create_batch",
            "gf_batch_size", "gf_ctx_instance", "gf_adjoint_grad", "gf_diagnostic",
            "gf_diagnostic_async", "gf_snapshot", "gf_snapshot_wait", "gf_host_alloc", "gf_host_free",
-           "gf_device_alloc", "gf_device_free", "gf_device_copy")
+           "gf_device_alloc", "gf_device_free", "gf_device_copy", "gf_use_graphs")
 
 
 class ProblemDesc(ctypes.Structure):
@@ -104,6 +104,7 @@ def _load():
     lib.gf_device_free.argtypes = [vp]
     lib.gf_device_free.restype = None
     lib.gf_device_copy.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.gf_use_graphs.argtypes = [vp, ip]
     for name in EXPORTS:
         getattr(lib, name)
     return lib

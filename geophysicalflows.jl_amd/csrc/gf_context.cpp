@@ -177,6 +177,7 @@ struct CtxBase {
   virtual void sync() = 0;
   virtual int64_t snapshot(void*) { throw GfError(GF_EINVAL, "snapshots are taken per problem; address one member through gf_ctx_instance"); }
   virtual void snapshot_wait(int64_t) {}
+  virtual void set_graphs(int) {}
   // ensembles (a plain context is an ensemble of one)
   virtual int ninstances() const { return 1; }
   virtual CtxBase* instance(int i) { return i == 0 ? this : nullptr; }
@@ -252,6 +253,7 @@ struct Ctx final : CtxBase {
   }
   ~Ctx() override {
     if (stream) { (void)hipStreamSynchronize(stream); if (own_stream) (void)hipStreamDestroy(stream); }
+    for (auto& ge : gexec) if (ge) (void)hipGraphExecDestroy(ge);
     if (copy_stream) {
       (void)hipStreamSynchronize(copy_stream);
       for (int k = 0; k < 2; ++k) { (void)hipEventDestroy(snap_ready[k]); (void)hipEventDestroy(snap_done[k]); }
@@ -652,9 +654,10 @@ struct Ctx final : CtxBase {
     }
   }
 
-  void calcN_launch(const C* in, int epi, bool full) {
+  void calcN_launch(const C* in, int epi, bool full, const int64_t* fstep_dev = nullptr) {
     K1Args<T> a; K2Args<T> b; K3Args<T> c;
     make_args(in, epi, full, a, b, c);
+    c.fstep_dev = fstep_dev;
     rec(0, true); launch_k1<T>(d.ny, a, stream); rec(0, false);
     rec(1, true); launch_k2<T>(d.nx, b, stream); rec(1, false);
     rec(2, true); launch_k3<T>(d.ny, c, stream); rec(2, false);
@@ -677,11 +680,49 @@ struct Ctx final : CtxBase {
     return 1;
   }
 
+  // ---------------- graphs ----------------
+  // Every step of a gf_step call but the last has the same launches and arguments (the last
+  // also computes the aliased band). That step is captured once into a HIP graph and replayed:
+  // one graph launch per step instead of 3 per stage. The ring forcing's step number, the
+  // one argument that changes, is read from step_dev, which the graph advances itself.
+  static constexpr int GRAPH_BLOCK = 8;   // interior steps in the larger graph
+  bool graphs = true;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};   // one interior step, GRAPH_BLOCK interior steps
+  DevBuf step_dev;
+
+  bool graph_usable() const { return graphs && !tm.on && !(ab3() && stepno < 3); }
+  void interior_launches(const int64_t* fstep_dev) {
+    const C* ins[4];
+    int epis[4];
+    const int ns = stages(ins, epis);
+    for (int k = 0; k < ns; ++k) calcN_launch(ins[k], epis[k], false, fstep_dev);
+    if (fstep_dev) launch_counter(step_dev.as<int64_t>(), 1, 1, stream);
+  }
+  void build_graph(int which) {
+    const bool ring = d.forcing == GF_FORCING_RING;
+    if (ring && !step_dev.p) step_dev.alloc(sizeof(int64_t));
+    hipGraph_t gr = nullptr;
+    HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+    try {
+      for (int s = 0; s < (which ? GRAPH_BLOCK : 1); ++s) interior_launches(ring ? step_dev.as<int64_t>() : nullptr);
+    } catch (...) {
+      (void)hipStreamEndCapture(stream, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      throw;
+    }
+    HIPCHECK(hipStreamEndCapture(stream, &gr));
+    const hipError_t e = hipGraphInstantiate(&gexec[which], gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) { gexec[which] = nullptr; throw GfError(GF_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+  }
+  void set_graphs(int enable) override { graphs = enable != 0; }
+
   void step(int n) override {
     const C* ins[4];
     int epis[4];
     const int ns = stages(ins, epis);
-    for (int s = 0; s < n; ++s) {
+    bool counter_set = false;
+    for (int s = 0; s < n;) {
       const bool full = s == n - 1;
       // the work diagnostics of stochastic forcing use the state at the start of the step
       // (prevsol, src/twodnavierstokes.jl:311-319); kept for the last step of the call
@@ -690,9 +731,23 @@ struct Ctx final : CtxBase {
         HIPCHECK(hipMemcpyAsync(prevsol.p, sol.p, sizeof(C) * fsz * nl, hipMemcpyDeviceToDevice, stream));
         prevsol_valid = true;
       }
-      for (int k = 0; k < ns; ++k) calcN_launch(ins[k], epis[k], full);
-      t += d.dt;
-      stepno += 1;
+      int done = 1;
+      if (!full && graph_usable()) {
+        const int which = (n - 1 - s >= GRAPH_BLOCK) ? 1 : 0;
+        if (!gexec[which]) build_graph(which);
+        if (d.forcing == GF_FORCING_RING && !counter_set) {
+          launch_counter(step_dev.as<int64_t>(), stepno, 0, stream);
+          counter_set = true;
+        }
+        HIPCHECK(hipGraphLaunch(gexec[which], stream));
+        done = which ? GRAPH_BLOCK : 1;
+      } else {
+        for (int k = 0; k < ns; ++k) calcN_launch(ins[k], epis[k], full);
+        counter_set = false;
+      }
+      s += done;
+      for (int j = 0; j < done; ++j) t += d.dt;   // the clock advances one dt at a time, as [FF]'s does
+      stepno += done;
     }
   }
 
@@ -1093,6 +1148,7 @@ struct BatchCtx final : CtxBase {
   }
   ~BatchCtx() override {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     inst.clear();
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -1133,22 +1189,50 @@ struct BatchCtx final : CtxBase {
     }
   }
 
+  void step_launches(int full) {
+    for (int k = 0; k < nst; ++k) {
+      const BatchItem<T>* it = items[k][full].as<BatchItem<T>>();
+      tm.rec(0, true, stream);
+      launch_batch<T, 1>(it, work[0][full].as<BatchWork>(), nwork[0][full], stream);
+      tm.rec(0, false, stream);
+      tm.rec(1, true, stream);
+      launch_batch<T, 2>(it, work[1][full].as<BatchWork>(), nwork[1][full], stream);
+      tm.rec(1, false, stream);
+      tm.rec(2, true, stream);
+      launch_batch<T, 3>(it, work[2][full].as<BatchWork>(), nwork[2][full], stream);
+      tm.rec(2, false, stream);
+    }
+    HIPCHECK(hipGetLastError());
+  }
+  // the interior step (all stages, every member) captured once and replayed, as for one problem
+  bool graphs = true;
+  hipGraphExec_t gexec = nullptr;
+  void set_graphs(int enable) override { graphs = enable != 0; }
+  void build_graph() {
+    hipGraph_t gr = nullptr;
+    HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+    try {
+      step_launches(0);
+    } catch (...) {
+      (void)hipStreamEndCapture(stream, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      throw;
+    }
+    HIPCHECK(hipStreamEndCapture(stream, &gr));
+    const hipError_t e = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) { gexec = nullptr; throw GfError(GF_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+  }
+
   void step(int n) override {
     for (int s = 0; s < n; ++s) {
       const int full = s == n - 1;
-      for (int k = 0; k < nst; ++k) {
-        const BatchItem<T>* it = items[k][full].as<BatchItem<T>>();
-        tm.rec(0, true, stream);
-        launch_batch<T, 1>(it, work[0][full].as<BatchWork>(), nwork[0][full], stream);
-        tm.rec(0, false, stream);
-        tm.rec(1, true, stream);
-        launch_batch<T, 2>(it, work[1][full].as<BatchWork>(), nwork[1][full], stream);
-        tm.rec(1, false, stream);
-        tm.rec(2, true, stream);
-        launch_batch<T, 3>(it, work[2][full].as<BatchWork>(), nwork[2][full], stream);
-        tm.rec(2, false, stream);
+      if (!full && graphs && !tm.on) {
+        if (!gexec) build_graph();
+        HIPCHECK(hipGraphLaunch(gexec, stream));
+      } else {
+        step_launches(full);
       }
-      HIPCHECK(hipGetLastError());
       for (auto& m : inst) { m->t += m->d.dt; m->stepno += 1; }
     }
     t = inst[0]->t;
@@ -1408,6 +1492,9 @@ int64_t gf_snapshot(gf_ctx* c, void* dst) {
   int64_t ticket = 0;
   const int rc = guarded(c, [&] { ticket = c->impl->snapshot(dst); });
   return rc == GF_OK ? ticket : rc;
+}
+int gf_use_graphs(gf_ctx* c, int enable) {
+  return guarded(c, [&] { c->impl->set_graphs(enable); });
 }
 int gf_snapshot_wait(gf_ctx* c, int64_t ticket) {
   return guarded(c, [&] { c->impl->snapshot_wait(ticket); });

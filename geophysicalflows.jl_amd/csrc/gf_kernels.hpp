@@ -104,6 +104,7 @@ struct K3Args {
   const cplx<T>* Fh;        // fixed forcing, nullable
   const T* sqrt_spec;       // ring forcing amplitude (blocked real), nullable
   uint64_t seed; int64_t fstep; T sqrt_dt; int fkx, fky; int instance;
+  const int64_t* fstep_dev; // nullable: the forcing step is read here (steps replayed from a graph)
   // MLQG bottom drag mu Krsq psi_n
   T mu_drag; const cplx<T>* drag_src;  // calcN input state (for psi_n)
   const T* sinv;
@@ -205,6 +206,7 @@ template <typename T> void launch_k3(int ny, const K3Args<T>& a, hipStream_t s);
 template <typename T> void launch_to_julia(const Geo<T>& g, const cplx<T>* src, cplx<T>* dst, int nl, hipStream_t s);
 template <typename T> void launch_from_julia(const Geo<T>& g, const cplx<T>* src, cplx<T>* dst, int nl, hipStream_t s);
 template <typename T> void launch_mask(const Geo<T>& g, cplx<T>* sol, int nl, hipStream_t s);
+void launch_counter(int64_t* c, int64_t v, int add, hipStream_t s);
 bool size_supported(int n);
 
 }  // namespace gf

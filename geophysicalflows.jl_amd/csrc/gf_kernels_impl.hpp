@@ -635,7 +635,8 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
         if (kr <= a.fkx && (l <= a.fky || l >= NY - a.fky)) {
           const T sp = at(a.sqrt_spec, idx);
           if (sp != (T)0) {
-            const cplx<T> xi = complex_normal<T>(a.seed, a.fstep, (uint32_t)(l * g.nkr + kr), (uint32_t)a.instance);
+            const int64_t fstep = a.fstep_dev != nullptr ? a.fstep_dev[0] : a.fstep;
+            const cplx<T> xi = complex_normal<T>(a.seed, fstep, (uint32_t)(l * g.nkr + kr), (uint32_t)a.instance);
             N = N + mk((xi.re * sp) / a.sqrt_dt, (xi.im * sp) / a.sqrt_dt);
           }
         }

@@ -48,6 +48,15 @@ template <typename T> void launch_mask(const Geo<T>& g, cplx<T>* sol, int nl, hi
   hipLaunchKernelGGL(mask_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, sol, nl);
 }
 
+// Forcing step counter of a captured step: set once before the replays (outside the graph),
+// advanced by one at the end of each replayed step (inside it). One lane stores (a vector store).
+__global__ void counter_kernel(int64_t* c, int64_t v, int add) {
+  if (threadIdx.x == 0) c[0] = add ? c[0] + v : v;
+}
+void launch_counter(int64_t* c, int64_t v, int add, hipStream_t s) {
+  hipLaunchKernelGGL(counter_kernel, dim3(1), dim3(64), 0, s, c, v, add);
+}
+
 bool size_supported(int n) {
   return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048;
 }

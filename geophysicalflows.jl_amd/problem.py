@@ -195,6 +195,10 @@ class Problem:
     def synchronize(self):
         check(lib.gf_synchronize(self._ctx), self._ctx)
 
+    def use_graphs(self, enable=True):
+        """Replay the captured step graph in ``stepforward`` (default) or launch every kernel."""
+        check(lib.gf_use_graphs(self._ctx, int(bool(enable))), self._ctx)
+
     def _diagnostic(self, kind, xi=0.0, nxi=0, nout=1):
         out = (ctypes.c_double * nout)()
         n = lib.gf_diagnostic(self._ctx, int(kind), float(xi), int(nxi), out, nout)
@@ -386,6 +390,10 @@ class Ensemble:
 
     def synchronize(self):
         check(lib.gf_synchronize(self._ctx), self._ctx)
+
+    def use_graphs(self, enable=True):
+        """Replay the captured step graph in ``stepforward`` (default) or launch every kernel."""
+        check(lib.gf_use_graphs(self._ctx, int(bool(enable))), self._ctx)
 
     def _diagnostic(self, kind, xi=0.0, nxi=0, nout=1):
         out = (ctypes.c_double * nout)()

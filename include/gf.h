@@ -121,8 +121,15 @@ int gf_calcN(gf_ctx* ctx, void* N, void* sol, double t);
  * test/test_twodnavierstokes.jl:239-243); Julia layout, host. */
 int gf_set_forcing(gf_ctx* ctx, const void* Fh);
 
-/* [FF] stepforward!(prob, nsteps): nsteps of the configured stepper. */
+/* [FF] stepforward!(prob, nsteps): nsteps of the configured stepper, queued on the context's
+ * stream (the call returns before they finish; every read of the state waits for them).
+ * Every step but the call's last replays one HIP graph of the whole step (all stages, or all
+ * stages of every ensemble member), captured at the first such step. */
 int gf_step(gf_ctx* ctx, int nsteps);
+
+/* Turn the graph replay of gf_step off (0) or on (1, the default); results are identical. No
+ * counterpart in the reference. */
+int gf_use_graphs(gf_ctx* ctx, int enable);
 
 /* [FF] prob.clock */
 int gf_clock(const gf_ctx* ctx, double* t, int64_t* step);
