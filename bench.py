@@ -31,7 +31,7 @@ KERNELS = ("y_inverse_K1", "x_pass_K2", "y_forward_stage_K3")
 # kernel symbols of the FP64 2048^2 headline path, as rocprofv3 names them
 SYMBOLS = {"y_inverse_K1": "k1_kernel<double, 2048, 8, 1, 1, 0, 0>",
            "x_pass_K2": "k2_kernel<double, 2048, 8, 1, 0, 1>",
-           "y_forward_stage_K3": "k3_kernel<double, 2048, 8, 1, 1>"}
+           "y_forward_stage_K3": "k3_kernel<double, 2048, 4, 1, 1>"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "twodns_2048_fp64_etdrk4_hbm_traffic.md")
 
 
@@ -80,6 +80,7 @@ def gpu_run(gf, steps, warmup, T, dt=CFG["dt"]):
                           achieved_GBps=per_launch / avg_s / 1e9)
     lib.gf_timing(prob._ctx, 0)
     finite = bool(np.isfinite(prob.sol).all())
+    kern["schedule"] = list(gf.schedule(prob))
     return elapsed, kern, finite
 
 
@@ -354,6 +355,7 @@ def main():
     gf = ge.load_package()
     steps, warmup = max(1, args.steps), max(0, args.warmup)
     elapsed, kern, finite = gpu_run(gf, steps, warmup, np.float64)
+    sched = kern.pop("schedule")
     ms_per_step = elapsed / steps * 1e3
     value = NX * NX * steps / elapsed
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
@@ -378,7 +380,8 @@ def main():
         "data": "synthetic (zeta0 = 0, counter-based ring forcing, seed 2)",
         "config": {"workload": "TwoDNavierStokes 2048x2048 L=2pi ETDRK4 dt=0.005 nu=1e-21 nnu=4 mu=0.1 "
                                "ring forcing kf=14 dkf=1.5 eps=0.1", "nx": NX, "ny": NX, "stepper": "ETDRK4",
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "fft_points_per_thread_k1_k2_k3": sched,
+                   "schedule_source": "geophysicalflows.jl_amd/schedules.json (autotuned on MI355X)"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["achieved_GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dk["achieved_GBps"] / HBM_PEAK_GBS,
                      "bytes_per_launch": dk["bytes_per_launch"], "avg_us": dk["avg_us"], "traffic": traffic.get(dom),
@@ -401,9 +404,11 @@ def main():
         out["adjoint_512_batch16"] = adjoint_batch_run(gf)
     if not args.no_fp32:
         e32, k32, f32 = gpu_run(gf, steps, warmup, np.float32, dt=0.001)
+        sched32 = k32.pop("schedule")
         s32 = 8.0 * (NX // 2 + 1) * NX
         out["fp32"] = {"workload": "the same forced 2048^2 problem in Float32 (dt = 0.001, which stays finite)",
                        "value": NX * NX * steps / e32, "ms_per_step": e32 / steps * 1e3, "state_finite": f32,
+                       "fft_points_per_thread_k1_k2_k3": sched32,
                        "step_model_frac_of_peak": 64 * s32 / (e32 / steps) / 1e9 / HBM_PEAK_GBS,
                        "kernels": {k: {"avg_us": v["avg_us"], "achieved_GBps": v["achieved_GBps"]} for k, v in k32.items()}}
     if not args.no_cpu:

@@ -13,8 +13,10 @@ from . import twodnavierstokes as TwoDNavierStokes  # noqa: F401
 from . import output  # noqa: F401
 from . import utils  # noqa: F401
 from .output import Output, get_sol, savediagnostic, saveoutput, saveproblem  # noqa: F401
-from .problem import CPU, GPU, Diagnostic, Ensemble, RingForcing, stepforward  # noqa: F401
+from .problem import (CPU, GPU, Diagnostic, Ensemble, RingForcing, autotune, load_schedules,  # noqa: F401
+                      save_schedules, schedule, set_schedule, stepforward)
 from .utils import peakedisotropicspectrum  # noqa: F401
 
 __all__ = ["TwoDNavierStokes", "SingleLayerQG", "MultiLayerQG", "GPU", "CPU", "Ensemble", "RingForcing", "Diagnostic", "stepforward",
-           "peakedisotropicspectrum", "Output", "get_sol", "saveproblem", "saveoutput", "savediagnostic"]
+           "peakedisotropicspectrum", "Output", "get_sol", "saveproblem", "saveoutput", "savediagnostic",
+           "autotune", "schedule", "set_schedule", "save_schedules", "load_schedules"]

@@ -27,7 +27,9 @@ EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string"
            "gf_timing_read", "gf_kernel_bytes", "gf_version", "gf_ctx_create_batch",
            "gf_batch_size", "gf_ctx_instance", "gf_adjoint_grad", "gf_diagnostic",
            "gf_diagnostic_async", "gf_snapshot", "gf_snapshot_wait", "gf_host_alloc", "gf_host_free",
-           "gf_device_alloc", "gf_device_free", "gf_device_copy", "gf_use_graphs")
+           "gf_device_alloc", "gf_device_free", "gf_device_copy", "gf_use_graphs",
+           "gf_set_schedule", "gf_get_schedule", "gf_autotune", "gf_schedules_save", "gf_schedules_load",
+           "gf_schedules_clear", "gf_schedules_error")
 
 
 class ProblemDesc(ctypes.Structure):
@@ -105,6 +107,15 @@ def _load():
     lib.gf_device_free.restype = None
     lib.gf_device_copy.argtypes = [vp, vp, ctypes.c_size_t]
     lib.gf_use_graphs.argtypes = [vp, ip]
+    lib.gf_set_schedule.argtypes = [vp, ip, ip, ip]
+    lib.gf_get_schedule.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
+    lib.gf_autotune.argtypes = [vp, ip, dp]
+    lib.gf_schedules_save.argtypes = [ctypes.c_char_p]
+    lib.gf_schedules_load.argtypes = [ctypes.c_char_p]
+    lib.gf_schedules_clear.argtypes = []
+    lib.gf_schedules_clear.restype = None
+    lib.gf_schedules_error.argtypes = []
+    lib.gf_schedules_error.restype = ctypes.c_char_p
     for name in EXPORTS:
         getattr(lib, name)
     return lib

@@ -3,14 +3,17 @@
 // The context mirrors what FourierFlows.Problem builds for the reference
 // (grid, Equation.L, time-stepper coefficients and buffers, clock) and drives
 // the fused kernels of gf_kernels_impl.hpp on one HIP stream.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -19,6 +22,7 @@
 #include <vector>
 
 #include "gf_kernels.hpp"
+#include "gf_schedule.hpp"
 #include "../../include/gf.h"
 
 using namespace gf;
@@ -178,6 +182,9 @@ struct CtxBase {
   virtual int64_t snapshot(void*) { throw GfError(GF_EINVAL, "snapshots are taken per problem; address one member through gf_ctx_instance"); }
   virtual void snapshot_wait(int64_t) {}
   virtual void set_graphs(int) {}
+  virtual void set_schedule(const int*) { throw GfError(GF_EUNSUPPORTED, "radix schedules are set per problem"); }
+  virtual void get_schedule(int*) const { throw GfError(GF_EUNSUPPORTED, "radix schedules are set per problem"); }
+  virtual void autotune(int, double*) { throw GfError(GF_EUNSUPPORTED, "the autotuner times one problem; address a member through gf_ctx_instance"); }
   // ensembles (a plain context is an ensemble of one)
   virtual int ninstances() const { return 1; }
   virtual CtxBase* instance(int i) { return i == 0 ? this : nullptr; }
@@ -249,6 +256,8 @@ struct Ctx final : CtxBase {
     build_stepper();
     build_forcing();
     if (d.forcing == GF_FORCING_FIXED) Fh.alloc(sizeof(C) * fsz * nl);   // zero until gf_set_forcing
+    Sched tuned;
+    if (nl == 1 && sched_lookup(sched_key(), tuned)) for (int k = 0; k < 3; ++k) sched[k] = tuned.p[k];
     HIPCHECK(hipStreamSynchronize(stream));
   }
   ~Ctx() override {
@@ -641,6 +650,7 @@ struct Ctx final : CtxBase {
     c.c0 = coef[0].p; c.c1 = coef[1].p; c.c2 = coef[2].p; c.c3 = coef[3].p; c.c4 = coef[4].p; c.c5 = coef[5].p;
     c.ccplx = ccplx; c.cl_layered = 0; c.dt = (T)d.dt; c.filter = filtered() ? filter.as<T>() : nullptr;
     c.tw = twy.as<C>();
+    a.p = sched[0]; b.p = sched[1]; c.p = sched[2];
     if (ab3()) {
       // the RHS history of the aliased band enters later steps: every AB3 step is computed in full
       c.ab3_start = stepno < 3;
@@ -678,6 +688,83 @@ struct Ctx final : CtxBase {
     }
     ins[0] = sol.as<C>(); epis[0] = ab3() ? E_AB3 : E_FE;
     return 1;
+  }
+
+  // ---------------- radix schedules and the autotuner ----------------
+  // sched[k]: FFT points per thread of K1, K2, K3 (0: the built-in choice); single-layer models.
+  int sched[3] = {0, 0, 0};
+  SchedKey sched_key() const { return SchedKey{d.model, nl, d.nx, d.ny, d.dtype}; }
+  int sched_len(int k) const { return k == 1 ? d.nx : d.ny; }   // K2 transforms rows, K1/K3 columns
+  void set_schedule(const int* p) override {
+    for (int k = 0; k < 3; ++k)
+      if (p[k] != 0 && !(nl == 1 && sched_candidate(sched_len(k), p[k])))
+        throw GfError(GF_EINVAL, "radix schedule: P = " + std::to_string(p[k]) + " is not offered for kernel " +
+                                 std::to_string(k + 1) + " of this problem");
+    bool changed = false;
+    for (int k = 0; k < 3; ++k) { changed |= sched[k] != p[k]; sched[k] = p[k]; }
+    if (changed)   // the captured steps hold the old launches
+      for (auto& ge : gexec) if (ge) { HIPCHECK(hipStreamSynchronize(stream)); (void)hipGraphExecDestroy(ge); ge = nullptr; }
+  }
+  void get_schedule(int* p) const override {
+    for (int k = 0; k < 3; ++k) p[k] = sched[k] ? sched[k] : (nl == 1 ? sched_default(sched_len(k)) : (d.ny >= 32 && k != 1 ? 4 : sched_default(sched_len(k))));
+  }
+  // Times every offered schedule on this problem (the state, its history and the clock are
+  // restored afterwards), keeps the fastest per kernel, and records it for this (model, size,
+  // dtype) so later contexts start with it.  us (nullable) receives 2 x 3 average launch times
+  // in microseconds, P = 4 then P = 8, NaN where a P is not offered.
+  void autotune(int steps, double* us) override {
+    if (steps < 1) throw GfError(GF_EINVAL, "gf_autotune: steps must be >= 1");
+    const int cand[2] = {4, 8};
+    if (us) for (int j = 0; j < 6; ++j) us[j] = std::nan("");
+    if (nl != 1) return;   // MultiLayerQG keeps its fixed schedule
+    HIPCHECK(hipStreamSynchronize(stream));
+    const size_t cb = sizeof(C) * fsz * nl;
+    DevBuf save[4];
+    DevBuf* keep[4] = {&sol, &racc, &n1, &prevsol};
+    for (int j = 0; j < 4; ++j)
+      if (keep[j]->p) { save[j].alloc(cb); HIPCHECK(hipMemcpy(save[j].p, keep[j]->p, cb, hipMemcpyDeviceToDevice)); }
+    const double t0 = t;
+    const int64_t step0 = stepno;
+    const bool pv0 = prevsol_valid;
+    const int old[3] = {sched[0], sched[1], sched[2]};
+    double best[3] = {INFINITY, INFINITY, INFINITY};
+    int pick[3] = {old[0], old[1], old[2]};
+    try {
+      for (int c = 0; c < 2; ++c) {
+        int q[3];
+        bool any = false;
+        for (int k = 0; k < 3; ++k) {
+          q[k] = sched_candidate(sched_len(k), cand[c]) ? cand[c] : 0;
+          any |= q[k] != 0;
+        }
+        if (!any) continue;
+        set_schedule(q);
+        step(2);
+        tm.reset(1, stream);
+        step(steps);
+        for (int k = 0; k < 3; ++k) {
+          if (!q[k]) continue;
+          int64_t n = 0;
+          double ms = 0;
+          tm.read(k, &n, &ms, stream);
+          const double a = n ? ms * 1e3 / n : INFINITY;
+          if (us) us[c * 3 + k] = a;
+          if (a < best[k]) { best[k] = a; pick[k] = q[k]; }
+        }
+        tm.reset(0, stream);
+      }
+    } catch (...) {
+      tm.reset(0, stream);
+      set_schedule(old);
+      throw;
+    }
+    for (int j = 0; j < 4; ++j)
+      if (save[j].p) HIPCHECK(hipMemcpy(keep[j]->p, save[j].p, cb, hipMemcpyDeviceToDevice));
+    t = t0; stepno = step0; prevsol_valid = pv0;
+    set_schedule(pick);
+    Sched e;
+    for (int k = 0; k < 3; ++k) { e.p[k] = pick[k]; e.us[k] = std::isfinite(best[k]) ? best[k] : 0; }
+    sched_store(sched_key(), e);
   }
 
   // ---------------- graphs ----------------
@@ -1354,8 +1441,33 @@ const char* gf_error_string(int code) {
   }
 }
 
+// The schedule table shipped beside the library (schedules.json in the directory of
+// libgfhip.so), or the file named by GF_SCHEDULES, is read once, before the first context is
+// created. A missing default file is not an error; a bad one is reported by gf_schedules_error.
+static std::string g_sched_err;
+static void autoload_schedules() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    std::string path;
+    if (const char* e = std::getenv("GF_SCHEDULES")) {
+      path = e;
+    } else {
+      Dl_info info;
+      if (dladdr((void*)&autoload_schedules, &info) == 0 || !info.dli_fname) return;
+      path = info.dli_fname;
+      const size_t slash = path.rfind('/');
+      path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/schedules.json";
+      if (FILE* f = std::fopen(path.c_str(), "rb")) std::fclose(f);
+      else return;
+    }
+    try { gf::sched_load(path); }
+    catch (const std::exception& ex) { g_sched_err = ex.what(); }
+  });
+}
+
 int gf_ctx_create(gf_ctx** out, const gf_problem_desc* desc) {
   if (!out) return GF_EINVAL;
+  autoload_schedules();
   *out = nullptr;
   std::string msg;
   int rc = validate(desc, msg);
@@ -1378,6 +1490,7 @@ int gf_ctx_create(gf_ctx** out, const gf_problem_desc* desc) {
 
 int gf_ctx_create_batch(gf_ctx** out, const gf_problem_desc* descs, int n) {
   if (!out) return GF_EINVAL;
+  autoload_schedules();
   *out = nullptr;
   std::string msg;
   int rc = validate_batch(descs, n, msg);
@@ -1493,6 +1606,36 @@ int64_t gf_snapshot(gf_ctx* c, void* dst) {
   const int rc = guarded(c, [&] { ticket = c->impl->snapshot(dst); });
   return rc == GF_OK ? ticket : rc;
 }
+int gf_set_schedule(gf_ctx* c, int p_k1, int p_k2, int p_k3) {
+  const int p[3] = {p_k1, p_k2, p_k3};
+  return guarded(c, [&] { c->impl->set_schedule(p); });
+}
+int gf_get_schedule(gf_ctx* c, int* p_k1, int* p_k2, int* p_k3) {
+  int p[3] = {0, 0, 0};
+  const int rc = guarded(c, [&] { c->impl->get_schedule(p); });
+  if (rc == GF_OK) {
+    if (p_k1) *p_k1 = p[0];
+    if (p_k2) *p_k2 = p[1];
+    if (p_k3) *p_k3 = p[2];
+  }
+  return rc;
+}
+int gf_autotune(gf_ctx* c, int steps, double* us) {
+  return guarded(c, [&] { c->impl->autotune(steps, us); });
+}
+int gf_schedules_save(const char* path) {
+  if (!path) return GF_EINVAL;
+  try { gf::sched_save(path); g_sched_err.clear(); return GF_OK; }
+  catch (const std::exception& e) { g_sched_err = e.what(); return GF_EINVAL; }
+}
+int gf_schedules_load(const char* path) {
+  if (!path) return GF_EINVAL;
+  autoload_schedules();
+  try { const int n = gf::sched_load(path); g_sched_err.clear(); return n; }
+  catch (const std::exception& e) { g_sched_err = e.what(); return GF_EINVAL; }
+}
+void gf_schedules_clear(void) { autoload_schedules(); gf::sched_clear(); }
+const char* gf_schedules_error(void) { return g_sched_err.c_str(); }
 int gf_use_graphs(gf_ctx* c, int enable) {
   return guarded(c, [&] { c->impl->set_graphs(enable); });
 }

@@ -54,6 +54,7 @@ struct K1Args {
   Geo<T> g;
   int model;
   int fieldset;             // FS_CALCN (u, v, q), FS_OUT (updatevars! outputs), FS_RAW (state)
+  int p;                    // FFT points per thread (radix schedule); 0: the built-in choice
   const cplx<T>* sol;       // input state (masked on read)
   cplx<T>* out;             // fields of every layer, stride fstride
   long fstride;
@@ -70,6 +71,7 @@ struct K2Args {
   Geo<T> g;
   int mode;
   int nfields;              // inverse-out / forward-in: number of fields per layer
+  int p;                    // FFT points per thread (radix schedule); 0: the built-in choice
   const cplx<T>* in;        // mixed fields (u, v, q per layer), stride fstride
   long fstride;
   int kin;                  // kr < kin present in the input mixed fields
@@ -95,6 +97,7 @@ struct K3Args {
   Geo<T> g;
   int model;
   const cplx<T>* in;        // flux fields (2 per layer), stride fstride
+  int p;                    // FFT points per thread (radix schedule); 0: the built-in choice
   long fstride;
   int ncols;                // kr columns to process (kr < ncols)
   int full;                 // also produce the aliased band (last step / diagnostics)
@@ -207,6 +210,10 @@ template <typename T> void launch_to_julia(const Geo<T>& g, const cplx<T>* src, 
 template <typename T> void launch_from_julia(const Geo<T>& g, const cplx<T>* src, cplx<T>* dst, int nl, hipStream_t s);
 template <typename T> void launch_mask(const Geo<T>& g, cplx<T>* sol, int nl, hipStream_t s);
 void launch_counter(int64_t* c, int64_t v, int add, hipStream_t s);
+// Radix schedules offered to the autotuner for an FFT of length n (the host side of PCand).
+inline bool sched_candidate(int n, int p) { return p == 4 ? n >= 16 : (p == 8 ? n >= 32 : false); }
+// The built-in choice (PY in the kernels): P = 4 below 32 points, else 8.
+inline int sched_default(int n) { return n < 32 ? 4 : 8; }
 bool size_supported(int n);
 
 }  // namespace gf

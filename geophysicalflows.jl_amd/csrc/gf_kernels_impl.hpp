@@ -771,9 +771,16 @@ template <int N, int P> struct GW {
 // K1/K3: G column pairs of 2 N/P threads each, at least one wave per workgroup
 template <int N, int P> struct GY { static constexpr int G = 2 * (N / P) >= 64 ? 1 : 64 / (2 * (N / P)); };
 
-template <typename T, int N, int NL>
-void k1_n(const K1Args<T>& a, hipStream_t s) {
-  constexpr int P = (NL > 1 && N >= 32) ? 4 : PY<N>::P;
+// Radix schedules the autotuner may choose for the single-layer calcN kernels: FFT points per
+// thread P (P = 4 doubles the threads per transform and halves each thread's butterfly chain
+// and register footprint).  P = 16 is not offered: at 2048 it needs all 256 VGPRs and spills in
+// K2.  Every other path uses the built-in choice.  Keep in step with sched_candidate() on the host.
+template <int N> struct PCand {
+  static constexpr bool has(int p) { return p == 4 ? N >= 16 : (p == 8 ? N >= 32 : false); }
+};
+
+template <typename T, int N, int NL, int P>
+void k1_np(const K1Args<T>& a, hipStream_t s) {
   constexpr int G = GY<N, P>::G;
   const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
   if (nblk <= 0) return;
@@ -782,6 +789,9 @@ void k1_n(const K1Args<T>& a, hipStream_t s) {
     if (a.fieldset == FS_CALCN) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_CALCN>), grid, blk, 0, s, a);
     else if (a.fieldset == FS_OUT) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_OUT>), grid, blk, 0, s, a);
     else hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_RAW>), grid, blk, 0, s, a);
+  } else if constexpr (P != PY<N>::P) {   // a tuned schedule: the calcN field set only
+    if (a.model == M_TWODNS) hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_CALCN>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_SLQG, FS_CALCN>), grid, blk, 0, s, a);
   } else {
     if (a.model == M_TWODNS) {
       if (a.fieldset == FS_CALCN) hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_CALCN>), grid, blk, 0, s, a);
@@ -795,24 +805,60 @@ void k1_n(const K1Args<T>& a, hipStream_t s) {
   }
 }
 template <typename T, int N, int NL>
-void k2_n(const K2Args<T>& a, hipStream_t s) {
-  constexpr int P = PY<N>::P;
+void k1_n(const K1Args<T>& a, hipStream_t s) {
+  if constexpr (NL > 1) {
+    k1_np<T, N, NL, (N >= 32 ? 4 : PY<N>::P)>(a, s);
+  } else {
+    const bool tuned = a.fieldset == FS_CALCN && a.p != 0 && a.p != PY<N>::P && PCand<N>::has(a.p);
+    if (tuned && a.p == 4) { if constexpr (PCand<N>::has(4) && PY<N>::P != 4) { k1_np<T, N, 1, 4>(a, s); return; } }
+    if (tuned && a.p == 8) { if constexpr (PCand<N>::has(8) && PY<N>::P != 8) { k1_np<T, N, 1, 8>(a, s); return; } }
+    k1_np<T, N, 1, PY<N>::P>(a, s);
+  }
+}
+template <typename T, int N, int NL, int P>
+void k2_np(const K2Args<T>& a, hipStream_t s) {
   constexpr int G = GW<N, P>::G;
   const dim3 grid((a.g.ny / 2 + G - 1) / G), blk(G * (N / P));
-  switch (a.mode) {
-    case K2_ADVECT: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT, 1>), grid, blk, 0, s, a); break;
-    case K2_ADVECT_FULL: hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT_FULL, NL>), grid, blk, 0, s, a); break;
-    case K2_INVERSE_OUT: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_INVERSE_OUT, 1>), grid, blk, 0, s, a); break;
-    case K2_FORWARD_IN: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_FORWARD_IN, 1>), grid, blk, 0, s, a); break;
-    default: break;
+  if constexpr (P != PY<N>::P) {          // a tuned schedule: the calcN modes only
+    if (a.mode == K2_ADVECT) { if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT, 1>), grid, blk, 0, s, a); }
+    else hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT_FULL, NL>), grid, blk, 0, s, a);
+  } else {
+    switch (a.mode) {
+      case K2_ADVECT: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT, 1>), grid, blk, 0, s, a); break;
+      case K2_ADVECT_FULL: hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT_FULL, NL>), grid, blk, 0, s, a); break;
+      case K2_INVERSE_OUT: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_INVERSE_OUT, 1>), grid, blk, 0, s, a); break;
+      case K2_FORWARD_IN: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_FORWARD_IN, 1>), grid, blk, 0, s, a); break;
+      default: break;
+    }
   }
 }
 template <typename T, int N, int NL>
-void k3_n(const K3Args<T>& a, hipStream_t s) {
-  constexpr int P = (NL > 1 && N >= 32) ? 4 : PY<N>::P;
+void k2_n(const K2Args<T>& a, hipStream_t s) {
+  const bool tuned = NL == 1 && (a.mode == K2_ADVECT || a.mode == K2_ADVECT_FULL) && a.p != 0 &&
+                     a.p != PY<N>::P && PCand<N>::has(a.p);
+  if constexpr (NL == 1) {
+    if (tuned && a.p == 4) { if constexpr (PCand<N>::has(4) && PY<N>::P != 4) { k2_np<T, N, 1, 4>(a, s); return; } }
+    if (tuned && a.p == 8) { if constexpr (PCand<N>::has(8) && PY<N>::P != 8) { k2_np<T, N, 1, 8>(a, s); return; } }
+  }
+  (void)tuned;
+  k2_np<T, N, NL, PY<N>::P>(a, s);
+}
+template <typename T, int N, int NL, int P>
+void k3_np(const K3Args<T>& a, hipStream_t s) {
   constexpr int G = GY<N, P>::G;
   const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
   if (nblk > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, G, NL>), dim3(nblk), dim3(2 * G * (N / P)), 0, s, a);
+}
+template <typename T, int N, int NL>
+void k3_n(const K3Args<T>& a, hipStream_t s) {
+  if constexpr (NL > 1) {
+    k3_np<T, N, NL, (N >= 32 ? 4 : PY<N>::P)>(a, s);
+  } else {
+    const bool tuned = a.p != 0 && a.p != PY<N>::P && PCand<N>::has(a.p);
+    if (tuned && a.p == 4) { if constexpr (PCand<N>::has(4) && PY<N>::P != 4) { k3_np<T, N, 1, 4>(a, s); return; } }
+    if (tuned && a.p == 8) { if constexpr (PCand<N>::has(8) && PY<N>::P != 8) { k3_np<T, N, 1, 8>(a, s); return; } }
+    k3_np<T, N, 1, PY<N>::P>(a, s);
+  }
 }
 
 }  // namespace gf

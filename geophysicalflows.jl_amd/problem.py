@@ -230,6 +230,42 @@ def stepforward(prob, nsteps=1, diags=None):
             d.increment()
 
 
+def set_schedule(prob, p_k1=0, p_k2=0, p_k3=0):
+    """FFT points per thread of the three calcN kernels (K1 inverse y-pass, K2 x-pass, K3 forward
+    y-pass and stage): 4 or 8, 0 for the built-in choice.  No counterpart in the reference."""
+    check(lib.gf_set_schedule(prob._ctx, int(p_k1), int(p_k2), int(p_k3)), prob._ctx)
+
+
+def schedule(prob):
+    """The radix schedule in use, ``(P_K1, P_K2, P_K3)``."""
+    p = [ctypes.c_int() for _ in range(3)]
+    check(lib.gf_get_schedule(prob._ctx, *[ctypes.byref(x) for x in p]), prob._ctx)
+    return tuple(x.value for x in p)
+
+
+def autotune(prob, steps=20):
+    """Time every offered radix schedule on ``prob`` (state and clock restored), apply the fastest
+    per kernel and record it for later problems of the same model, size and dtype.  Returns the
+    chosen schedule and the measured average launch times in microseconds per P."""
+    us = (ctypes.c_double * 6)()
+    check(lib.gf_autotune(prob._ctx, int(steps), us), prob._ctx)
+    return schedule(prob), {4: list(us[0:3]), 8: list(us[3:6])}
+
+
+def save_schedules(path):
+    """Write the process's tuned schedules as a JSON table."""
+    if lib.gf_schedules_save(str(path).encode()) != 0:
+        raise _lib.GFError(_lib.GF_EINVAL, lib.gf_schedules_error().decode())
+
+
+def load_schedules(path):
+    """Read a table written by ``save_schedules``; problems created afterwards use its entries."""
+    n = lib.gf_schedules_load(str(path).encode())
+    if n < 0:
+        raise _lib.GFError(n, lib.gf_schedules_error().decode())
+    return n
+
+
 def device_diagnostic(spec, unpack=None):
     """Marks a diagnostic function as computable on the device: ``spec(prob)`` returns
     ``(kind, xi, nxi, nout)`` for ``gf_diagnostic_async``; ``unpack`` turns one recorded row of

@@ -131,6 +131,35 @@ int gf_step(gf_ctx* ctx, int nsteps);
  * counterpart in the reference. */
 int gf_use_graphs(gf_ctx* ctx, int enable);
 
+/* Radix schedules (no counterpart in the reference, whose FFTs are FFTW's or cuFFT's plans).
+ * For single-layer problems the FFT of each calcN kernel runs with P points per thread:
+ *   K1 (inverse y-pass), K2 (x-pass with the products), K3 (forward y-pass and stage).
+ * P is 4 or 8 (8 only from 32 points); 0 restores the built-in choice (4 below 32 points,
+ * else 8). Results differ from another schedule's only by rounding. gf_get_schedule reports
+ * the P in use. MultiLayerQG has a fixed schedule. */
+int gf_set_schedule(gf_ctx* ctx, int p_k1, int p_k2, int p_k3);
+int gf_get_schedule(gf_ctx* ctx, int* p_k1, int* p_k2, int* p_k3);
+
+/* The autotuner. It times `steps` steps under every offered schedule on this problem, picks
+ * the fastest P per kernel, and applies it. The state, its stepper history and the clock are
+ * restored afterwards. The choice is recorded for (model, nlayers, nx, ny, dtype): contexts
+ * created later in the process start with it. us (nullable) receives 6 average launch times in
+ * microseconds, [K1, K2, K3] for P = 4 then for P = 8, NaN where a P is not offered. */
+int gf_autotune(gf_ctx* ctx, int steps, double* us);
+
+/* The process-wide table of tuned schedules as JSON ("gf-schedules-1": one entry per (model,
+ * nlayers, nx, ny, dtype) with its P triple and measured times). gf_schedules_load returns the
+ * number of entries read (>= 0), or GF_EINVAL for an unreadable or malformed file, or for a
+ * schedule this build does not offer. gf_schedules_error() then names the problem.
+ * Loaded entries apply to contexts created afterwards. Before the first context is created,
+ * the library reads the file named by the environment variable GF_SCHEDULES, or else
+ * schedules.json in the directory of libgfhip.so (the table tuned on MI355X that ships with
+ * it) if that file exists. */
+int gf_schedules_save(const char* path);
+int gf_schedules_load(const char* path);
+void gf_schedules_clear(void);
+const char* gf_schedules_error(void);
+
 /* [FF] prob.clock */
 int gf_clock(const gf_ctx* ctx, double* t, int64_t* step);
 int gf_set_clock(gf_ctx* ctx, double t, int64_t step);

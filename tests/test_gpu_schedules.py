@@ -1,0 +1,109 @@
+"""Radix schedules and the autotuner on the device.  Every offered schedule computes calcN and
+steps within the oracle tolerance (schedules differ from each other only by rounding), and the
+autotuner leaves the state, the stepper history and the clock as it found them."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import singlelayerqg as osl
+from oracle import twodnavierstokes as o2d
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+SCHEDULES = [(4, 4, 4), (8, 8, 8), (4, 8, 4), (8, 4, 8)]
+
+
+def _spec(rng, n):
+    return rng.standard_normal((n, n // 2 + 1)) + 1j * rng.standard_normal((n, n // 2 + 1))
+
+
+@pytest.mark.parametrize("n", [64, 256])
+@pytest.mark.parametrize("sched", SCHEDULES)
+def test_twodns_every_schedule_matches_oracle(gf, n, sched):
+    rng = np.random.default_rng(80 + n)
+    s = _spec(rng, n)
+    p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, ν=1e-3, nν=1, μ=0.05, dt=0.001, stepper="ETDRK4",
+                                    calcF=gf.RingForcing(6.0, 1.5, 0.1, seed=4))
+    gf.set_schedule(p, *sched)
+    assert gf.schedule(p) == sched
+    r = o2d.Problem(nx=n, nu=1e-3, nnu=1, mu=0.05, dt=0.001, stepper="ETDRK4", calcF=("ring", 6.0, 1.5, 0.1, 4))
+    N, _ = p.calcN(s)
+    Nr, sr = np.zeros_like(s), s.copy()
+    o2d.calcN(Nr, sr, 0.0, r.clock, r.vars, r.params, r.grid)
+    assert rel(N, Nr) < 1e-13
+    s0 = 0.05 * s
+    s0[0, 0] = 0
+    p.sol = s0
+    r.sol[...] = s0
+    gf.stepforward(p, 5)
+    from oracle import fourierflows as off
+    off.stepforward(r, 5)
+    assert rel(p.sol, r.sol) < 1e-12
+
+
+@pytest.mark.parametrize("sched", SCHEDULES)
+def test_slqg_schedules_topography_and_varying_U(gf, sched):
+    n = 128
+    x = -math.pi + 2 * math.pi * np.arange(n) / n
+    X, Y = np.meshgrid(x, x)
+    eta = 0.2 * np.cos(3 * X) * np.sin(2 * Y)
+    Uy = 0.1 * np.cos(Y[:, 0])
+    for kw in (dict(U=0.3, eta=eta), dict(U=Uy, eta=eta)):
+        p = gf.SingleLayerQG.Problem(gf.GPU(), nx=n, β=2.0, μ=0.01, dt=0.002, stepper="FilteredRK4", **kw)
+        gf.set_schedule(p, *sched)
+        rk = dict(kw)
+        r = osl.Problem(nx=n, beta=2.0, mu=0.01, dt=0.002, stepper="FilteredRK4", **rk)
+        s = _spec(np.random.default_rng(90), n)
+        N, _ = p.calcN(s)
+        Nr, sr = np.zeros_like(s), s.copy()
+        osl.calcN(Nr, sr, 0.0, r.clock, r.vars, r.params, r.grid)
+        assert rel(N, Nr) < 1e-13, kw.keys()
+
+
+def test_schedule_not_offered_is_rejected(gf):
+    p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=16, stepper="RK4")
+    with pytest.raises(gf._lib.GFError, match="not offered"):
+        gf.set_schedule(p, 4, 8, 4)            # P = 8 needs 32 points
+    with pytest.raises(gf._lib.GFError, match="not offered"):
+        gf.set_schedule(p, 16, 4, 4)
+    assert gf.schedule(p) == (4, 4, 4)         # the built-in choice below 32 points
+    assert gf.schedule(gf.TwoDNavierStokes.Problem(gf.GPU(), nx=32, stepper="RK4")) == (8, 8, 8)
+    m = gf.MultiLayerQG.Problem(2, gf.GPU(), nx=64, f0=1, H=[1.0, 1.0], b=[0, -1.0], dt=0.01, stepper="RK4")
+    with pytest.raises(gf._lib.GFError, match="not offered"):
+        gf.set_schedule(m, 4, 4, 4)
+
+
+def test_autotune_restores_state_and_records_the_choice(gf, tmp_path):
+    n = 256
+    T = gf.TwoDNavierStokes
+    # AB3 treats L explicitly: nu Krsq^2 dt stays below 0.01 at the largest kept wavenumber
+    kw = dict(nx=n, ν=1e-9, nν=2, μ=0.1, dt=0.002, stepper="AB3", calcF=gf.RingForcing(8.0, 1.5, 0.1, seed=6))
+    p, q = T.Problem(gf.GPU(), **kw), T.Problem(gf.GPU(), **kw)
+    z = 0.2 * np.random.default_rng(91).standard_normal((n, n))
+    T.set_ζ(p, z)
+    T.set_ζ(q, z)
+    gf.stepforward(p, 5)                       # AB3 history and prevsol in use
+    gf.stepforward(q, 5)
+    before, t0, k0 = p.sol, p.clock.t, p.clock.step
+    w0 = T.energy_work(p)
+    chosen, us = gf.autotune(p, steps=6)
+    assert np.array_equal(p.sol, before) and p.clock.t == t0 and p.clock.step == k0
+    assert T.energy_work(p) == w0
+    assert all(c in (4, 8) for c in chosen) and gf.schedule(p) == chosen
+    assert all(np.isfinite(us[4])) and all(np.isfinite(us[8])) and min(us[4] + us[8]) > 0
+    gf.set_schedule(q, *chosen)
+    gf.stepforward(p, 7)                       # the history survived: same steps as the twin
+    gf.stepforward(q, 7)
+    assert np.isfinite(p.sol).all()
+    assert np.array_equal(p.sol, q.sol)
+    # later problems of the same kind start with the tuned schedule; the table round-trips
+    r = T.Problem(gf.GPU(), **kw)
+    assert gf.schedule(r) == chosen
+    path = tmp_path / "sched.json"
+    gf.save_schedules(path)
+    gf._lib.lib.gf_schedules_clear()
+    assert gf.schedule(T.Problem(gf.GPU(), **kw)) == (8, 8, 8)
+    assert gf.load_schedules(path) >= 1
+    assert gf.schedule(T.Problem(gf.GPU(), **kw)) == chosen
+    gf._lib.lib.gf_schedules_clear()

@@ -1,0 +1,56 @@
+"""The autotuner's schedule table on the host (no device): JSON round trip, and rejection of
+malformed files and of schedules this build does not offer."""
+import json
+
+import pytest
+
+
+def _entry(**kw):
+    e = {"model": 0, "nlayers": 1, "nx": 256, "ny": 256, "dtype": 0, "p": [4, 8, 4], "us": [1.5, 2.0, 3.25]}
+    e.update(kw)
+    return e
+
+
+def test_round_trip(gf, tmp_path):
+    gf._lib.lib.gf_schedules_clear()
+    src = tmp_path / "in.json"
+    src.write_text(json.dumps({"format": "gf-schedules-1",
+                               "entries": [_entry(), _entry(nx=64, ny=32, p=[4, 8, 0], dtype=1)]}))
+    assert gf.load_schedules(src) == 2
+    out = tmp_path / "out.json"
+    gf.save_schedules(out)
+    got = json.loads(out.read_text())
+    assert got["format"] == "gf-schedules-1"
+    ents = sorted(got["entries"], key=lambda e: e["nx"])
+    assert [e["p"] for e in ents] == [[4, 8, 0], [4, 8, 4]]
+    assert ents[1]["us"] == [1.5, 2.0, 3.25] and ents[0]["ny"] == 32 and ents[0]["dtype"] == 1
+    gf._lib.lib.gf_schedules_clear()
+    gf.save_schedules(out)
+    assert json.loads(out.read_text())["entries"] == []
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("{", "expected a string"),
+    ("[1, 2", "expected .]."),
+    ('{"format": "other", "entries": []}', "not a gf-schedules-1 file"),
+    ('{"format": "gf-schedules-1", "entries": {}}', "must be an array"),
+    ('{"format": "gf-schedules-1", "entries": [{"model": 0}]}', "must be an integer"),
+    (json.dumps({"format": "gf-schedules-1", "entries": [_entry(p=[16, 8, 8])]}), "does not offer"),
+    (json.dumps({"format": "gf-schedules-1", "entries": [_entry(nx=16, p=[4, 8, 4])]}), "does not offer"),
+    (json.dumps({"format": "gf-schedules-1", "entries": [_entry(p=[4, 8])]}), "3 integers"),
+    ('{"format": "gf-schedules-1", "entries": []} x', "trailing"),
+])
+def test_malformed_tables_are_rejected(gf, tmp_path, text, msg):
+    gf._lib.lib.gf_schedules_clear()
+    f = tmp_path / "bad.json"
+    f.write_text(text)
+    with pytest.raises(gf._lib.GFError, match=msg):
+        gf.load_schedules(f)
+    out = tmp_path / "after.json"
+    gf.save_schedules(out)
+    assert json.loads(out.read_text())["entries"] == []     # nothing of a bad file is kept
+
+
+def test_missing_file(gf, tmp_path):
+    with pytest.raises(gf._lib.GFError, match="cannot open"):
+        gf.load_schedules(tmp_path / "none.json")
