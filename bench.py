@@ -327,6 +327,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-adjoint-batch", action="store_true",
+                    help="with --only-adjoint: the single-problem gradient only (no 16-thread leg)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the Float32 variant")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=100)
@@ -341,7 +343,10 @@ def main():
     if args.only_adjoint:
         import __graft_entry__ as ge
         gf = ge.load_package()
-        print(json.dumps({"adjoint_512": adjoint_run(gf), "adjoint_512_batch16": adjoint_batch_run(gf)}), flush=True)
+        out = {"adjoint_512": adjoint_run(gf)}
+        if not args.no_adjoint_batch:
+            out["adjoint_512_batch16"] = adjoint_batch_run(gf)
+        print(json.dumps(out), flush=True)
         return
     if args.only_ensemble:
         import __graft_entry__ as ge

@@ -708,14 +708,15 @@ struct Ctx final : CtxBase {
   void get_schedule(int* p) const override {
     for (int k = 0; k < 3; ++k) p[k] = sched[k] ? sched[k] : (nl == 1 ? sched_default(sched_len(k)) : (d.ny >= 32 && k != 1 ? 4 : sched_default(sched_len(k))));
   }
-  // Times every offered schedule on this problem (the state, its history and the clock are
-  // restored afterwards), keeps the fastest per kernel, and records it for this (model, size,
-  // dtype) so later contexts start with it.  us (nullable) receives 2 x 3 average launch times
-  // in microseconds, P = 4 then P = 8, NaN where a P is not offered.
+  // Times whole steps under every offered combination of schedules, P in {4, 8} for each of
+  // K1, K2, K3 (two passes, the second in reverse order, best of the two), keeps the fastest
+  // and records it for this (model, size, dtype) so that later contexts start with it. The
+  // state, its stepper history and the clock are restored afterwards. us (nullable) receives
+  // the 8 step times in microseconds, combination c = 4 (P_K1 == 8) + 2 (P_K2 == 8) + (P_K3 == 8),
+  // NaN where a combination is not offered.
   void autotune(int steps, double* us) override {
     if (steps < 1) throw GfError(GF_EINVAL, "gf_autotune: steps must be >= 1");
-    const int cand[2] = {4, 8};
-    if (us) for (int j = 0; j < 6; ++j) us[j] = std::nan("");
+    if (us) for (int j = 0; j < 8; ++j) us[j] = std::nan("");
     if (nl != 1) return;   // MultiLayerQG keeps its fixed schedule
     HIPCHECK(hipStreamSynchronize(stream));
     const size_t cb = sizeof(C) * fsz * nl;
@@ -727,43 +728,56 @@ struct Ctx final : CtxBase {
     const int64_t step0 = stepno;
     const bool pv0 = prevsol_valid;
     const int old[3] = {sched[0], sched[1], sched[2]};
-    double best[3] = {INFINITY, INFINITY, INFINITY};
-    int pick[3] = {old[0], old[1], old[2]};
+    std::vector<int> combos;
+    for (int c = 0; c < 8; ++c) {
+      bool ok = true;
+      for (int k = 0; k < 3; ++k) ok &= sched_candidate(sched_len(k), (c >> (2 - k)) & 1 ? 8 : 4);
+      if (ok) combos.push_back(c);
+    }
+    double best_t[8];
+    for (double& b : best_t) b = INFINITY;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    auto cleanup = [&] {
+      for (auto& e : ev) if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    };
     try {
-      for (int c = 0; c < 2; ++c) {
-        int q[3];
-        bool any = false;
-        for (int k = 0; k < 3; ++k) {
-          q[k] = sched_candidate(sched_len(k), cand[c]) ? cand[c] : 0;
-          any |= q[k] != 0;
+      HIPCHECK(hipEventCreate(&ev[0]));
+      HIPCHECK(hipEventCreate(&ev[1]));
+      for (int pass = 0; pass < 2; ++pass) {
+        for (size_t i = 0; i < combos.size(); ++i) {
+          const int c = combos[pass ? combos.size() - 1 - i : i];
+          const int q[3] = {(c & 4) ? 8 : 4, (c & 2) ? 8 : 4, (c & 1) ? 8 : 4};
+          set_schedule(q);
+          step(2);
+          HIPCHECK(hipEventRecord(ev[0], stream));
+          step(steps);
+          HIPCHECK(hipEventRecord(ev[1], stream));
+          HIPCHECK(hipEventSynchronize(ev[1]));
+          float ms = 0;
+          HIPCHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+          best_t[c] = std::min(best_t[c], 1e3 * (double)ms / steps);
         }
-        if (!any) continue;
-        set_schedule(q);
-        step(2);
-        tm.reset(1, stream);
-        step(steps);
-        for (int k = 0; k < 3; ++k) {
-          if (!q[k]) continue;
-          int64_t n = 0;
-          double ms = 0;
-          tm.read(k, &n, &ms, stream);
-          const double a = n ? ms * 1e3 / n : INFINITY;
-          if (us) us[c * 3 + k] = a;
-          if (a < best[k]) { best[k] = a; pick[k] = q[k]; }
-        }
-        tm.reset(0, stream);
       }
     } catch (...) {
-      tm.reset(0, stream);
+      cleanup();
       set_schedule(old);
       throw;
     }
+    cleanup();
     for (int j = 0; j < 4; ++j)
       if (save[j].p) HIPCHECK(hipMemcpy(keep[j]->p, save[j].p, cb, hipMemcpyDeviceToDevice));
     t = t0; stepno = step0; prevsol_valid = pv0;
+    int bc = -1;
+    for (int c : combos) {
+      if (us) us[c] = best_t[c];
+      if (bc < 0 || best_t[c] < best_t[bc]) bc = c;
+    }
+    if (bc < 0) { set_schedule(old); return; }
+    const int pick[3] = {(bc & 4) ? 8 : 4, (bc & 2) ? 8 : 4, (bc & 1) ? 8 : 4};
     set_schedule(pick);
     Sched e;
-    for (int k = 0; k < 3; ++k) { e.p[k] = pick[k]; e.us[k] = std::isfinite(best[k]) ? best[k] : 0; }
+    for (int k = 0; k < 3; ++k) e.p[k] = pick[k];
+    e.step_us = best_t[bc];
     sched_store(sched_key(), e);
   }
 
@@ -773,7 +787,9 @@ struct Ctx final : CtxBase {
   // one graph launch per step instead of 3 per stage. The ring forcing's step number, the
   // one argument that changes, is read from step_dev, which the graph advances itself.
   static constexpr int GRAPH_BLOCK = 8;   // interior steps in the larger graph
-  bool graphs = true;
+  // Off unless gf_use_graphs(ctx, 1): for one problem the replay gains at most a few percent
+  // (the steps are bound by kernel time, not by launches).
+  bool graphs = false;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};   // one interior step, GRAPH_BLOCK interior steps
   DevBuf step_dev;
 

@@ -1,6 +1,6 @@
 // The schedule table and its JSON form:
 //   {"format": "gf-schedules-1", "entries": [
-//     {"model": 0, "nlayers": 1, "nx": 2048, "ny": 2048, "dtype": 0, "p": [4, 8, 4], "us": [20.1, 46.3, 30.2]}, ...]}
+//     {"model": 0, "nlayers": 1, "nx": 2048, "ny": 2048, "dtype": 0, "p": [8, 8, 4], "step_us": 441.2}, ...]}
 #include "gf_schedule.hpp"
 
 #include <cctype>
@@ -129,12 +129,9 @@ void sched_save(const std::string& path) {
       const Sched& e = kv.second;
       o << (first ? "\n" : ",\n") << "  {\"model\": " << k[0] << ", \"nlayers\": " << k[1] << ", \"nx\": " << k[2]
         << ", \"ny\": " << k[3] << ", \"dtype\": " << k[4] << ", \"p\": [" << e.p[0] << ", " << e.p[1] << ", "
-        << e.p[2] << "], \"us\": [";
-      for (int j = 0; j < 3; ++j) {
-        std::snprintf(buf, sizeof buf, "%.3f", e.us[j]);
-        o << (j ? ", " : "") << buf;
-      }
-      o << "]}";
+        << e.p[2] << "], \"step_us\": ";
+      std::snprintf(buf, sizeof buf, "%.3f", e.step_us);
+      o << buf << "}";
       first = false;
     }
     o << "\n]}\n";
@@ -169,9 +166,8 @@ int sched_load(const std::string& path) {
     if (!p || p->kind != Value::ARR || p->arr.size() != 3) throw std::runtime_error("schedule table: 'p' must hold 3 integers");
     Sched s;
     for (int j = 0; j < 3; ++j) s.p[j] = as_int(&p->arr[j], "p");
-    const Value* us = e.get("us");
-    if (us && us->kind == Value::ARR && us->arr.size() == 3)
-      for (int j = 0; j < 3; ++j) s.us[j] = us->arr[j].kind == Value::NUM ? us->arr[j].num : 0;
+    const Value* us = e.get("step_us");
+    if (us && us->kind == Value::NUM) s.step_us = us->num;
     // K1 and K3 transform columns (length ny), K2 rows (length nx)
     if (!offered(k[3], s.p[0]) || !offered(k[2], s.p[1]) || !offered(k[3], s.p[2]))
       throw std::runtime_error("schedule table: entry for " + std::to_string(k[2]) + "x" + std::to_string(k[3]) +

@@ -10,7 +10,7 @@ using SchedKey = std::array<int, 5>;   // model, nlayers, nx, ny, dtype
 
 struct Sched {
   int p[3] = {0, 0, 0};         // FFT points per thread of K1, K2, K3 (0: built-in choice)
-  double us[3] = {0, 0, 0};     // measured average launch time of each, microseconds
+  double step_us = 0;           // measured time of one step under this schedule, microseconds
 };
 
 bool sched_lookup(const SchedKey& key, Sched& out);

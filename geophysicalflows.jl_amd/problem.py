@@ -196,7 +196,8 @@ class Problem:
         check(lib.gf_synchronize(self._ctx), self._ctx)
 
     def use_graphs(self, enable=True):
-        """Replay the captured step graph in ``stepforward`` (default) or launch every kernel."""
+        """Replay a captured graph of the step in ``stepforward`` (the default for ensembles) or
+        launch every kernel (the default for single problems)."""
         check(lib.gf_use_graphs(self._ctx, int(bool(enable))), self._ctx)
 
     def _diagnostic(self, kind, xi=0.0, nxi=0, nout=1):
@@ -244,12 +245,16 @@ def schedule(prob):
 
 
 def autotune(prob, steps=20):
-    """Time every offered radix schedule on ``prob`` (state and clock restored), apply the fastest
-    per kernel and record it for later problems of the same model, size and dtype.  Returns the
-    chosen schedule and the measured average launch times in microseconds per P."""
-    us = (ctypes.c_double * 6)()
+    """Time whole steps of ``prob`` under every offered radix schedule (state and clock restored),
+    apply the fastest and record it for later problems of the same model, size and dtype.
+    Returns the chosen schedule and the step time in microseconds of each schedule tried."""
+    us = (ctypes.c_double * 8)()
     check(lib.gf_autotune(prob._ctx, int(steps), us), prob._ctx)
-    return schedule(prob), {4: list(us[0:3]), 8: list(us[3:6])}
+    times = {}
+    for c in range(8):
+        if math.isfinite(us[c]):
+            times[(8 if c & 4 else 4, 8 if c & 2 else 4, 8 if c & 1 else 4)] = us[c]
+    return schedule(prob), times
 
 
 def save_schedules(path):
@@ -428,7 +433,8 @@ class Ensemble:
         check(lib.gf_synchronize(self._ctx), self._ctx)
 
     def use_graphs(self, enable=True):
-        """Replay the captured step graph in ``stepforward`` (default) or launch every kernel."""
+        """Replay a captured graph of the step in ``stepforward`` (the default for ensembles) or
+        launch every kernel (the default for single problems)."""
         check(lib.gf_use_graphs(self._ctx, int(bool(enable))), self._ctx)
 
     def _diagnostic(self, kind, xi=0.0, nxi=0, nout=1):

@@ -9,7 +9,6 @@ libgfhip.so unless ``--out`` says otherwise.  Problems of the same model, size a
 an entry; topography, forcing and the stepper do not change which kernels run.
 """
 import argparse
-import math
 import os
 import sys
 
@@ -44,9 +43,10 @@ def main(argv=None):
         (m.set_ζ if mod == "TwoDNavierStokes" else m.set_q)(p, f)
         gf.stepforward(p, 3)
         chosen, us = gf.autotune(p, steps=args.steps)
-        fmt = lambda v: "[" + ", ".join("-" if not math.isfinite(x) else f"{x:.2f}" for x in v) + "]"
-        print(f"{mod:17s} {n:5d}^2 {np.dtype(T).name:8s} schedule {chosen}  us P=4 {fmt(us[4])}  P=8 {fmt(us[8])}",
-              flush=True)
+        base = us.get((8, 8, 8)) or us.get((4, 4, 4))
+        print(f"{mod:17s} {n:5d}^2 {np.dtype(T).name:8s} schedule {chosen}  step {us[chosen]:.2f} us"
+              f"  (built-in {base:.2f} us; all: " + ", ".join(f"{''.join(map(str, k))} {v:.1f}" for k, v in
+                                                          sorted(us.items())) + ")", flush=True)
     gf.save_schedules(args.out)
     print("wrote", args.out)
 

@@ -123,12 +123,12 @@ int gf_set_forcing(gf_ctx* ctx, const void* Fh);
 
 /* [FF] stepforward!(prob, nsteps): nsteps of the configured stepper, queued on the context's
  * stream (the call returns before they finish; every read of the state waits for them).
- * Every step but the call's last replays one HIP graph of the whole step (all stages, or all
- * stages of every ensemble member), captured at the first such step. */
+ * With graph replay on, every step but the call's last replays a HIP graph of the whole step
+ * (all stages, or all stages of every ensemble member), captured at the first such step. */
 int gf_step(gf_ctx* ctx, int nsteps);
 
-/* Turn the graph replay of gf_step off (0) or on (1, the default); results are identical. No
- * counterpart in the reference. */
+/* Turn the graph replay of gf_step on (1) or off (0); results are identical. On by default
+ * for ensembles, off by default for single problems. No counterpart in the reference. */
 int gf_use_graphs(gf_ctx* ctx, int enable);
 
 /* Radix schedules (no counterpart in the reference, whose FFTs are FFTW's or cuFFT's plans).
@@ -140,15 +140,17 @@ int gf_use_graphs(gf_ctx* ctx, int enable);
 int gf_set_schedule(gf_ctx* ctx, int p_k1, int p_k2, int p_k3);
 int gf_get_schedule(gf_ctx* ctx, int* p_k1, int* p_k2, int* p_k3);
 
-/* The autotuner. It times `steps` steps under every offered schedule on this problem, picks
- * the fastest P per kernel, and applies it. The state, its stepper history and the clock are
+/* The autotuner. It times `steps` whole steps under every offered combination of P for
+ * (K1, K2, K3), in two passes with the second in reverse order, keeping each combination's
+ * best. The fastest combination is applied. The state, its stepper history and the clock are
  * restored afterwards. The choice is recorded for (model, nlayers, nx, ny, dtype): contexts
- * created later in the process start with it. us (nullable) receives 6 average launch times in
- * microseconds, [K1, K2, K3] for P = 4 then for P = 8, NaN where a P is not offered. */
+ * created later in the process start with it. us (nullable) receives 8 step times in
+ * microseconds, indexed 4 (P_K1 == 8) + 2 (P_K2 == 8) + (P_K3 == 8); NaN where a combination
+ * is not offered. */
 int gf_autotune(gf_ctx* ctx, int steps, double* us);
 
 /* The process-wide table of tuned schedules as JSON ("gf-schedules-1": one entry per (model,
- * nlayers, nx, ny, dtype) with its P triple and measured times). gf_schedules_load returns the
+ * nlayers, nx, ny, dtype) with its P triple and measured step time). gf_schedules_load returns the
  * number of entries read (>= 0), or GF_EINVAL for an unreadable or malformed file, or for a
  * schedule this build does not offer. gf_schedules_error() then names the problem.
  * Loaded entries apply to contexts created afterwards. Before the first context is created,

@@ -12,6 +12,7 @@ pytestmark = pytest.mark.gpu
 
 def _pair(ctor, **kw):
     a, b = ctor(**kw), ctor(**kw)
+    a.use_graphs(True)
     b.use_graphs(False)
     return a, b
 
@@ -80,7 +81,7 @@ def test_slqg_topography_and_mlqg_graphs(gf):
 def test_ensemble_graph_equals_launches(gf):
     S = gf.SingleLayerQG
     members = [(S.Problem, dict(nx=n, β=10.0, μ=0.01, dt=0.01, stepper="FilteredRK4")) for n in (64, 128, 64, 256)]
-    ea, eb = gf.Ensemble(members), gf.Ensemble(members)
+    ea, eb = gf.Ensemble(members), gf.Ensemble(members)     # graphs are on by default for ensembles
     eb.use_graphs(False)
     rng = np.random.default_rng(74)
     for pa, pb in zip(ea.problems, eb.problems):
@@ -93,6 +94,7 @@ def test_ensemble_graph_equals_launches(gf):
         for pa, pb in zip(ea.problems, eb.problems):
             assert np.array_equal(pa.sol, pb.sol)
     # a member stepped alone replays its own graph
+    ea.problems[1].use_graphs(True)
     gf.stepforward(ea.problems[1], 5)
     gf.stepforward(eb.problems[1], 5)
     assert np.array_equal(ea.problems[1].sol, eb.problems[1].sol)

@@ -6,7 +6,7 @@ import pytest
 
 
 def _entry(**kw):
-    e = {"model": 0, "nlayers": 1, "nx": 256, "ny": 256, "dtype": 0, "p": [4, 8, 4], "us": [1.5, 2.0, 3.25]}
+    e = {"model": 0, "nlayers": 1, "nx": 256, "ny": 256, "dtype": 0, "p": [4, 8, 4], "step_us": 81.25}
     e.update(kw)
     return e
 
@@ -23,7 +23,7 @@ def test_round_trip(gf, tmp_path):
     assert got["format"] == "gf-schedules-1"
     ents = sorted(got["entries"], key=lambda e: e["nx"])
     assert [e["p"] for e in ents] == [[4, 8, 0], [4, 8, 4]]
-    assert ents[1]["us"] == [1.5, 2.0, 3.25] and ents[0]["ny"] == 32 and ents[0]["dtype"] == 1
+    assert ents[1]["step_us"] == 81.25 and ents[0]["ny"] == 32 and ents[0]["dtype"] == 1
     gf._lib.lib.gf_schedules_clear()
     gf.save_schedules(out)
     assert json.loads(out.read_text())["entries"] == []
