@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -202,6 +203,12 @@ struct CtxBase {
   int64_t stepno = 0;
 };
 
+// Gradients in flight in this process. The two-stream sweep of gf_adjoint_grad pays when one
+// gradient alone leaves the GPU partly idle. With several at once the GPU is already full,
+// and the extra streams (beyond the 4 hardware queues) wait on each other, so a gradient that
+// starts while another is running keeps its sweep on one stream.
+static std::atomic<int> g_adjoint_active{0};
+
 template <typename T>
 struct Ctx final : CtxBase {
   using C = cplx<T>;
@@ -263,6 +270,12 @@ struct Ctx final : CtxBase {
   ~Ctx() override {
     if (stream) { (void)hipStreamSynchronize(stream); if (own_stream) (void)hipStreamDestroy(stream); }
     for (auto& ge : gexec) if (ge) (void)hipGraphExecDestroy(ge);
+    if (rstream) {
+      (void)hipStreamSynchronize(rstream);
+      (void)hipStreamDestroy(rstream);
+      (void)hipEventDestroy(fwd_done);
+      for (int k = 0; k < 2; ++k) { (void)hipEventDestroy(adj_done[k]); (void)hipEventDestroy(rec_done[k]); }
+    }
     if (copy_stream) {
       (void)hipStreamSynchronize(copy_stream);
       for (int k = 0; k < 2; ++k) { (void)hipEventDestroy(snap_ready[k]); (void)hipEventDestroy(snap_done[k]); }
@@ -1046,7 +1059,12 @@ struct Ctx final : CtxBase {
   }
 
   // ---------------- adjoint (TwoDNavierStokes, ETDRK4, FP64) ----------------
-  DevBuf adj_mixed, adj_flux, lamb, l0b, l1b, lN1b, lN2b, lN3b, nu_part, dtab[6], tape, ckpts;
+  DevBuf adj_mixed, adj_flux, lamb, l0b, l1b, lN1b, lN2b, lN3b, nu_part, dtab[6], tape, tape2, ckpts;
+  // With checkpoints the sweep is pipelined over two streams: segment s - 1 is recomputed into
+  // one tape on rstream while the adjoint of segment s reads the other tape on the main stream.
+  hipStream_t rstream = nullptr;
+  hipEvent_t adj_done[2] = {}, rec_done[2] = {}, fwd_done = nullptr;
+  long tape2_steps = 0;
   long tape_steps = 0;       // steps the tape holds (grown on demand, never shrunk:
   long ckpt_slots = 0;       // freeing device memory synchronises the whole device)
   int ka3_blocks = 0;
@@ -1078,36 +1096,38 @@ struct Ctx final : CtxBase {
     for (int k = 0; k < 6; ++k) upload_blocked(dtab[k], t6[k]);
   }
 
-  C* slot(long i, int f) const { return tape.as<C>() + ((size_t)i * 8 + f) * fsz; }
+  // slot i, field f of tape tp (0: tape, 1: tape2)
+  C* slot(long i, int f, int tp = 0) const { return (tp ? tape2 : tape).as<C>() + ((size_t)i * 8 + f) * fsz; }
 
   // One ETDRK4 step from tape slot i (s0) writing s1, s2, s3 and N1..N4 of that slot and the
-  // new state to next (kept modes only: every later read masks)
-  void tape_step(long i, C* next) {
-    C* s0 = slot(i, 0);
-    C* s1 = slot(i, 1);
-    C* s2 = slot(i, 2);
-    C* s3 = slot(i, 3);
+  // new state to next (kept modes only: every later read masks), on stream st. Its scratch
+  // (mixed, flux, n1, n23) is the forward path's; the adjoint kernels use none of it.
+  void tape_step(long i, C* next, int tp, hipStream_t st) {
+    C* s0 = slot(i, 0, tp);
+    C* s1 = slot(i, 1, tp);
+    C* s2 = slot(i, 2, tp);
+    C* s3 = slot(i, 3, tp);
     K1Args<T> a; K2Args<T> b; K3Args<T> k;
     auto run = [&]() {
-      launch_k1<T>(d.ny, a, stream); launch_k2<T>(d.nx, b, stream); launch_k3<T>(d.ny, k, stream);
+      launch_k1<T>(d.ny, a, st); launch_k2<T>(d.nx, b, st); launch_k3<T>(d.ny, k, st);
     };
     make_args(s0, E_ETD1, false, a, b, k);
-    k.sol = s0; k.s1 = s1; k.nsave = slot(i, 4); run();
+    k.sol = s0; k.s1 = s1; k.nsave = slot(i, 4, tp); run();
     make_args(s1, E_ETD2, false, a, b, k);
-    k.sol = s0; k.s2 = s2; k.nsave = slot(i, 5); run();
+    k.sol = s0; k.s2 = s2; k.nsave = slot(i, 5, tp); run();
     make_args(s2, E_ETD3, false, a, b, k);
-    k.s1 = s1; k.s2 = s3; k.nsave = slot(i, 6); run();
+    k.s1 = s1; k.s2 = s3; k.nsave = slot(i, 6, tp); run();
     make_args(s3, E_ETD4, false, a, b, k);
-    k.sol = s0; k.sol_out = next; k.nsave = slot(i, 7); run();
+    k.sol = s0; k.sol_out = next; k.nsave = slot(i, 7, tp); run();
     HIPCHECK(hipGetLastError());
   }
 
-  // adjoint of the step taped in slot i: lamb (lambda of the new state) -> lambda of s0
-  void adjoint_step(long i, bool want_nu) {
+  // adjoint of the step taped in slot i of tape tp: lamb (lambda of the new state) -> lambda of s0
+  void adjoint_step(long i, bool want_nu, int tp = 0) {
     if constexpr (std::is_same<T, double>::value) {
       for (int stage = 4; stage >= 1; --stage) {
         AdjK1Args<T> a{};
-        a.g = g; a.s = slot(i, stage - 1);
+        a.g = g; a.s = slot(i, stage - 1, tp);
         a.x = stage == 4 ? lamb.as<C>() : stage == 3 ? lN3b.as<C>() : stage == 2 ? lN2b.as<C>() : lN1b.as<C>();
         a.lamc = stage == 4 ? (const T*)coef[5].p : nullptr;
         a.out = adj_mixed.as<C>(); a.fstride = fsz; a.ncols = g.kmax; a.tw = twy.as<C>();
@@ -1122,8 +1142,8 @@ struct Ctx final : CtxBase {
         k.Al = (const T*)coef[3].p; k.Be = (const T*)coef[4].p;
         k.dE = dtab[0].as<T>(); k.dE2 = dtab[1].as<T>(); k.dZ = dtab[2].as<T>();
         k.dA = dtab[3].as<T>(); k.dB = dtab[4].as<T>(); k.dG = dtab[5].as<T>();
-        k.s0 = slot(i, 0); k.s1 = slot(i, 1);
-        k.N1 = slot(i, 4); k.N2 = slot(i, 5); k.N3 = slot(i, 6); k.N4 = slot(i, 7);
+        k.s0 = slot(i, 0, tp); k.s1 = slot(i, 1, tp);
+        k.N1 = slot(i, 4, tp); k.N2 = slot(i, 5, tp); k.N3 = slot(i, 6, tp); k.N4 = slot(i, 7, tp);
         k.nu_part = want_nu ? nu_part.as<T>() : nullptr;
         k.tw = twy.as<C>();
         rec(0, true); launch_adj<T>(1, d.ny, &a, stream); rec(0, false);
@@ -1156,11 +1176,29 @@ struct Ctx final : CtxBase {
       if (d.nx != d.ny) throw GfError(GF_EUNSUPPORTED, "the adjoint needs a square grid");
       if (nsteps < 1 || every < 1) throw GfError(GF_EINVAL, "nsteps and checkpoint_every must be >= 1");
       adjoint_setup();
+      struct Active {
+        int others;
+        Active() : others(g_adjoint_active.fetch_add(1)) {}
+        ~Active() { g_adjoint_active.fetch_sub(1); }
+      } active;
       const long seg = std::min(every, nsteps);
       const long nseg = (nsteps + seg - 1) / seg;
+      const bool pipe = nseg > 1 && active.others == 0;   // recompute on rstream beside the sweep
       if (tape_steps < seg) {
         tape.alloc(sizeof(C) * fsz * (8 * seg + 1));
         tape_steps = seg;
+      }
+      if (pipe && tape2_steps < seg) {
+        tape2.alloc(sizeof(C) * fsz * (8 * seg + 1));
+        tape2_steps = seg;
+      }
+      if (pipe && !rstream) {
+        HIPCHECK(hipStreamCreateWithFlags(&rstream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&fwd_done, hipEventDisableTiming));
+        for (int k = 0; k < 2; ++k) {
+          HIPCHECK(hipEventCreateWithFlags(&adj_done[k], hipEventDisableTiming));
+          HIPCHECK(hipEventCreateWithFlags(&rec_done[k], hipEventDisableTiming));
+        }
       }
       if (ckpt_slots < nseg) {
         ckpts.alloc(sizeof(C) * fsz * nseg);
@@ -1168,18 +1206,22 @@ struct Ctx final : CtxBase {
       }
       const size_t fb = sizeof(C) * fsz;
       auto len = [&](long s) { return std::min(seg, (long)nsteps - s * seg); };
-      // forward: checkpoints at segment starts; the last segment's tape stays for the sweep
-      HIPCHECK(hipMemcpyAsync(slot(0, 0), sol.p, fb, hipMemcpyDeviceToDevice, stream));
+      // forward: checkpoints at segment starts; segment s is taped into tape s mod 2 (into the
+      // one tape without the pipeline), and the last segment's tape stays for the sweep
+      auto tape_of = [&](long s) { return pipe ? (int)(s & 1) : 0; };
       for (long s = 0; s < nseg; ++s) {
-        HIPCHECK(hipMemcpyAsync(ckpts.as<C>() + s * fsz, slot(0, 0), fb, hipMemcpyDeviceToDevice, stream));
+        const int tp = tape_of(s);
+        HIPCHECK(hipMemcpyAsync(slot(0, 0, tp), s == 0 ? (const C*)sol.p : slot(len(s - 1), 0, tape_of(s - 1)), fb,
+                                hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(ckpts.as<C>() + s * fsz, slot(0, 0, tp), fb, hipMemcpyDeviceToDevice, stream));
         const long n = len(s);
-        for (long i = 0; i < n; ++i) tape_step(i, slot(i + 1, 0));
-        if (s + 1 < nseg) HIPCHECK(hipMemcpyAsync(slot(0, 0), slot(n, 0), fb, hipMemcpyDeviceToDevice, stream));
+        for (long i = 0; i < n; ++i) tape_step(i, slot(i + 1, 0, tp), tp, stream);
       }
       // loss and lambda_T = M (w/N) rfft2(zeta_T - target), zeta_T = irfft2(M sol_T)
       const long nlast = len(nseg - 1);
+      const int tlast = tape_of(nseg - 1);
       const size_t np = (size_t)d.nx * d.ny;
-      HIPCHECK(hipMemcpyAsync(cinp.p, slot(nlast, 0), fb, hipMemcpyDeviceToDevice, stream));
+      HIPCHECK(hipMemcpyAsync(cinp.p, slot(nlast, 0, tlast), fb, hipMemcpyDeviceToDevice, stream));
       launch_mask<T>(g, cinp.as<C>(), 1, stream);
       inv_device(cinp.as<C>(), phys.as<T>(), 1);
       std::vector<T> r(np);
@@ -1195,14 +1237,37 @@ struct Ctx final : CtxBase {
       host_spectral(h, true, 1.0 / (double)np);
       HIPCHECK(hipMemcpyAsync(lamb.p, h.data(), fb, hipMemcpyHostToDevice, stream));
       HIPCHECK(hipMemsetAsync(nu_part.p, 0, sizeof(double) * ka3_blocks, stream));
-      // reverse sweep, recomputing every segment but the last
-      for (long s = nseg - 1; s >= 0; --s) {
+      // reverse sweep: while the main stream runs the adjoint of segment s from tape s mod 2,
+      // rstream recomputes segment s - 1 into the other tape, once the adjoint of segment s + 1
+      // (the last reader of that tape) is done
+      if (pipe) {
+        HIPCHECK(hipEventRecord(fwd_done, stream));
+        HIPCHECK(hipStreamWaitEvent(rstream, fwd_done, 0));
+      }
+      for (long s = nseg - 1; s >= 0 && !pipe; --s) {
+        // one stream: recompute segment s into the tape, then its adjoint
         const long n = len(s);
         if (s != nseg - 1) {
           HIPCHECK(hipMemcpyAsync(slot(0, 0), ckpts.as<C>() + s * fsz, fb, hipMemcpyDeviceToDevice, stream));
-          for (long i = 0; i < n; ++i) tape_step(i, slot(i + 1, 0));
+          for (long i = 0; i < n; ++i) tape_step(i, slot(i + 1, 0), 0, stream);
         }
-        for (long i = n - 1; i >= 0; --i) adjoint_step(i, dnu != nullptr);
+        for (long i = n - 1; i >= 0; --i) adjoint_step(i, dnu != nullptr, 0);
+      }
+      // (segment nseg - 2 is still in its tape from the forward pass and is not recomputed)
+      for (long s = nseg - 1; s >= 0 && pipe; --s) {
+        const int tp = (int)(s & 1);
+        if (s >= 1 && s - 1 != nseg - 2) {
+          const int tq = tp ^ 1;
+          if (s + 1 <= nseg - 1) HIPCHECK(hipStreamWaitEvent(rstream, adj_done[tq], 0));
+          HIPCHECK(hipMemcpyAsync(slot(0, 0, tq), ckpts.as<C>() + (s - 1) * fsz, fb, hipMemcpyDeviceToDevice, rstream));
+          const long m = len(s - 1);
+          for (long i = 0; i < m; ++i) tape_step(i, slot(i + 1, 0, tq), tq, rstream);
+          HIPCHECK(hipEventRecord(rec_done[tq], rstream));
+        }
+        if (s < nseg - 2) HIPCHECK(hipStreamWaitEvent(stream, rec_done[tp], 0));
+        const long n = len(s);
+        for (long i = n - 1; i >= 0; --i) adjoint_step(i, dnu != nullptr, tp);
+        HIPCHECK(hipEventRecord(adj_done[tp], stream));
       }
       // dJ/dzeta0 = N irfft2(Z M lambda_0 / w)
       HIPCHECK(hipMemcpyAsync(h.data(), lamb.p, fb, hipMemcpyDeviceToHost, stream));

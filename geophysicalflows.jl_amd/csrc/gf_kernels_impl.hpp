@@ -641,7 +641,11 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
           }
         }
       }
-      const uint32_t ci = a.cl_layered ? ix : idx;
+      // The stepper's coefficient tables and the filter depend on kr and Krsq only, so rows l and
+      // NY - l hold equal values: reading the lower row lets the mirrored point's read hit in
+      // cache and halves the tables' HBM traffic.
+      const uint32_t cf = boff32(kr, l <= NY / 2 ? l : NY - l, NY);
+      const uint32_t ci = a.cl_layered ? ix : cf;
       const cplx<T> sv = kept ? at(a.sol, ix) : czero<T>();
       if (a.nsave != nullptr) at(a.nsave, ix) = N;
       switch (a.epi) {
@@ -677,7 +681,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
           cplx<T> r = E.mul(ci, sv) + Al.mul(ci, at(a.n1, ix));
           r = r + Be.mul(ci, mk((T)2 * n23v.re, (T)2 * n23v.im));
           r = r + Ga.mul(ci, N);
-          if (a.filter) r = rmul(at(a.filter, idx), r);
+          if (a.filter) r = rmul(at(a.filter, cf), r);
           at(a.sol_out != nullptr ? a.sol_out : a.sol, ix) = r;
         } break;
         case E_RK1:
@@ -702,14 +706,14 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
             const cplx<T> ra = at(a.racc, ix);
             const cplx<T> acc = mk(ra.re + R.re / (T)6, ra.im + R.im / (T)6);
             cplx<T> r = sv + rmul(a.dt, acc);
-            if (a.filter) r = rmul(at(a.filter, idx), r);
+            if (a.filter) r = rmul(at(a.filter, cf), r);
             at(a.sol, ix) = r;
           }
         } break;
         case E_FE: {
           const Coef<T> Lc{a.c0, a.ccplx};
           cplx<T> r = sv + rmul(a.dt, N + Lc.mul(ci, sv));
-          if (a.filter) r = rmul(at(a.filter, idx), r);
+          if (a.filter) r = rmul(at(a.filter, cf), r);
           at(a.sol, ix) = r;
         } break;
         case E_AB3: {
@@ -726,16 +730,17 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
             const cplx<T> acc = (rmul((T)(23.0 / 12.0), R) - rmul((T)(16.0 / 12.0), R1)) + rmul((T)(5.0 / 12.0), R2);
             r = sv + rmul(a.dt, acc);
           }
-          if (a.filter) r = rmul(at(a.filter, idx), r);
+          if (a.filter) r = rmul(at(a.filter, cf), r);
           at(a.sol, ix) = r;
           at(a.n1, ix) = R1;
           at(a.racc, ix) = R;
         } break;
         default: break;
       }
-      // one point's loads and stores at a time: keeps the epilogue within 128 VGPRs (two
-      // 512-thread workgroups per CU) with less spilling than eight points in flight
-      __builtin_amdgcn_sched_barrier(0);
+      // With 8 points per thread, one point's loads and stores at a time keeps the epilogue
+      // within 128 VGPRs (two 512-thread workgroups per CU) with less spilling than eight
+      // points in flight; with 4 the registers allow all points' loads to be in flight at once.
+      if constexpr (P >= 8) __builtin_amdgcn_sched_barrier(0);
     }
   }
 }

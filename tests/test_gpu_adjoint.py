@@ -49,7 +49,7 @@ def test_gradient_matches_oracle(gf, n, nsteps, ckpt):
 def test_checkpoint_interval_does_not_change_the_gradient(gf):
     prob, z0, target = _setup(gf, 64, 1e-3, 5)
     base = gf.TwoDNavierStokes.adjoint_gradient(prob, target, 13, 13)
-    for ckpt in (1, 4, 6):
+    for ckpt in (1, 4, 6, 3, 5, 12):          # 13, 4, 3, 5, 3 and 2 segments, uneven last ones
         other = gf.TwoDNavierStokes.adjoint_gradient(prob, target, 13, ckpt)
         assert other[0] == base[0]
         assert np.array_equal(other[1], base[1])
@@ -88,3 +88,16 @@ def test_state_unchanged_and_unsupported_problems(gf):
     with pytest.raises(gf._lib.GFError) as e:
         gf.TwoDNavierStokes.adjoint_gradient(q, target, 5)
     assert e.value.code == gf._lib.GF_EUNSUPPORTED
+
+
+def test_concurrent_gradients_equal_sequential_ones(gf):
+    # a gradient that starts while another runs keeps its sweep on one stream; alone, the
+    # recomputation runs on a second stream beside the sweep: both give the same bits
+    from concurrent.futures import ThreadPoolExecutor
+    setups = [_setup(gf, 64, 1e-3, 70 + k) for k in range(4)]
+    seq = [gf.TwoDNavierStokes.adjoint_gradient(p, tgt, 40, 6) for p, _, tgt in setups]
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        par = list(pool.map(lambda s: gf.TwoDNavierStokes.adjoint_gradient(s[0], s[2], 40, 6), setups))
+    for a, b in zip(seq, par):
+        assert a[0] == b[0] and a[2] == b[2]
+        assert np.array_equal(a[1], b[1])
