@@ -227,6 +227,7 @@ struct Ctx final : CtxBase {
   DevBuf sol, s1, s2, n1, n23, racc, nout, mixed, flux, twx, twy, phys, jl, cinp;
   DevBuf coef[6], filter, eta, etah, etax, etay, Uy, Qy1, sinv, Fh, sqrt_spec;
   DevBuf prevsol, diag_part;
+  DevBuf psib;               // multi-layer with drag: the bottom layer's psih, K1 -> K3
   bool prevsol_valid = false;
   std::vector<double> Hlay, gprime;   // MLQG layer depths and reduced gravities
   KernelTimer tm;
@@ -263,6 +264,7 @@ struct Ctx final : CtxBase {
     build_stepper();
     build_forcing();
     if (d.forcing == GF_FORCING_FIXED) Fh.alloc(sizeof(C) * fsz * nl);   // zero until gf_set_forcing
+    if (nl > 1 && d.mu != 0) psib.alloc(sizeof(C) * fsz);
     Sched tuned;
     if (nl == 1 && sched_lookup(sched_key(), tuned)) for (int k = 0; k < 3; ++k) sched[k] = tuned.p[k];
     HIPCHECK(hipStreamSynchronize(stream));
@@ -658,6 +660,13 @@ struct Ctx final : CtxBase {
     c.seed = d.seed; c.fstep = stepno; c.sqrt_dt = (T)std::sqrt(d.dt); c.fkx = fkx; c.fky = fky;
     c.instance = d.instance;
     if (d.model == GF_MODEL_MLQG && d.mu != 0) { c.mu_drag = (T)d.mu; c.drag_src = in; c.sinv = sinv.as<T>(); }
+    if (nl > 1) {
+      // one layer per workgroup row: three times the workgroups of the layer loop (at 512^2 the
+      // x pass had one 64-thread workgroup per row pair); K1 hands the drag's psih to K3
+      a.lsplit = b.lsplit = c.lsplit = 1;
+      a.psi_out = psib.as<C>();
+      c.psi_in = psib.as<C>();
+    }
     c.sol = sol.as<C>(); c.s1 = s1.as<C>(); c.s2 = s2.as<C>(); c.n1 = n1.as<C>(); c.n23 = n23.as<C>();
     c.racc = racc.as<C>(); c.nout = nout.as<C>();
     c.c0 = coef[0].p; c.c1 = coef[1].p; c.c2 = coef[2].p; c.c3 = coef[3].p; c.c4 = coef[4].p; c.c5 = coef[5].p;

@@ -63,6 +63,8 @@ struct K1Args {
   T ell2inv;
   const T* sinv;            // MLQG: S^-1 per mode, n*n reals, blocked layout (n*n per mode)
   int nomask;               // MLQG linearised equations: the state is read without dealiasing
+  int lsplit;               // multi-layer: one layer per workgroup row (blockIdx.y)
+  cplx<T>* psi_out;         // multi-layer calcN, nullable: the bottom layer's psih (for K3's drag)
   const cplx<T>* tw;
 };
 
@@ -82,6 +84,7 @@ struct K2Args {
   const T* etax;            // ny*nx, bottom layer, nullable
   const T* etay;            // ny*nx, bottom layer, nullable
   int linear;               // MLQG linearised equations: advect q by U only
+  int lsplit;               // multi-layer: one layer per workgroup row (blockIdx.y)
   int has_bg;               // any nonzero Qx/Qy/U term
   cplx<T>* out;             // flux (2 fields per layer) or mixed (forward-in)
   long ostride;
@@ -120,6 +123,8 @@ struct K3Args {
   const T* filter;          // nullable
   int zero_mean;            // E_STORE: set mode (0,0) to zero
   int nomask;               // MLQG linearised equations: no dealiasing of the state
+  int lsplit;               // multi-layer: one layer per workgroup row (blockIdx.y)
+  const cplx<T>* psi_in;    // multi-layer, nullable: the bottom layer's psih written by K1
   int ab3_start;            // AB3: forward Euler start-up step
   // adjoint tape (ETDRK4): nsave receives each stage's N, sol_out the new state of
   // E_ETD4 (both nullable: no tape / update sol in place)

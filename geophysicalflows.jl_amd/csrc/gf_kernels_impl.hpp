@@ -144,8 +144,11 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
     if (active && lk && krk) v = at(a.sol, (uint32_t)(j * g.lstride) + boff32(kr, l, NY));
     return v;
   };
-#pragma unroll
-  for (int jl = 0; jl < NL; ++jl) {
+  // multi-layer problems may give each layer its own workgroups (blockIdx.y): the inversion
+  // reads every layer, the fields and transforms are the layer's own
+  const int jl0 = (NL > 1 && a.lsplit) ? (int)blockIdx.y : 0;
+  const int jl1 = (NL > 1 && a.lsplit) ? jl0 + 1 : NL;
+  for (int jl = jl0; jl < jl1; ++jl) {
     // per-point inversion factor, computed once before the fields that use it:
     // TwoDNS 1/|k|^2 (invKrsq, [FF] TwoDGrid), QG psih (src/singlelayerqg.jl:484-490,
     // src/multilayerqg.jl:495-509)
@@ -167,6 +170,8 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
             for (int k = 0; k < NL; ++k) acc = acc + rmul(mm[k], sval(k, l));
           }
           ps[q] = acc;
+          // the bottom layer's psih, for K3's bottom drag of the same stage
+          if (FS == FS_CALCN && jl == NL - 1 && a.psi_out != nullptr && active) at(a.psi_out, boff32(kr, l, NY)) = acc;
         } else {
           const cplx<T> sv = sval(jl, l);
           if (a.eb) ps[q] = mk(-sv.re / (K2 + a.ell2inv), -sv.im / (K2 + a.ell2inv));
@@ -390,8 +395,9 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
     // ---- advection with background gradients: per layer, u, v, q -> (u q, v q, u Qx + v Qy) ----
     constexpr bool FULL = MODE == K2_ADVECT_FULL;
     constexpr int NF = FULL ? 3 : 2;
-#pragma unroll
-    for (int jl = 0; jl < NL; ++jl) {
+    const int jl0 = (NL > 1 && a.lsplit) ? (int)blockIdx.y : 0;
+    const int jl1 = (NL > 1 && a.lsplit) ? jl0 + 1 : NL;
+    for (int jl = jl0; jl < jl1; ++jl) {
       const cplx<T>* fu = a.in + (long)(3 * jl + 0) * a.fstride;
       const cplx<T>* fv = a.in + (long)(3 * jl + 1) * a.fstride;
       const cplx<T>* fq = a.in + (long)(3 * jl + 2) * a.fstride;
@@ -572,22 +578,32 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
   // Bottom drag mu Krsq psih_n (src/multilayerqg.jl:658) needs psi_n of the calcN
   // input; the stage epilogue below may overwrite that input in place (ETDRK4
   // stage 3, RK4 stages 2-3), so psi_n is formed before any layer is written.
+  // With one layer per workgroup (lsplit), K1 of the same stage has stored it (psi_in): a
+  // workgroup of another layer may already have overwritten that layer's input.
   cplx<T> psin[P];
+  const int jl0 = (NL > 1 && a.lsplit) ? (int)blockIdx.y : 0;
+  const int jl1 = (NL > 1 && a.lsplit) ? jl0 + 1 : NL;
   if constexpr (NL > 1) {
+    if (jl1 == NL) {
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-      const int l = t + TPT * q;
-      cplx<T> ps = czero<T>();
-      if (active && a.drag_src != nullptr && ((kept_l(g, l) && kr < g.kmax) || a.nomask)) {
-        const long idx = boff(kr, l, NY);
-        const T* m = a.sinv + idx * (NL * NL) + (NL - 1) * NL;
+      for (int q = 0; q < P; ++q) {
+        const int l = t + TPT * q;
+        cplx<T> ps = czero<T>();
+        if (active && a.drag_src != nullptr && ((kept_l(g, l) && kr < g.kmax) || a.nomask)) {
+          const long idx = boff(kr, l, NY);
+          if (a.psi_in != nullptr) {
+            ps = a.psi_in[idx];
+          } else {
+            const T* m = a.sinv + idx * (NL * NL) + (NL - 1) * NL;
 #pragma unroll
-        for (int k = 0; k < NL; ++k) ps = ps + rmul(m[k], a.drag_src[k * g.lstride + idx]);
+            for (int k = 0; k < NL; ++k) ps = ps + rmul(m[k], a.drag_src[k * g.lstride + idx]);
+          }
+        }
+        psin[q] = ps;
       }
-      psin[q] = ps;
     }
   }
-  for (int jl = 0; jl < NL; ++jl) {
+  for (int jl = jl0; jl < jl1; ++jl) {
     cplx<T> A[1][P], B[1][P];
     const cplx<T>* fa = a.in + (long)(2 * jl) * a.fstride;
     const cplx<T>* fb = a.in + (long)(2 * jl + 1) * a.fstride;
@@ -789,7 +805,7 @@ void k1_np(const K1Args<T>& a, hipStream_t s) {
   constexpr int G = GY<N, P>::G;
   const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
   if (nblk <= 0) return;
-  const dim3 grid(nblk), blk(2 * G * (N / P));
+  const dim3 grid(nblk, (NL > 1 && a.lsplit) ? NL : 1), blk(2 * G * (N / P));
   if constexpr (NL > 1) {
     if (a.fieldset == FS_CALCN) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_CALCN>), grid, blk, 0, s, a);
     else if (a.fieldset == FS_OUT) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_OUT>), grid, blk, 0, s, a);
@@ -823,7 +839,7 @@ void k1_n(const K1Args<T>& a, hipStream_t s) {
 template <typename T, int N, int NL, int P>
 void k2_np(const K2Args<T>& a, hipStream_t s) {
   constexpr int G = GW<N, P>::G;
-  const dim3 grid((a.g.ny / 2 + G - 1) / G), blk(G * (N / P));
+  const dim3 grid((a.g.ny / 2 + G - 1) / G, (NL > 1 && a.lsplit) ? NL : 1), blk(G * (N / P));
   if constexpr (P != PY<N>::P) {          // a tuned schedule: the calcN modes only
     if (a.mode == K2_ADVECT) { if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT, 1>), grid, blk, 0, s, a); }
     else hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT_FULL, NL>), grid, blk, 0, s, a);
@@ -846,13 +862,19 @@ void k2_n(const K2Args<T>& a, hipStream_t s) {
     if (tuned && a.p == 8) { if constexpr (PCand<N>::has(8) && PY<N>::P != 8) { k2_np<T, N, 1, 8>(a, s); return; } }
   }
   (void)tuned;
+  if constexpr (NL > 1) {
+    // multi-layer x pass: 4 points per thread (128-thread transforms at 512) -- one layer per
+    // workgroup row leaves the kernel latency-bound, and more threads per row pair shorten it
+    if (a.mode == K2_ADVECT_FULL && N >= 64) { k2_np<T, N, NL, 4>(a, s); return; }
+  }
   k2_np<T, N, NL, PY<N>::P>(a, s);
 }
 template <typename T, int N, int NL, int P>
 void k3_np(const K3Args<T>& a, hipStream_t s) {
   constexpr int G = GY<N, P>::G;
   const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
-  if (nblk > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, G, NL>), dim3(nblk), dim3(2 * G * (N / P)), 0, s, a);
+  if (nblk > 0)
+    hipLaunchKernelGGL((k3_kernel<T, N, P, G, NL>), dim3(nblk, (NL > 1 && a.lsplit) ? NL : 1), dim3(2 * G * (N / P)), 0, s, a);
 }
 template <typename T, int N, int NL>
 void k3_n(const K3Args<T>& a, hipStream_t s) {
