@@ -14,9 +14,9 @@ from . import output  # noqa: F401
 from . import utils  # noqa: F401
 from .output import Output, get_sol, savediagnostic, saveoutput, saveproblem  # noqa: F401
 from .problem import (CPU, GPU, Diagnostic, Ensemble, RingForcing, autotune, load_schedules,  # noqa: F401
-                      save_schedules, schedule, set_schedule, stepforward)
+                      save_schedules, schedule, set_schedule, set_splits, splits, stepforward)
 from .utils import peakedisotropicspectrum  # noqa: F401
 
 __all__ = ["TwoDNavierStokes", "SingleLayerQG", "MultiLayerQG", "GPU", "CPU", "Ensemble", "RingForcing", "Diagnostic", "stepforward",
            "peakedisotropicspectrum", "Output", "get_sol", "saveproblem", "saveoutput", "savediagnostic",
-           "autotune", "schedule", "set_schedule", "save_schedules", "load_schedules"]
+           "autotune", "schedule", "set_schedule", "splits", "set_splits", "save_schedules", "load_schedules"]

@@ -185,6 +185,8 @@ struct CtxBase {
   virtual void set_graphs(int) {}
   virtual void set_schedule(const int*) { throw GfError(GF_EUNSUPPORTED, "radix schedules are set per problem"); }
   virtual void get_schedule(int*) const { throw GfError(GF_EUNSUPPORTED, "radix schedules are set per problem"); }
+  virtual void set_splits(const int*) { throw GfError(GF_EUNSUPPORTED, "workgroup splits are set per problem"); }
+  virtual void get_splits(int*) const { throw GfError(GF_EUNSUPPORTED, "workgroup splits are set per problem"); }
   virtual void autotune(int, double*) { throw GfError(GF_EUNSUPPORTED, "the autotuner times one problem; address a member through gf_ctx_instance"); }
   // ensembles (a plain context is an ensemble of one)
   virtual int ninstances() const { return 1; }
@@ -266,7 +268,10 @@ struct Ctx final : CtxBase {
     if (d.forcing == GF_FORCING_FIXED) Fh.alloc(sizeof(C) * fsz * nl);   // zero until gf_set_forcing
     if (nl > 1 && d.mu != 0) psib.alloc(sizeof(C) * fsz);
     Sched tuned;
-    if (nl == 1 && sched_lookup(sched_key(), tuned)) for (int k = 0; k < 3; ++k) sched[k] = tuned.p[k];
+    if (nl == 1 && sched_lookup(sched_key(), tuned)) {
+      for (int k = 0; k < 3; ++k) sched[k] = tuned.p[k];
+      for (int k = 0; k < 2; ++k) split[k] = tuned.split[k];
+    }
     HIPCHECK(hipStreamSynchronize(stream));
   }
   ~Ctx() override {
@@ -673,6 +678,7 @@ struct Ctx final : CtxBase {
     c.ccplx = ccplx; c.cl_layered = 0; c.dt = (T)d.dt; c.filter = filtered() ? filter.as<T>() : nullptr;
     c.tw = twy.as<C>();
     a.p = sched[0]; b.p = sched[1]; c.p = sched[2];
+    if (nl == 1) { a.fsplit = split[0]; b.hsplit = split[1]; }
     if (ab3()) {
       // the RHS history of the aliased band enters later steps: every AB3 step is computed in full
       c.ab3_start = stepno < 3;
@@ -715,6 +721,17 @@ struct Ctx final : CtxBase {
   // ---------------- radix schedules and the autotuner ----------------
   // sched[k]: FFT points per thread of K1, K2, K3 (0: the built-in choice); single-layer models.
   int sched[3] = {0, 0, 0};
+  // split[0]: K1 transforms one field per workgroup row; split[1]: K2 forms u q and v q in two
+  // workgroup rows. More, shorter workgroups for grids too small to fill the GPU.
+  int split[2] = {0, 0};
+  void set_splits(const int* v) override {
+    if (nl != 1 && (v[0] || v[1])) throw GfError(GF_EINVAL, "workgroup splits are for single-layer problems");
+    bool changed = false;
+    for (int k = 0; k < 2; ++k) { changed |= split[k] != (v[k] != 0); split[k] = v[k] != 0; }
+    if (changed)
+      for (auto& ge : gexec) if (ge) { HIPCHECK(hipStreamSynchronize(stream)); (void)hipGraphExecDestroy(ge); ge = nullptr; }
+  }
+  void get_splits(int* v) const override { v[0] = split[0]; v[1] = split[1]; }
   SchedKey sched_key() const { return SchedKey{d.model, nl, d.nx, d.ny, d.dtype}; }
   int sched_len(int k) const { return k == 1 ? d.nx : d.ny; }   // K2 transforms rows, K1/K3 columns
   void set_schedule(const int* p) override {
@@ -730,15 +747,16 @@ struct Ctx final : CtxBase {
   void get_schedule(int* p) const override {
     for (int k = 0; k < 3; ++k) p[k] = sched[k] ? sched[k] : (nl == 1 ? sched_default(sched_len(k)) : (d.ny >= 32 && k != 1 ? 4 : sched_default(sched_len(k))));
   }
-  // Times whole steps under every offered combination of schedules, P in {4, 8} for each of
-  // K1, K2, K3 (two passes, the second in reverse order, best of the two), keeps the fastest
-  // and records it for this (model, size, dtype) so that later contexts start with it. The
-  // state, its stepper history and the clock are restored afterwards. us (nullable) receives
-  // the 8 step times in microseconds, combination c = 4 (P_K1 == 8) + 2 (P_K2 == 8) + (P_K3 == 8),
-  // NaN where a combination is not offered.
+  // Times whole steps under every offered schedule -- P in {4, 8} for each of K1, K2, K3 and
+  // the two workgroup splits on or off, up to 32 combinations -- in two passes (the second in
+  // reverse order, best of the two), keeps the fastest and records it for this (model, size,
+  // dtype) so that later contexts start with it. The state, its stepper history and the clock
+  // are restored afterwards. us (nullable) receives 32 step times in microseconds, combination
+  // c = 16 split_K1 + 8 split_K2 + 4 (P_K1 == 8) + 2 (P_K2 == 8) + (P_K3 == 8), NaN where a
+  // combination is not offered.
   void autotune(int steps, double* us) override {
     if (steps < 1) throw GfError(GF_EINVAL, "gf_autotune: steps must be >= 1");
-    if (us) for (int j = 0; j < 8; ++j) us[j] = std::nan("");
+    if (us) for (int j = 0; j < 32; ++j) us[j] = std::nan("");
     if (nl != 1) return;   // MultiLayerQG keeps its fixed schedule
     HIPCHECK(hipStreamSynchronize(stream));
     const size_t cb = sizeof(C) * fsz * nl;
@@ -750,13 +768,20 @@ struct Ctx final : CtxBase {
     const int64_t step0 = stepno;
     const bool pv0 = prevsol_valid;
     const int old[3] = {sched[0], sched[1], sched[2]};
+    const int old_split[2] = {split[0], split[1]};
+    auto apply = [&](int c) {
+      const int q[3] = {(c & 4) ? 8 : 4, (c & 2) ? 8 : 4, (c & 1) ? 8 : 4};
+      const int v[2] = {(c >> 4) & 1, (c >> 3) & 1};
+      set_schedule(q);
+      set_splits(v);
+    };
     std::vector<int> combos;
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < 32; ++c) {
       bool ok = true;
       for (int k = 0; k < 3; ++k) ok &= sched_candidate(sched_len(k), (c >> (2 - k)) & 1 ? 8 : 4);
       if (ok) combos.push_back(c);
     }
-    double best_t[8];
+    double best_t[32];
     for (double& b : best_t) b = INFINITY;
     hipEvent_t ev[2] = {nullptr, nullptr};
     auto cleanup = [&] {
@@ -768,8 +793,7 @@ struct Ctx final : CtxBase {
       for (int pass = 0; pass < 2; ++pass) {
         for (size_t i = 0; i < combos.size(); ++i) {
           const int c = combos[pass ? combos.size() - 1 - i : i];
-          const int q[3] = {(c & 4) ? 8 : 4, (c & 2) ? 8 : 4, (c & 1) ? 8 : 4};
-          set_schedule(q);
+          apply(c);
           step(2);
           HIPCHECK(hipEventRecord(ev[0], stream));
           step(steps);
@@ -783,6 +807,7 @@ struct Ctx final : CtxBase {
     } catch (...) {
       cleanup();
       set_schedule(old);
+      set_splits(old_split);
       throw;
     }
     cleanup();
@@ -794,11 +819,11 @@ struct Ctx final : CtxBase {
       if (us) us[c] = best_t[c];
       if (bc < 0 || best_t[c] < best_t[bc]) bc = c;
     }
-    if (bc < 0) { set_schedule(old); return; }
-    const int pick[3] = {(bc & 4) ? 8 : 4, (bc & 2) ? 8 : 4, (bc & 1) ? 8 : 4};
-    set_schedule(pick);
+    if (bc < 0) { set_schedule(old); set_splits(old_split); return; }
+    apply(bc);
     Sched e;
-    for (int k = 0; k < 3; ++k) e.p[k] = pick[k];
+    for (int k = 0; k < 3; ++k) e.p[k] = sched[k];
+    for (int k = 0; k < 2; ++k) e.split[k] = split[k];
     e.step_us = best_t[bc];
     sched_store(sched_key(), e);
   }
@@ -1347,6 +1372,8 @@ struct BatchCtx final : CtxBase {
           h[i] = BatchItem<T>{};
           h[i].n = m.d.nx;
           m.make_args(ins[k], epis[k], full != 0, h[i].k1, h[i].k2, h[i].k3);
+          h[i].k1.fsplit = 0;   // the batched launches are one-dimensional
+          h[i].k2.hsplit = 0;
         }
         items[k][full].alloc(sizeof(BatchItem<T>) * n);
         HIPCHECK(hipMemcpy(items[k][full].p, h.data(), sizeof(BatchItem<T>) * n, hipMemcpyHostToDevice));
@@ -1707,6 +1734,19 @@ int gf_get_schedule(gf_ctx* c, int* p_k1, int* p_k2, int* p_k3) {
     if (p_k1) *p_k1 = p[0];
     if (p_k2) *p_k2 = p[1];
     if (p_k3) *p_k3 = p[2];
+  }
+  return rc;
+}
+int gf_set_splits(gf_ctx* c, int k1_fields, int k2_halves) {
+  const int v[2] = {k1_fields, k2_halves};
+  return guarded(c, [&] { c->impl->set_splits(v); });
+}
+int gf_get_splits(gf_ctx* c, int* k1_fields, int* k2_halves) {
+  int v[2] = {0, 0};
+  const int rc = guarded(c, [&] { c->impl->get_splits(v); });
+  if (rc == GF_OK) {
+    if (k1_fields) *k1_fields = v[0];
+    if (k2_halves) *k2_halves = v[1];
   }
   return rc;
 }

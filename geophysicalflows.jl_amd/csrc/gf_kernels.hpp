@@ -64,6 +64,7 @@ struct K1Args {
   const T* sinv;            // MLQG: S^-1 per mode, n*n reals, blocked layout (n*n per mode)
   int nomask;               // MLQG linearised equations: the state is read without dealiasing
   int lsplit;               // multi-layer: one layer per workgroup row (blockIdx.y)
+  int fsplit;               // single-layer calcN: one output field per workgroup row (blockIdx.y)
   cplx<T>* psi_out;         // multi-layer calcN, nullable: the bottom layer's psih (for K3's drag)
   const cplx<T>* tw;
 };
@@ -85,6 +86,7 @@ struct K2Args {
   const T* etay;            // ny*nx, bottom layer, nullable
   int linear;               // MLQG linearised equations: advect q by U only
   int lsplit;               // multi-layer: one layer per workgroup row (blockIdx.y)
+  int hsplit;               // single-layer advection: row 0 forms u q, row 1 v q (blockIdx.y)
   int has_bg;               // any nonzero Qx/Qy/U term
   cplx<T>* out;             // flux (2 fields per layer) or mixed (forward-in)
   long ostride;

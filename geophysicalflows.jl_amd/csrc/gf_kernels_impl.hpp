@@ -181,6 +181,11 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
     }
 #pragma unroll
     for (int f = 0; f < FSET::N; ++f) {
+      // with fsplit each workgroup row transforms one field (a uniform branch: every wave of
+      // the workgroup skips the same fields)
+      if constexpr (NL == 1 && FS == FS_CALCN) {
+        if (a.fsplit && f != (int)blockIdx.y) continue;
+      }
       const int code = FSET::code(f);
       cplx<T> w[1][P];
 #pragma unroll
@@ -377,14 +382,16 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
         }
       }
     }
-    {
+    // with hsplit, workgroup row 0 forms u q and row 1 v q (each transforms q itself)
+    const int half = a.hsplit ? (int)blockIdx.y : 2;
+    if (half != 1) {
       cplx<T> w[1][P];
       inverse_rows(fu, w);
 #pragma unroll
       for (int q = 0; q < P; ++q) w[0][q] = mk(w[0][q].re * q0[q], -w[0][q].im * q1[q]);   // u q, both rows
       forward_rows(a.out, w);
     }
-    {
+    if (half != 0) {
       cplx<T> w[1][P];
       inverse_rows(fv, w);
 #pragma unroll
@@ -805,7 +812,8 @@ void k1_np(const K1Args<T>& a, hipStream_t s) {
   constexpr int G = GY<N, P>::G;
   const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
   if (nblk <= 0) return;
-  const dim3 grid(nblk, (NL > 1 && a.lsplit) ? NL : 1), blk(2 * G * (N / P));
+  const int gy = (NL > 1 && a.lsplit) ? NL : ((NL == 1 && a.fsplit && a.fieldset == FS_CALCN) ? 3 : 1);
+  const dim3 grid(nblk, gy), blk(2 * G * (N / P));
   if constexpr (NL > 1) {
     if (a.fieldset == FS_CALCN) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_CALCN>), grid, blk, 0, s, a);
     else if (a.fieldset == FS_OUT) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_OUT>), grid, blk, 0, s, a);
@@ -839,7 +847,8 @@ void k1_n(const K1Args<T>& a, hipStream_t s) {
 template <typename T, int N, int NL, int P>
 void k2_np(const K2Args<T>& a, hipStream_t s) {
   constexpr int G = GW<N, P>::G;
-  const dim3 grid((a.g.ny / 2 + G - 1) / G, (NL > 1 && a.lsplit) ? NL : 1), blk(G * (N / P));
+  const int gy = (NL > 1 && a.lsplit) ? NL : ((NL == 1 && a.hsplit && a.mode == K2_ADVECT) ? 2 : 1);
+  const dim3 grid((a.g.ny / 2 + G - 1) / G, gy), blk(G * (N / P));
   if constexpr (P != PY<N>::P) {          // a tuned schedule: the calcN modes only
     if (a.mode == K2_ADVECT) { if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT, 1>), grid, blk, 0, s, a); }
     else hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT_FULL, NL>), grid, blk, 0, s, a);

@@ -1,6 +1,8 @@
 // The schedule table and its JSON form:
 //   {"format": "gf-schedules-1", "entries": [
-//     {"model": 0, "nlayers": 1, "nx": 2048, "ny": 2048, "dtype": 0, "p": [8, 8, 4], "step_us": 441.2}, ...]}
+//     {"model": 0, "nlayers": 1, "nx": 2048, "ny": 2048, "dtype": 0, "p": [8, 8, 4], "split": [1, 0],
+//      "step_us": 441.2}, ...]}
+// ("split" may be absent: no splits).
 #include "gf_schedule.hpp"
 
 #include <cctype>
@@ -129,7 +131,7 @@ void sched_save(const std::string& path) {
       const Sched& e = kv.second;
       o << (first ? "\n" : ",\n") << "  {\"model\": " << k[0] << ", \"nlayers\": " << k[1] << ", \"nx\": " << k[2]
         << ", \"ny\": " << k[3] << ", \"dtype\": " << k[4] << ", \"p\": [" << e.p[0] << ", " << e.p[1] << ", "
-        << e.p[2] << "], \"step_us\": ";
+        << e.p[2] << "], \"split\": [" << e.split[0] << ", " << e.split[1] << "], \"step_us\": ";
       std::snprintf(buf, sizeof buf, "%.3f", e.step_us);
       o << buf << "}";
       first = false;
@@ -166,6 +168,16 @@ int sched_load(const std::string& path) {
     if (!p || p->kind != Value::ARR || p->arr.size() != 3) throw std::runtime_error("schedule table: 'p' must hold 3 integers");
     Sched s;
     for (int j = 0; j < 3; ++j) s.p[j] = as_int(&p->arr[j], "p");
+    const Value* sp = e.get("split");
+    if (sp) {
+      if (sp->kind != Value::ARR || sp->arr.size() != 2) throw std::runtime_error("schedule table: 'split' must hold 2 integers");
+      for (int j = 0; j < 2; ++j) {
+        s.split[j] = as_int(&sp->arr[j], "split");
+        if (s.split[j] != 0 && s.split[j] != 1) throw std::runtime_error("schedule table: 'split' entries are 0 or 1");
+      }
+      if ((s.split[0] || s.split[1]) && k[1] != 1)
+        throw std::runtime_error("schedule table: workgroup splits are for single-layer problems");
+    }
     const Value* us = e.get("step_us");
     if (us && us->kind == Value::NUM) s.step_us = us->num;
     // K1 and K3 transform columns (length ny), K2 rows (length nx)

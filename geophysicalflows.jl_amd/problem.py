@@ -244,17 +244,32 @@ def schedule(prob):
     return tuple(x.value for x in p)
 
 
+def set_splits(prob, k1_fields=0, k2_halves=0):
+    """Workgroup splits of a single-layer problem: K1 one field per workgroup row, K2 u q and v q
+    in two rows (more, shorter workgroups for small grids).  No counterpart in the reference."""
+    check(lib.gf_set_splits(prob._ctx, int(k1_fields), int(k2_halves)), prob._ctx)
+
+
+def splits(prob):
+    """The workgroup splits in use, ``(k1_fields, k2_halves)``."""
+    v = [ctypes.c_int() for _ in range(2)]
+    check(lib.gf_get_splits(prob._ctx, *[ctypes.byref(x) for x in v]), prob._ctx)
+    return tuple(x.value for x in v)
+
+
 def autotune(prob, steps=20):
-    """Time whole steps of ``prob`` under every offered radix schedule (state and clock restored),
-    apply the fastest and record it for later problems of the same model, size and dtype.
-    Returns the chosen schedule and the step time in microseconds of each schedule tried."""
-    us = (ctypes.c_double * 8)()
+    """Time whole steps of ``prob`` under every offered schedule (radix and splits; state and
+    clock restored), apply the fastest and record it for later problems of the same model, size
+    and dtype.  Returns the chosen ``schedule + splits`` 5-tuple and the step time in
+    microseconds of each combination tried, keyed the same way."""
+    us = (ctypes.c_double * 32)()
     check(lib.gf_autotune(prob._ctx, int(steps), us), prob._ctx)
     times = {}
-    for c in range(8):
+    for c in range(32):
         if math.isfinite(us[c]):
-            times[(8 if c & 4 else 4, 8 if c & 2 else 4, 8 if c & 1 else 4)] = us[c]
-    return schedule(prob), times
+            key = (8 if c & 4 else 4, 8 if c & 2 else 4, 8 if c & 1 else 4, (c >> 4) & 1, (c >> 3) & 1)
+            times[key] = us[c]
+    return schedule(prob) + splits(prob), times
 
 
 def save_schedules(path):

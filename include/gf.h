@@ -140,17 +140,25 @@ int gf_use_graphs(gf_ctx* ctx, int enable);
 int gf_set_schedule(gf_ctx* ctx, int p_k1, int p_k2, int p_k3);
 int gf_get_schedule(gf_ctx* ctx, int* p_k1, int* p_k2, int* p_k3);
 
-/* The autotuner. It times `steps` whole steps under every offered combination of P for
- * (K1, K2, K3), in two passes with the second in reverse order, keeping each combination's
- * best. The fastest combination is applied. The state, its stepper history and the clock are
- * restored afterwards. The choice is recorded for (model, nlayers, nx, ny, dtype): contexts
- * created later in the process start with it. us (nullable) receives 8 step times in
- * microseconds, indexed 4 (P_K1 == 8) + 2 (P_K2 == 8) + (P_K3 == 8); NaN where a combination
- * is not offered. */
+/* Workgroup splits for single-layer problems (0 off, 1 on), part of the schedule:
+ *   k1_fields: K1 gives each of u, v, q its own row of workgroups;
+ *   k2_halves: K2 forms u q and v q in two rows of workgroups, each transforming q itself.
+ * More and shorter workgroups, for grids too small to fill the GPU; the results are the same. */
+int gf_set_splits(gf_ctx* ctx, int k1_fields, int k2_halves);
+int gf_get_splits(gf_ctx* ctx, int* k1_fields, int* k2_halves);
+
+/* The autotuner. It times `steps` whole steps under every offered schedule on this problem:
+ * P in {4, 8} for each of K1, K2, K3, with each workgroup split on or off, up to 32
+ * combinations. It runs two passes, the second in reverse order, keeping each combination's
+ * best, then applies the fastest. The state, its stepper history and the clock are restored
+ * afterwards. The choice is recorded for (model, nlayers, nx, ny, dtype): contexts created later
+ * in the process start with it. us (nullable) receives 32 step times in microseconds, indexed
+ * 16 split_K1 + 8 split_K2 + 4 (P_K1 == 8) + 2 (P_K2 == 8) + (P_K3 == 8); NaN where a
+ * combination is not offered. */
 int gf_autotune(gf_ctx* ctx, int steps, double* us);
 
 /* The process-wide table of tuned schedules as JSON ("gf-schedules-1": one entry per (model,
- * nlayers, nx, ny, dtype) with its P triple and measured step time). gf_schedules_load returns the
+ * nlayers, nx, ny, dtype) with its P triple, its splits and the measured step time). gf_schedules_load returns the
  * number of entries read (>= 0), or GF_EINVAL for an unreadable or malformed file, or for a
  * schedule this build does not offer. gf_schedules_error() then names the problem.
  * Loaded entries apply to contexts created afterwards. Before the first context is created,

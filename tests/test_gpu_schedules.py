@@ -20,13 +20,15 @@ def _spec(rng, n):
 
 @pytest.mark.parametrize("n", [64, 256])
 @pytest.mark.parametrize("sched", SCHEDULES)
-def test_twodns_every_schedule_matches_oracle(gf, n, sched):
+@pytest.mark.parametrize("split", [(0, 0), (1, 1), (1, 0)])
+def test_twodns_every_schedule_matches_oracle(gf, n, sched, split):
     rng = np.random.default_rng(80 + n)
     s = _spec(rng, n)
     p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, ν=1e-3, nν=1, μ=0.05, dt=0.001, stepper="ETDRK4",
                                     calcF=gf.RingForcing(6.0, 1.5, 0.1, seed=4))
     gf.set_schedule(p, *sched)
-    assert gf.schedule(p) == sched
+    gf.set_splits(p, *split)
+    assert gf.schedule(p) == sched and gf.splits(p) == split
     r = o2d.Problem(nx=n, nu=1e-3, nnu=1, mu=0.05, dt=0.001, stepper="ETDRK4", calcF=("ring", 6.0, 1.5, 0.1, 4))
     N, _ = p.calcN(s)
     Nr, sr = np.zeros_like(s), s.copy()
@@ -52,6 +54,7 @@ def test_slqg_schedules_topography_and_varying_U(gf, sched):
     for kw in (dict(U=0.3, eta=eta), dict(U=Uy, eta=eta)):
         p = gf.SingleLayerQG.Problem(gf.GPU(), nx=n, β=2.0, μ=0.01, dt=0.002, stepper="FilteredRK4", **kw)
         gf.set_schedule(p, *sched)
+        gf.set_splits(p, 1, 1)
         rk = dict(kw)
         r = osl.Problem(nx=n, beta=2.0, mu=0.01, dt=0.002, stepper="FilteredRK4", **rk)
         s = _spec(np.random.default_rng(90), n)
@@ -72,6 +75,8 @@ def test_schedule_not_offered_is_rejected(gf):
     m = gf.MultiLayerQG.Problem(2, gf.GPU(), nx=64, f0=1, H=[1.0, 1.0], b=[0, -1.0], dt=0.01, stepper="RK4")
     with pytest.raises(gf._lib.GFError, match="not offered"):
         gf.set_schedule(m, 4, 4, 4)
+    with pytest.raises(gf._lib.GFError, match="single-layer"):
+        gf.set_splits(m, 1, 0)
 
 
 def test_autotune_restores_state_and_records_the_choice(gf, tmp_path):
@@ -90,20 +95,23 @@ def test_autotune_restores_state_and_records_the_choice(gf, tmp_path):
     chosen, us = gf.autotune(p, steps=6)
     assert np.array_equal(p.sol, before) and p.clock.t == t0 and p.clock.step == k0
     assert T.energy_work(p) == w0
-    assert all(c in (4, 8) for c in chosen) and gf.schedule(p) == chosen
-    assert len(us) == 8 and min(us.values()) > 0 and us[chosen] == min(us.values())
-    gf.set_schedule(q, *chosen)
+    assert all(c in (4, 8) for c in chosen[:3]) and gf.schedule(p) + gf.splits(p) == chosen
+    assert len(us) == 32 and min(us.values()) > 0 and us[chosen] == min(us.values())
+    gf.set_schedule(q, *chosen[:3])
+    gf.set_splits(q, *chosen[3:])
     gf.stepforward(p, 7)                       # the history survived: same steps as the twin
     gf.stepforward(q, 7)
     assert np.isfinite(p.sol).all()
     assert np.array_equal(p.sol, q.sol)
     # later problems of the same kind start with the tuned schedule; the table round-trips
     r = T.Problem(gf.GPU(), **kw)
-    assert gf.schedule(r) == chosen
+    assert gf.schedule(r) + gf.splits(r) == chosen
     path = tmp_path / "sched.json"
     gf.save_schedules(path)
     gf._lib.lib.gf_schedules_clear()
-    assert gf.schedule(T.Problem(gf.GPU(), **kw)) == (8, 8, 8)
+    r = T.Problem(gf.GPU(), **kw)
+    assert gf.schedule(r) == (8, 8, 8) and gf.splits(r) == (0, 0)
     assert gf.load_schedules(path) >= 1
-    assert gf.schedule(T.Problem(gf.GPU(), **kw)) == chosen
+    r = T.Problem(gf.GPU(), **kw)
+    assert gf.schedule(r) + gf.splits(r) == chosen
     gf._lib.lib.gf_schedules_clear()
