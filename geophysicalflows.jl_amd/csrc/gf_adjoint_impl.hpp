@@ -44,6 +44,7 @@ __device__ __forceinline__ void ka1_body(const AdjK1Args<T>& a, int blk, cplx<T>
   };
 #pragma unroll
   for (int f = 0; f < 5; ++f) {
+    if (a.fsplit && f != (int)blockIdx.y) continue;   // one field per workgroup row (uniform)
     cplx<T> w[1][P];
 #pragma unroll
     for (int q = 0; q < P; ++q) {
@@ -130,46 +131,54 @@ __device__ __forceinline__ void ka2_body(const AdjK2Args<T>& a, int blk, cplx<T>
     }
     __syncthreads();
   };
+  // with hsplit, workgroup row 0 forms zeta pA, row 1 zeta pB and row 2 u pA + v pB, each
+  // transforming the fields it needs (uniform branches); otherwise one workgroup forms all three
+  const int part = a.hsplit ? (int)blockIdx.y : 3;
+  const bool all = part == 3;
   T z0[P], z1[P], pa0[P], pa1[P], pb0[P], pb1[P];
-  {
+  if (all || part == 0 || part == 1) {
     cplx<T> w[1][P];
     inverse_rows(2, w);                                 // zeta
 #pragma unroll
     for (int q = 0; q < P; ++q) { z0[q] = w[0][q].re; z1[q] = -w[0][q].im; }
   }
-  {
+  if (all || part == 0 || part == 2) {
     cplx<T> w[1][P];
     inverse_rows(3, w);                                 // pA
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-      pa0[q] = w[0][q].re; pa1[q] = -w[0][q].im;
-      w[0][q] = mk(z0[q] * pa0[q], z1[q] * pa1[q]);    // zeta pA, both rows
+    for (int q = 0; q < P; ++q) { pa0[q] = w[0][q].re; pa1[q] = -w[0][q].im; }
+    if (all || part == 0) {
+#pragma unroll
+      for (int q = 0; q < P; ++q) w[0][q] = mk(z0[q] * pa0[q], z1[q] * pa1[q]);   // zeta pA, both rows
+      forward_rows(0, w);
     }
-    forward_rows(0, w);
   }
-  {
+  if (all || part == 1 || part == 2) {
     cplx<T> w[1][P];
     inverse_rows(4, w);                                 // pB
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-      pb0[q] = w[0][q].re; pb1[q] = -w[0][q].im;
-      w[0][q] = mk(z0[q] * pb0[q], z1[q] * pb1[q]);    // zeta pB
+    for (int q = 0; q < P; ++q) { pb0[q] = w[0][q].re; pb1[q] = -w[0][q].im; }
+    if (all || part == 1) {
+#pragma unroll
+      for (int q = 0; q < P; ++q) w[0][q] = mk(z0[q] * pb0[q], z1[q] * pb1[q]);   // zeta pB
+      forward_rows(1, w);
     }
-    forward_rows(1, w);
   }
-  T c0[P], c1[P];
-  {
-    cplx<T> w[1][P];
-    inverse_rows(0, w);                                 // u
+  if (all || part == 2) {
+    T c0[P], c1[P];
+    {
+      cplx<T> w[1][P];
+      inverse_rows(0, w);                               // u
 #pragma unroll
-    for (int q = 0; q < P; ++q) { c0[q] = w[0][q].re * pa0[q]; c1[q] = -w[0][q].im * pa1[q]; }
-  }
-  {
-    cplx<T> w[1][P];
-    inverse_rows(1, w);                                 // v
+      for (int q = 0; q < P; ++q) { c0[q] = w[0][q].re * pa0[q]; c1[q] = -w[0][q].im * pa1[q]; }
+    }
+    {
+      cplx<T> w[1][P];
+      inverse_rows(1, w);                               // v
 #pragma unroll
-    for (int q = 0; q < P; ++q) w[0][q] = mk(c0[q] + w[0][q].re * pb0[q], c1[q] + (-w[0][q].im) * pb1[q]);
-    forward_rows(2, w);                                 // u pA + v pB
+      for (int q = 0; q < P; ++q) w[0][q] = mk(c0[q] + w[0][q].re * pb0[q], c1[q] + (-w[0][q].im) * pb1[q]);
+      forward_rows(2, w);                               // u pA + v pB
+    }
   }
 }
 
@@ -296,14 +305,14 @@ void adj_n(int k, const void* args, hipStream_t s) {
   if (k == 2) {
     const AdjK2Args<T>& a = *(const AdjK2Args<T>*)args;
     constexpr int G = GW<N, P>::G;
-    hipLaunchKernelGGL((ka2_kernel<T, N, P, G>), dim3((a.g.ny / 2 + G - 1) / G), dim3(G * (N / P)), 0, s, a);
+    hipLaunchKernelGGL((ka2_kernel<T, N, P, G>), dim3((a.g.ny / 2 + G - 1) / G, a.hsplit ? 3 : 1), dim3(G * (N / P)), 0, s, a);
     return;
   }
   constexpr int G = GY<N, P>::G;
   if (k == 1) {
     const AdjK1Args<T>& a = *(const AdjK1Args<T>*)args;
     const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
-    if (nblk > 0) hipLaunchKernelGGL((ka1_kernel<T, N, P, G>), dim3(nblk), dim3(2 * G * (N / P)), 0, s, a);
+    if (nblk > 0) hipLaunchKernelGGL((ka1_kernel<T, N, P, G>), dim3(nblk, a.fsplit ? 5 : 1), dim3(2 * G * (N / P)), 0, s, a);
   } else {
     const AdjK3Args<T>& a = *(const AdjK3Args<T>*)args;
     const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;

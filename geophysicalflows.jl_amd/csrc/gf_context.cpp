@@ -1165,9 +1165,11 @@ struct Ctx final : CtxBase {
         a.x = stage == 4 ? lamb.as<C>() : stage == 3 ? lN3b.as<C>() : stage == 2 ? lN2b.as<C>() : lN1b.as<C>();
         a.lamc = stage == 4 ? (const T*)coef[5].p : nullptr;
         a.out = adj_mixed.as<C>(); a.fstride = fsz; a.ncols = g.kmax; a.tw = twy.as<C>();
+        a.fsplit = split[0];   // the forward problem's tuned workgroup splits
         AdjK2Args<T> b{};
         b.g = g; b.in = adj_mixed.as<C>(); b.fstride = fsz; b.kin = g.kmax;
         b.out = adj_flux.as<C>(); b.ostride = fsz; b.kout = g.kmax; b.tw = twx.as<C>();
+        b.hsplit = split[1];
         AdjK3Args<T> k{};
         k.g = g; k.in = adj_flux.as<C>(); k.fstride = fsz; k.ncols = g.kmax; k.stage = stage;
         k.lam = lamb.as<C>(); k.l0 = l0b.as<C>(); k.l1 = l1b.as<C>();

@@ -145,6 +145,7 @@ struct AdjK1Args {
   cplx<T>* out;             // 5 mixed fields: u, v, zeta, pA, pB
   long fstride;
   int ncols;                // kmax
+  int fsplit;               // one field per workgroup row (blockIdx.y)
   const cplx<T>* tw;
 };
 
@@ -157,6 +158,7 @@ struct AdjK2Args {
   cplx<T>* out;             // 3 flux fields: x-spectra of zeta pA, zeta pB, u pA + v pB
   long ostride;
   int kout;
+  int hsplit;               // one flux field per workgroup row (blockIdx.y)
   const cplx<T>* tw;
 };
 
