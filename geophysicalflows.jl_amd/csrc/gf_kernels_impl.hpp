@@ -339,7 +339,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
     const cplx<T>* fu = a.in;
     const cplx<T>* fv = a.in + a.fstride;
     const cplx<T>* fq = a.in + 2 * a.fstride;
-    auto inverse_rows = [&](const cplx<T>* f, cplx<T> (&w)[1][P]) {
+    auto load_rows = [&](const cplx<T>* f, cplx<T> (&w)[1][P]) {
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         const int k = t + TPT * q;
@@ -347,6 +347,9 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
         const cplx<T> B = herm_load<T, NX>(f, k, y0 + 1, ny, a.kin, active);
         w[0][q] = mk(A.re - B.im, -(A.im + B.re));     // conj(A + i B)
       }
+    };
+    auto inverse_rows = [&](const cplx<T>* f, cplx<T> (&w)[1][P]) {
+      load_rows(f, w);
       fft_forward<T, NX, P, 1>(w, L, t, tw);             // row y0 = Re, row y0 + 1 = -Im
     };
     auto forward_rows = [&](cplx<T>* o, cplx<T> (&w)[1][P]) {
