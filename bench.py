@@ -354,7 +354,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
-              "the multi-GPU decomposition is described in DESIGN.md and not built.", file=sys.stderr)
+              "the decomposition over several GPUs is designed but not built.", file=sys.stderr)
         sys.exit(2)
     import __graft_entry__ as ge
     gf = ge.load_package()

@@ -338,11 +338,19 @@ class Diagnostic:
 
     def __del__(self):
         dev = getattr(self, "_dev", None)
-        if dev:
+        if not dev:
+            return
+        self._dev = None
+        try:
             if getattr(self.prob, "_ctx", None) is not None:
                 self.prob.synchronize()        # no record may still be in flight
-            lib.gf_device_free(ctypes.c_void_p(dev))
-            self._dev = None
+        except Exception:
+            pass                               # (hipFree below waits for the device anyway)
+        finally:
+            try:
+                lib.gf_device_free(ctypes.c_void_p(dev))
+            except Exception:
+                pass
 
     @property
     def ndata(self):
