@@ -747,16 +747,17 @@ struct Ctx final : CtxBase {
   void get_schedule(int* p) const override {
     for (int k = 0; k < 3; ++k) p[k] = sched[k] ? sched[k] : (nl == 1 ? sched_default(sched_len(k)) : (d.ny >= 32 && k != 1 ? 4 : sched_default(sched_len(k))));
   }
-  // Times whole steps under every offered schedule -- P in {4, 8} for each of K1, K2, K3 and
-  // the two workgroup splits on or off, up to 32 combinations -- in two passes (the second in
-  // reverse order, best of the two), keeps the fastest and records it for this (model, size,
+  // Times whole steps under every offered schedule -- P in {4, 8, 16} for each of K1, K2, K3
+  // and the two workgroup splits on or off, up to 108 combinations -- in two passes (the second
+  // in reverse order, best of the two), keeps the fastest and records it for this (model, size,
   // dtype) so that later contexts start with it. The state, its stepper history and the clock
-  // are restored afterwards. us (nullable) receives 32 step times in microseconds, combination
-  // c = 16 split_K1 + 8 split_K2 + 4 (P_K1 == 8) + 2 (P_K2 == 8) + (P_K3 == 8), NaN where a
-  // combination is not offered.
+  // are restored afterwards. us (nullable) receives 108 step times in microseconds, combination
+  // c = (((i1 3 + i2) 3 + i3) 2 + split_K1) 2 + split_K2 with i = 0, 1, 2 for P = 4, 8, 16,
+  // NaN where a combination is not offered.
   void autotune(int steps, double* us) override {
     if (steps < 1) throw GfError(GF_EINVAL, "gf_autotune: steps must be >= 1");
-    if (us) for (int j = 0; j < 32; ++j) us[j] = std::nan("");
+    constexpr int NC = 108;
+    if (us) for (int j = 0; j < NC; ++j) us[j] = std::nan("");
     if (nl != 1) return;   // MultiLayerQG keeps its fixed schedule
     HIPCHECK(hipStreamSynchronize(stream));
     const size_t cb = sizeof(C) * fsz * nl;
@@ -769,19 +770,29 @@ struct Ctx final : CtxBase {
     const bool pv0 = prevsol_valid;
     const int old[3] = {sched[0], sched[1], sched[2]};
     const int old_split[2] = {split[0], split[1]};
+    static constexpr int PV[3] = {4, 8, 16};
+    auto decode = [](int c, int* q, int* v) {
+      v[1] = c & 1; v[0] = (c >> 1) & 1;
+      int r = c >> 2;
+      q[2] = PV[r % 3]; r /= 3;
+      q[1] = PV[r % 3]; r /= 3;
+      q[0] = PV[r % 3];
+    };
     auto apply = [&](int c) {
-      const int q[3] = {(c & 4) ? 8 : 4, (c & 2) ? 8 : 4, (c & 1) ? 8 : 4};
-      const int v[2] = {(c >> 4) & 1, (c >> 3) & 1};
+      int q[3], v[2];
+      decode(c, q, v);
       set_schedule(q);
       set_splits(v);
     };
     std::vector<int> combos;
-    for (int c = 0; c < 32; ++c) {
+    for (int c = 0; c < NC; ++c) {
+      int q[3], v[2];
+      decode(c, q, v);
       bool ok = true;
-      for (int k = 0; k < 3; ++k) ok &= sched_candidate(sched_len(k), (c >> (2 - k)) & 1 ? 8 : 4);
+      for (int k = 0; k < 3; ++k) ok &= sched_candidate(sched_len(k), q[k]);
       if (ok) combos.push_back(c);
     }
-    double best_t[32];
+    double best_t[NC];
     for (double& b : best_t) b = INFINITY;
     hipEvent_t ev[2] = {nullptr, nullptr};
     auto cleanup = [&] {

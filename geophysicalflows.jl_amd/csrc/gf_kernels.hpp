@@ -220,7 +220,7 @@ template <typename T> void launch_from_julia(const Geo<T>& g, const cplx<T>* src
 template <typename T> void launch_mask(const Geo<T>& g, cplx<T>* sol, int nl, hipStream_t s);
 void launch_counter(int64_t* c, int64_t v, int add, hipStream_t s);
 // Radix schedules offered to the autotuner for an FFT of length n (the host side of PCand).
-inline bool sched_candidate(int n, int p) { return p == 4 ? n >= 16 : (p == 8 ? n >= 32 : false); }
+inline bool sched_candidate(int n, int p) { return p == 4 ? n >= 16 : (p == 8 ? n >= 32 : (p == 16 ? n >= 1024 : false)); }
 // The built-in choice (PY in the kernels): P = 4 below 32 points, else 8.
 inline int sched_default(int n) { return n < 32 ? 4 : 8; }
 bool size_supported(int n);

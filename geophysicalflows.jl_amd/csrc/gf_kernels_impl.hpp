@@ -803,11 +803,12 @@ template <int N, int P> struct GW {
 template <int N, int P> struct GY { static constexpr int G = 2 * (N / P) >= 64 ? 1 : 64 / (2 * (N / P)); };
 
 // Radix schedules the autotuner may choose for the single-layer calcN kernels: FFT points per
-// thread P (P = 4 doubles the threads per transform and halves each thread's butterfly chain
-// and register footprint).  P = 16 is not offered: at 2048 it needs all 256 VGPRs and spills in
-// K2.  Every other path uses the built-in choice.  Keep in step with sched_candidate() on the host.
+// thread P.  P = 4 doubles the threads per transform and halves each thread's butterfly chain
+// and register footprint.  P = 16 (from 1024 points) needs one LDS exchange fewer per transform
+// (2048 = 16 16 8) at twice the registers, which Float32 can afford.  Every other path uses the
+// built-in choice.  Keep in step with sched_candidate() on the host.
 template <int N> struct PCand {
-  static constexpr bool has(int p) { return p == 4 ? N >= 16 : (p == 8 ? N >= 32 : false); }
+  static constexpr bool has(int p) { return p == 4 ? N >= 16 : (p == 8 ? N >= 32 : (p == 16 ? N >= 1024 : false)); }
 };
 
 template <typename T, int N, int NL, int P>
@@ -844,6 +845,7 @@ void k1_n(const K1Args<T>& a, hipStream_t s) {
     const bool tuned = a.fieldset == FS_CALCN && a.p != 0 && a.p != PY<N>::P && PCand<N>::has(a.p);
     if (tuned && a.p == 4) { if constexpr (PCand<N>::has(4) && PY<N>::P != 4) { k1_np<T, N, 1, 4>(a, s); return; } }
     if (tuned && a.p == 8) { if constexpr (PCand<N>::has(8) && PY<N>::P != 8) { k1_np<T, N, 1, 8>(a, s); return; } }
+    if (tuned && a.p == 16) { if constexpr (PCand<N>::has(16) && PY<N>::P != 16) { k1_np<T, N, 1, 16>(a, s); return; } }
     k1_np<T, N, 1, PY<N>::P>(a, s);
   }
 }
@@ -872,6 +874,7 @@ void k2_n(const K2Args<T>& a, hipStream_t s) {
   if constexpr (NL == 1) {
     if (tuned && a.p == 4) { if constexpr (PCand<N>::has(4) && PY<N>::P != 4) { k2_np<T, N, 1, 4>(a, s); return; } }
     if (tuned && a.p == 8) { if constexpr (PCand<N>::has(8) && PY<N>::P != 8) { k2_np<T, N, 1, 8>(a, s); return; } }
+    if (tuned && a.p == 16) { if constexpr (PCand<N>::has(16) && PY<N>::P != 16) { k2_np<T, N, 1, 16>(a, s); return; } }
   }
   (void)tuned;
   if constexpr (NL > 1) {
@@ -896,6 +899,7 @@ void k3_n(const K3Args<T>& a, hipStream_t s) {
     const bool tuned = a.p != 0 && a.p != PY<N>::P && PCand<N>::has(a.p);
     if (tuned && a.p == 4) { if constexpr (PCand<N>::has(4) && PY<N>::P != 4) { k3_np<T, N, 1, 4>(a, s); return; } }
     if (tuned && a.p == 8) { if constexpr (PCand<N>::has(8) && PY<N>::P != 8) { k3_np<T, N, 1, 8>(a, s); return; } }
+    if (tuned && a.p == 16) { if constexpr (PCand<N>::has(16) && PY<N>::P != 16) { k3_np<T, N, 1, 16>(a, s); return; } }
     k3_np<T, N, 1, PY<N>::P>(a, s);
   }
 }

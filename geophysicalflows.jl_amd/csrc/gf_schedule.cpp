@@ -24,7 +24,7 @@ std::map<SchedKey, Sched>& table() {
   return t;
 }
 
-bool offered(int n, int p) { return p == 0 || (p == 4 && n >= 16) || (p == 8 && n >= 32); }
+bool offered(int n, int p) { return p == 0 || (p == 4 && n >= 16) || (p == 8 && n >= 32) || (p == 16 && n >= 1024); }
 
 // A small JSON reader for the table's own format: objects, arrays, numbers and strings.
 struct Value {

@@ -233,7 +233,7 @@ def stepforward(prob, nsteps=1, diags=None):
 
 def set_schedule(prob, p_k1=0, p_k2=0, p_k3=0):
     """FFT points per thread of the three calcN kernels (K1 inverse y-pass, K2 x-pass, K3 forward
-    y-pass and stage): 4 or 8, 0 for the built-in choice.  No counterpart in the reference."""
+    y-pass and stage): 4, 8 or 16, 0 for the built-in choice.  No counterpart in the reference."""
     check(lib.gf_set_schedule(prob._ctx, int(p_k1), int(p_k2), int(p_k3)), prob._ctx)
 
 
@@ -262,12 +262,14 @@ def autotune(prob, steps=20):
     clock restored), apply the fastest and record it for later problems of the same model, size
     and dtype.  Returns the chosen ``schedule + splits`` 5-tuple and the step time in
     microseconds of each combination tried, keyed the same way."""
-    us = (ctypes.c_double * 32)()
+    us = (ctypes.c_double * 108)()
     check(lib.gf_autotune(prob._ctx, int(steps), us), prob._ctx)
     times = {}
-    for c in range(32):
+    pv = (4, 8, 16)
+    for c in range(108):
         if math.isfinite(us[c]):
-            key = (8 if c & 4 else 4, 8 if c & 2 else 4, 8 if c & 1 else 4, (c >> 4) & 1, (c >> 3) & 1)
+            r = c >> 2
+            key = (pv[r // 9], pv[(r // 3) % 3], pv[r % 3], (c >> 1) & 1, c & 1)
             times[key] = us[c]
     return schedule(prob) + splits(prob), times
 

@@ -134,9 +134,9 @@ int gf_use_graphs(gf_ctx* ctx, int enable);
 /* Radix schedules (no counterpart in the reference, whose FFTs are FFTW's or cuFFT's plans).
  * For single-layer problems the FFT of each calcN kernel runs with P points per thread:
  *   K1 (inverse y-pass), K2 (x-pass with the products), K3 (forward y-pass and stage).
- * P is 4 or 8 (8 only from 32 points); 0 restores the built-in choice (4 below 32 points,
- * else 8). Results differ from another schedule's only by rounding. gf_get_schedule reports
- * the P in use. MultiLayerQG has a fixed schedule. */
+ * P is 4, 8 (from 32 points) or 16 (from 1024 points); 0 restores the built-in choice (4
+ * below 32 points, else 8). Results differ from another schedule's only by rounding.
+ * gf_get_schedule reports the P in use. MultiLayerQG has a fixed schedule. */
 int gf_set_schedule(gf_ctx* ctx, int p_k1, int p_k2, int p_k3);
 int gf_get_schedule(gf_ctx* ctx, int* p_k1, int* p_k2, int* p_k3);
 
@@ -148,13 +148,13 @@ int gf_set_splits(gf_ctx* ctx, int k1_fields, int k2_halves);
 int gf_get_splits(gf_ctx* ctx, int* k1_fields, int* k2_halves);
 
 /* The autotuner. It times `steps` whole steps under every offered schedule on this problem:
- * P in {4, 8} for each of K1, K2, K3, with each workgroup split on or off, up to 32
+ * P in {4, 8, 16} for each of K1, K2, K3, with each workgroup split on or off, up to 108
  * combinations. It runs two passes, the second in reverse order, keeping each combination's
  * best, then applies the fastest. The state, its stepper history and the clock are restored
  * afterwards. The choice is recorded for (model, nlayers, nx, ny, dtype): contexts created later
- * in the process start with it. us (nullable) receives 32 step times in microseconds, indexed
- * 16 split_K1 + 8 split_K2 + 4 (P_K1 == 8) + 2 (P_K2 == 8) + (P_K3 == 8); NaN where a
- * combination is not offered. */
+ * in the process start with it. us (nullable) receives 108 step times in microseconds, indexed
+ * (((i1 * 3 + i2) * 3 + i3) * 2 + split_K1) * 2 + split_K2, where i = 0, 1, 2 for P = 4, 8,
+ * 16; NaN where a combination is not offered. */
 int gf_autotune(gf_ctx* ctx, int steps, double* us);
 
 /* The process-wide table of tuned schedules as JSON ("gf-schedules-1": one entry per (model,

@@ -95,7 +95,7 @@ def test_autotune_restores_state_and_records_the_choice(gf, tmp_path):
     chosen, us = gf.autotune(p, steps=6)
     assert np.array_equal(p.sol, before) and p.clock.t == t0 and p.clock.step == k0
     assert T.energy_work(p) == w0
-    assert all(c in (4, 8) for c in chosen[:3]) and gf.schedule(p) + gf.splits(p) == chosen
+    assert all(c in (4, 8) for c in chosen[:3]) and gf.schedule(p) + gf.splits(p) == chosen   # no 16 at 256
     assert len(us) == 32 and min(us.values()) > 0 and us[chosen] == min(us.values())
     gf.set_schedule(q, *chosen[:3])
     gf.set_splits(q, *chosen[3:])
@@ -115,3 +115,19 @@ def test_autotune_restores_state_and_records_the_choice(gf, tmp_path):
     r = T.Problem(gf.GPU(), **kw)
     assert gf.schedule(r) + gf.splits(r) == chosen
     gf._lib.lib.gf_schedules_clear()
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 2e-6)])
+@pytest.mark.parametrize("sched", [(16, 16, 16), (16, 8, 4), (8, 16, 16)])
+def test_sixteen_point_schedules_at_1024(gf, dtype, tol, sched):
+    n = 1024
+    rng = np.random.default_rng(95)
+    s = _spec(rng, n)
+    p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, ν=1e-3, nν=1, dt=0.001, stepper="ETDRK4", T=dtype)
+    gf.set_schedule(p, *sched)
+    assert gf.schedule(p) == sched
+    r = o2d.Problem(nx=n, nu=1e-3, nnu=1, dt=0.001, stepper="ETDRK4")
+    N, _ = p.calcN(s)
+    Nr, sr = np.zeros_like(s), s.copy()
+    o2d.calcN(Nr, sr, 0.0, r.clock, r.vars, r.params, r.grid)
+    assert rel(N, Nr) < tol
