@@ -13,10 +13,10 @@ from . import twodnavierstokes as TwoDNavierStokes  # noqa: F401
 from . import output  # noqa: F401
 from . import utils  # noqa: F401
 from .output import Output, get_sol, savediagnostic, saveoutput, saveproblem  # noqa: F401
-from .problem import (CPU, GPU, Diagnostic, Ensemble, RingForcing, autotune, load_schedules,  # noqa: F401
-                      save_schedules, schedule, set_schedule, set_splits, splits, stepforward)
+from .problem import (CPU, GPU, Diagnostic, Ensemble, HarmonicForcing, RingForcing, autotune,  # noqa: F401
+                      load_schedules, save_schedules, schedule, set_schedule, set_splits, splits, stepforward)
 from .utils import peakedisotropicspectrum  # noqa: F401
 
-__all__ = ["TwoDNavierStokes", "SingleLayerQG", "MultiLayerQG", "GPU", "CPU", "Ensemble", "RingForcing", "Diagnostic", "stepforward",
+__all__ = ["TwoDNavierStokes", "SingleLayerQG", "MultiLayerQG", "GPU", "CPU", "Ensemble", "RingForcing", "HarmonicForcing", "Diagnostic", "stepforward",
            "peakedisotropicspectrum", "Output", "get_sol", "saveproblem", "saveoutput", "savediagnostic",
            "autotune", "schedule", "set_schedule", "splits", "set_splits", "save_schedules", "load_schedules"]

@@ -29,7 +29,8 @@ EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string"
            "gf_diagnostic_async", "gf_snapshot", "gf_snapshot_wait", "gf_host_alloc", "gf_host_free",
            "gf_device_alloc", "gf_device_free", "gf_device_copy", "gf_use_graphs",
            "gf_set_schedule", "gf_get_schedule", "gf_autotune", "gf_schedules_save", "gf_schedules_load",
-           "gf_schedules_clear", "gf_schedules_error", "gf_set_splits", "gf_get_splits")
+           "gf_schedules_clear", "gf_schedules_error", "gf_set_splits", "gf_get_splits",
+           "gf_set_forcing_harmonic")
 
 
 class ProblemDesc(ctypes.Structure):
@@ -111,6 +112,7 @@ def _load():
     lib.gf_get_schedule.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
     lib.gf_autotune.argtypes = [vp, ip, dp]
     lib.gf_set_splits.argtypes = [vp, ip, ip]
+    lib.gf_set_forcing_harmonic.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double]
     lib.gf_get_splits.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.gf_schedules_save.argtypes = [ctypes.c_char_p]
     lib.gf_schedules_load.argtypes = [ctypes.c_char_p]

@@ -174,6 +174,8 @@ struct CtxBase {
   virtual void updatevars(void* out) = 0;
   virtual void calcN(void* N, void* sol, double t) = 0;
   virtual void set_forcing(const void* Fh) = 0;
+  virtual void set_forcing_harmonic(const void*, double, double) { throw GfError(GF_EINVAL, "forcing is set per problem"); }
+  virtual bool is_harmonic() const { return false; }
   virtual void step(int n) = 0;
   virtual int nvars() const = 0;
   virtual void timing(int enable) = 0;
@@ -231,6 +233,14 @@ struct Ctx final : CtxBase {
   DevBuf prevsol, diag_part;
   DevBuf psib;               // multi-layer with drag: the bottom layer's psih, K1 -> K3
   bool prevsol_valid = false;
+  // time-harmonic forcing F(t) = Fh cos(omega t + phase) (gf_set_forcing_harmonic): K3 scales
+  // the table by cur_fscale, evaluated at each stage's time; last_fscale is the factor of the
+  // last evaluation, which the work diagnostics use (0 before any: the reference's vars.Fh
+  // starts at zero)
+  bool harmonic = false;
+  double f_omega = 0, f_phase = 0;
+  T cur_fscale = 1;
+  double last_fscale = 1;
   std::vector<double> Hlay, gprime;   // MLQG layer depths and reduced gravities
   KernelTimer tm;
 
@@ -661,6 +671,7 @@ struct Ctx final : CtxBase {
     c.etah = (d.model == GF_MODEL_SLQG && !advect_full && d.U != 0) ? etah.as<C>() : nullptr;
     c.Ueta = (T)d.U;
     c.Fh = (d.forcing == GF_FORCING_FIXED) ? Fh.as<C>() : nullptr;
+    c.fscale = cur_fscale;
     c.sqrt_spec = (d.forcing == GF_FORCING_RING && fkx >= 0) ? sqrt_spec.as<T>() : nullptr;
     c.seed = d.seed; c.fstep = stepno; c.sqrt_dt = (T)std::sqrt(d.dt); c.fkx = fkx; c.fky = fky;
     c.instance = d.instance;
@@ -851,7 +862,10 @@ struct Ctx final : CtxBase {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};   // one interior step, GRAPH_BLOCK interior steps
   DevBuf step_dev;
 
-  bool graph_usable() const { return graphs && !tm.on && !(ab3() && stepno < 3); }
+  bool graph_usable() const { return graphs && !tm.on && !(ab3() && stepno < 3) && !harmonic; }
+  // stage times of the configured stepper, as fractions of dt after the step's start
+  // ([FF] ETDRK4 and RK4: t, t + dt/2, t + dt/2, t + dt; ForwardEuler and AB3: t)
+  static double stage_offset(int k, int ns) { return ns == 1 ? 0.0 : (k == 0 ? 0.0 : (k == 3 ? 1.0 : 0.5)); }
   void interior_launches(const int64_t* fstep_dev) {
     const C* ins[4];
     int epis[4];
@@ -903,7 +917,14 @@ struct Ctx final : CtxBase {
         HIPCHECK(hipGraphLaunch(gexec[which], stream));
         done = which ? GRAPH_BLOCK : 1;
       } else {
-        for (int k = 0; k < ns; ++k) calcN_launch(ins[k], epis[k], full);
+        for (int k = 0; k < ns; ++k) {
+          if (harmonic) {
+            cur_fscale = (T)std::cos(f_omega * (t + stage_offset(k, ns) * d.dt) + f_phase);
+            last_fscale = (double)cur_fscale;
+          }
+          calcN_launch(ins[k], epis[k], full);
+        }
+        cur_fscale = 1;
         counter_set = false;
       }
       s += done;
@@ -956,11 +977,12 @@ struct Ctx final : CtxBase {
     HIPCHECK(hipStreamSynchronize(stream));
   }
   void calcN(void* N, void* solh, double tt) override {
-    (void)tt;
     const size_t bytes = sizeof(C) * (size_t)g.nkr * d.ny * nl;
     HIPCHECK(hipMemcpyAsync(jl.p, solh, bytes, hipMemcpyHostToDevice, stream));
     launch_from_julia<T>(g, jl.as<C>(), cinp.as<C>(), nl, stream);
+    if (harmonic) cur_fscale = (T)std::cos(f_omega * tt + f_phase);   // calcF!(Fh, sol, t, ...) at t
     calcN_launch(cinp.as<C>(), E_CALCN, true);
+    cur_fscale = 1;
     launch_to_julia<T>(g, nout.as<C>(), jl.as<C>(), nl, stream);
     HIPCHECK(hipMemcpyAsync(N, jl.p, bytes, hipMemcpyDeviceToHost, stream));
     if (!(d.model == GF_MODEL_MLQG && d.linear)) launch_mask<T>(g, cinp.as<C>(), nl, stream);   // calcN!'s dealias!
@@ -968,8 +990,17 @@ struct Ctx final : CtxBase {
     HIPCHECK(hipMemcpyAsync(solh, jl.p, bytes, hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
   }
+  bool is_harmonic() const override { return harmonic; }
+  void set_forcing_harmonic(const void* F, double omega, double phase) override {
+    set_forcing(F);
+    harmonic = true;
+    f_omega = omega;
+    f_phase = phase;
+    last_fscale = 0;
+  }
   void set_forcing(const void* F) override {
     if (d.forcing != GF_FORCING_FIXED) throw GfError(GF_EINVAL, "gf_set_forcing: problem was not created with GF_FORCING_FIXED");
+    harmonic = false;
     const size_t bytes = sizeof(C) * (size_t)g.nkr * d.ny * nl;
     HIPCHECK(hipMemcpyAsync(jl.p, F, bytes, hipMemcpyHostToDevice, stream));
     launch_from_julia<T>(g, jl.as<C>(), Fh.as<C>(), nl, stream);
@@ -1052,7 +1083,7 @@ struct Ctx final : CtxBase {
         case DIAG_ENERGY_WORK: case DIAG_ENSTROPHY_WORK:
           if (slqg && eb) bad("not implemented for finite deformation radius (src/singlelayerqg.jl)");
           if (d.forcing == GF_FORCING_NONE) bad("the problem has no forcing");
-          if (d.forcing == GF_FORCING_FIXED) a.Fh = Fh.as<C>();
+          if (d.forcing == GF_FORCING_FIXED) { a.Fh = Fh.as<C>(); a.fscale = harmonic ? (T)last_fscale : (T)1; }
           if (d.forcing == GF_FORCING_RING && prevsol_valid) {
             // before the first step Fh is still zero, and so is the work (as the reference's
             // zero-initialised vars.Fh gives)
@@ -1442,6 +1473,9 @@ struct BatchCtx final : CtxBase {
   }
 
   void step(int n) override {
+    for (auto& m : inst)
+      if (m->is_harmonic())
+        throw GfError(GF_EUNSUPPORTED, "time-harmonic forcing is not supported in ensembles (step the member alone)");
     for (int s = 0; s < n; ++s) {
       const int full = s == n - 1;
       if (!full && graphs && !tm.on) {
@@ -1697,6 +1731,10 @@ int gf_num_vars(const gf_ctx* c) { return c ? c->impl->nvars() : GF_EINVAL; }
 int gf_calcN(gf_ctx* c, void* N, void* sol, double t) {
   if (!N || !sol) return GF_EINVAL;
   return guarded(c, [&] { c->impl->calcN(N, sol, t); });
+}
+int gf_set_forcing_harmonic(gf_ctx* c, const void* F, double omega, double phase) {
+  if (!F) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->set_forcing_harmonic(F, omega, phase); });
 }
 int gf_set_forcing(gf_ctx* c, const void* F) {
   if (!F) return GF_EINVAL;

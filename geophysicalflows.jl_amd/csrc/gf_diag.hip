@@ -98,7 +98,7 @@ __global__ __launch_bounds__(DIAG_THREADS) void diag_kernel(const DiagArgs<T> a,
         // invKrsq sol conj(Fh) or sol conj(Fh); with stochastic forcing (prevsol + sol)/2
         // and Fh the step's realisation (src/twodnavierstokes.jl:494-535)
         cplx<T> F = czero<T>();
-        if (a.Fh != nullptr) F = at(a.Fh, ix);
+        if (a.Fh != nullptr) F = rmul(a.fscale, at(a.Fh, ix));
         if (a.sqrt_spec != nullptr && kr <= a.fkx && (l <= a.fky || l >= ny - a.fky)) {
           const T sp = at(a.sqrt_spec, ix);
           if (sp != (T)0) {

@@ -110,6 +110,7 @@ struct K3Args {
   int apply_kr;             // N = -i kr A - i l B (else A - i l B)
   const cplx<T>* etah; T Ueta;   // SLQG constant-U topography term
   const cplx<T>* Fh;        // fixed forcing, nullable
+  T fscale;                 // factor on Fh: 1, or cos(omega t + phase) at the stage time
   const T* sqrt_spec;       // ring forcing amplitude (blocked real), nullable
   uint64_t seed; int64_t fstep; T sqrt_dt; int fkx, fky; int instance;
   const int64_t* fstep_dev; // nullable: the forcing step is read here (steps replayed from a graph)
@@ -201,6 +202,7 @@ struct DiagArgs {
   const cplx<T>* sol;       // raw state (not dealiased: the reference's diagnostics read sol as is)
   const cplx<T>* prevsol;   // stochastic forcing: state at the start of the last step, nullable
   const cplx<T>* Fh;        // fixed forcing table, nullable
+  T fscale;                 // factor on Fh at the last forcing evaluation
   const T* sqrt_spec;       // ring forcing, nullable
   uint64_t seed; int64_t fstep; T sqrt_dt; int fkx, fky, instance;
   const cplx<T>* etah;      // SLQG topographic PV spectrum, nullable

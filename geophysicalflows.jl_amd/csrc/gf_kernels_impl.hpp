@@ -656,7 +656,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
           N = N + rmul(a.mu_drag * K2, psin[q]);
         }
       }
-      if (a.Fh != nullptr) N = N + at(a.Fh, ix);
+      if (a.Fh != nullptr) N = N + rmul(a.fscale, at(a.Fh, ix));
       if (a.sqrt_spec != nullptr) {
         if (kr <= a.fkx && (l <= a.fky || l >= NY - a.fky)) {
           const T sp = at(a.sqrt_spec, idx);

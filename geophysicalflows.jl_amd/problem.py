@@ -43,6 +43,15 @@ class RingForcing:
         self.kf, self.dkf, self.eps, self.seed, self.instance = float(kf), float(dkf), float(eps), int(seed), int(instance)
 
 
+class HarmonicForcing:
+    """Time-harmonic forcing ``Fh cos(omega t + phase)``, evaluated at each stage's time as the
+    reference's ``calcF!`` is (``test/test_twodnavierstokes.jl:141-145``:
+    ``Fh = fh * cos(2t)``).  ``Fh`` is a spectral array in the layout of ``sol``."""
+
+    def __init__(self, Fh, omega, phase=0.0):
+        self.Fh, self.omega, self.phase = np.asarray(Fh), float(omega), float(phase)
+
+
 class Clock:
     """[FF] ``Clock``: ``dt``, ``t``, ``step`` (read from the device context)."""
 
@@ -186,10 +195,15 @@ class Problem:
         return N, s
 
     def set_forcing(self, Fh):
-        a = np.ascontiguousarray(np.asarray(Fh, dtype=self._ctype).reshape(self._sshape))
+        """A fixed forcing table, or a ``HarmonicForcing``."""
         if self._ctx is None:
-            self._pending_Fh = a
+            self._pending_Fh = Fh
             return
+        if isinstance(Fh, HarmonicForcing):
+            a = np.ascontiguousarray(np.asarray(Fh.Fh, dtype=self._ctype).reshape(self._sshape))
+            check(lib.gf_set_forcing_harmonic(self._ctx, a.ctypes.data, Fh.omega, Fh.phase), self._ctx)
+            return
+        a = np.ascontiguousarray(np.asarray(Fh, dtype=self._ctype).reshape(self._sshape))
         check(lib.gf_set_forcing(self._ctx, a.ctypes.data), self._ctx)
 
     def synchronize(self):
@@ -501,7 +515,8 @@ def dptr(a):
 
 
 def attach_forcing(d, calcF):
-    """Translate a calcF argument into the descriptor; returns the fixed table or None."""
+    """Translate a calcF argument into the descriptor; returns what ``set_forcing`` takes
+    (a fixed table or a ``HarmonicForcing``) or None."""
     if calcF is None:
         d.forcing = _lib.GF_FORCING_NONE
         return None
@@ -509,9 +524,12 @@ def attach_forcing(d, calcF):
         d.forcing = _lib.GF_FORCING_RING
         d.kf, d.dkf, d.eps, d.seed, d.instance = calcF.kf, calcF.dkf, calcF.eps, calcF.seed, calcF.instance
         return None
+    if isinstance(calcF, HarmonicForcing):
+        d.forcing = _lib.GF_FORCING_FIXED
+        return calcF
     if callable(calcF):
         raise NotImplementedError(
             "arbitrary calcF closures cannot run on the device path; pass a fixed spectral "
-            "array or a RingForcing")
+            "array, a HarmonicForcing or a RingForcing")
     d.forcing = _lib.GF_FORCING_FIXED
     return np.asarray(calcF)

@@ -121,6 +121,13 @@ int gf_calcN(gf_ctx* ctx, void* N, void* sol, double t);
  * test/test_twodnavierstokes.jl:239-243); Julia layout, host. */
 int gf_set_forcing(gf_ctx* ctx, const void* Fh);
 
+/* Time-harmonic forcing F(t) = Fh cos(omega t + phase), evaluated at each stage's time as
+ * calcF! is (the calcF! of test/test_twodnavierstokes.jl:141-145 and
+ * test/test_singlelayerqg.jl:127-131 is Fh cos(2t)). For problems created with
+ * GF_FORCING_FIXED. The work diagnostics use the last evaluation's F, which is zero before the
+ * first step. Ensembles do not support it: step such a member alone. */
+int gf_set_forcing_harmonic(gf_ctx* ctx, const void* Fh, double omega, double phase);
+
 /* [FF] stepforward!(prob, nsteps): nsteps of the configured stepper, queued on the context's
  * stream (the call returns before they finish; every read of the state waits for them).
  * With graph replay on, every step but the call's last replays a HIP graph of the whole step
