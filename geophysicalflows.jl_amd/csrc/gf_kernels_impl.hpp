@@ -95,6 +95,17 @@ template <typename T> struct Coef {
 #ifndef GF_K2_WAVES
 #define GF_K2_WAVES 2
 #endif
+// Sizes from which each kernel's transforms recompute their addresses (fft_forward's FRESH),
+// chosen per precision by whole-step timings at 2048² (tuned schedules, ring-forced ETDRK4):
+// - FP64 K2 (P = 8) drops from 147 to 122 VGPRs and fits 4 workgroups per CU, all 1024 row
+//   pairs at once; FP64 K3 at P = 8 spills less.  0.436 -> 0.406 ms/step.
+// - FP32 K1 at P = 8 goes from 82 to 60 VGPRs, 5 to 8 waves per SIMD; FP32 K2 and K3 only pay
+//   the extra integer work.
+// Below 2048 every kernel already has the workgroups it needs in flight and keeps its addresses.
+constexpr int kFreshNever = 1 << 30;
+template <typename T> struct FreshFrom;
+template <> struct FreshFrom<double> { static constexpr int k1 = kFreshNever, k2 = 2048, k3 = 2048; };
+template <> struct FreshFrom<float> { static constexpr int k1 = 2048, k2 = kFreshNever, k3 = kFreshNever; };
 #ifndef GF_K3_WAVES
 #define GF_K3_WAVES 4   // measured faster than 3 despite a small spill (two 512-thread workgroups per CU)
 #endif
@@ -211,7 +222,7 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
         // normalised inverse = conj(F(conj(x)))/(nx ny); the scale is a power of two
         w[0][q] = mk(r.re * nrm, -(r.im * nrm));
       }
-      fft_forward<T, NY, P, 1>(w, L, t, tw);
+      fft_forward<T, NY, P, 1, (NY >= FreshFrom<T>::k1)>(w, L, t, tw);
       if (active) {
         cplx<T>* o = a.out + (long)(jl * FSET::N + f) * a.fstride;
 #pragma unroll
@@ -289,7 +300,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
         }
         w[0][q] = mk(r0, r1);
       }
-      fft_forward<T, NX, P, 1>(w, L, t, tw);
+      fft_forward<T, NX, P, 1, (NX >= FreshFrom<T>::k2)>(w, L, t, tw);
 #pragma unroll
       for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = w[0][q];
       __syncthreads();
@@ -317,7 +328,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
           const cplx<T> B = two ? herm_load<T, NX>(fb, k, y0 + row, ny, a.kin, active) : czero<T>();
           w[0][q] = mk(A.re - B.im, -(A.im + B.re));     // conj(A + i B)
         }
-        fft_forward<T, NX, P, 1>(w, L, t, tw);
+        fft_forward<T, NX, P, 1, (NX >= FreshFrom<T>::k2)>(w, L, t, tw);
         if (active) {
 #pragma unroll
           for (int q = 0; q < P; ++q) {
@@ -350,10 +361,10 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
     };
     auto inverse_rows = [&](const cplx<T>* f, cplx<T> (&w)[1][P]) {
       load_rows(f, w);
-      fft_forward<T, NX, P, 1>(w, L, t, tw);             // row y0 = Re, row y0 + 1 = -Im
+      fft_forward<T, NX, P, 1, (NX >= FreshFrom<T>::k2)>(w, L, t, tw);             // row y0 = Re, row y0 + 1 = -Im
     };
     auto forward_rows = [&](cplx<T>* o, cplx<T> (&w)[1][P]) {
-      fft_forward<T, NX, P, 1>(w, L, t, tw);             // of (row y0 + i row y0 + 1)
+      fft_forward<T, NX, P, 1, (NX >= FreshFrom<T>::k2)>(w, L, t, tw);             // of (row y0 + i row y0 + 1)
 #pragma unroll
       for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = w[0][q];
       __syncthreads();
@@ -427,7 +438,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
           const cplx<T> B = herm_load<T, NX>(fb, k, yb, ny, a.kin, active);
           wr[0][q] = mk(A.re - B.im, -(A.im + B.re));     // conj(A + i B)
         }
-        fft_forward<T, NX, P, 1>(wr, L, t, tw);
+        fft_forward<T, NX, P, 1, (NX >= FreshFrom<T>::k2)>(wr, L, t, tw);
 #pragma unroll
         for (int q = 0; q < P; ++q) w[r][q] = wr[0][q];
       }
@@ -479,7 +490,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
         cplx<T> zr[1][P];
 #pragma unroll
         for (int q = 0; q < P; ++q) zr[0][q] = z[r][q];
-        fft_forward<T, NX, P, 1>(zr, L, t, tw);
+        fft_forward<T, NX, P, 1, (NX >= FreshFrom<T>::k2)>(zr, L, t, tw);
 #pragma unroll
         for (int q = 0; q < P; ++q) z[r][q] = zr[0][q];
       }
@@ -535,7 +546,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
 }
 
 #ifndef GF_K2_ADVECT_WAVES
-#define GF_K2_ADVECT_WAVES 3   // 147 VGPRs, no spill; 4 waves spills and adds scratch traffic
+#define GF_K2_ADVECT_WAVES 3   // a floor: at 2048, P = 8 the kernel needs 122 VGPRs and runs at 4
 #endif
 template <typename T, int NX, int P, int G, int MODE, int NL>
 __global__ __launch_bounds__(G * (NX / P))
@@ -571,7 +582,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       const cplx<T>* f = a.in + (long)jl * a.fstride;
 #pragma unroll
       for (int q = 0; q < P; ++q) w[0][q] = active ? f[boff(kr, t + TPT * q, NY)] : czero<T>();
-      fft_forward<T, NY, P, 1>(w, L, t, tw);
+      fft_forward<T, NY, P, 1, (NY >= FreshFrom<T>::k3)>(w, L, t, tw);
       if (active) {
 #pragma unroll
         for (int q = 0; q < P; ++q) {
@@ -623,8 +634,8 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       A[0][q] = active ? at(fa, boff32(kr, y, NY)) : czero<T>();
       B[0][q] = active ? at(fb, boff32(kr, y, NY)) : czero<T>();
     }
-    fft_forward<T, NY, P, 1>(A, L, t, tw);
-    fft_forward<T, NY, P, 1>(B, L, t, tw);
+    fft_forward<T, NY, P, 1, (NY >= FreshFrom<T>::k3)>(A, L, t, tw);
+    fft_forward<T, NY, P, 1, (NY >= FreshFrom<T>::k3)>(B, L, t, tw);
     if (!active) continue;
     const long lo = (long)jl * g.lstride;
 #pragma unroll
