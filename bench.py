@@ -28,21 +28,27 @@ HBM_PEAK_GBS = 8000.0
 NX = 2048
 CFG = dict(nx=NX, Lx=2 * np.pi, nu=1e-21, nnu=4, mu=0.1, nmu=0, dt=0.005, kf=14.0, dkf=1.5, eps=0.1, seed=2)
 KERNELS = ("y_inverse_K1", "x_pass_K2", "y_forward_stage_K3")
-# kernel symbols of the FP64 2048^2 headline path, as rocprofv3 names them
-SYMBOLS = {"y_inverse_K1": "k1_kernel<double, 2048, 8, 1, 1, 0, 0>",
-           "x_pass_K2": "k2_kernel<double, 2048, 8, 1, 0, 1>",
-           "y_forward_stage_K3": "k3_kernel<double, 2048, 4, 1, 1>"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "twodns_2048_fp64_etdrk4_hbm_traffic.md")
 
 
-def measured_traffic():
+def symbols(sched):
+    """Kernel symbols of the FP64 2048^2 headline path, as rocprofv3 names them, under the radix
+    schedule sched = (P_K1, P_K2, P_K3) (one thread group per workgroup at this size)."""
+    p1, p2, p3 = sched
+    return {"y_inverse_K1": f"k1_kernel<double, 2048, {p1}, 1, 1, 0, 0>",
+            "x_pass_K2": f"k2_kernel<double, 2048, {p2}, 1, 0, 1>",
+            "y_forward_stage_K3": f"k3_kernel<double, 2048, {p3}, 1, 1>"}
+
+
+def measured_traffic(sched):
     """HBM bytes per launch of each headline kernel from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (scaled by the copy-kernel calibration); {} if it is absent."""
+    WRITE_SIZE summary (scaled by the copy-kernel calibration); a kernel is left out when the
+    summary was taken under another schedule, and {} is returned if the file is absent."""
     try:
         txt = open(TRAFFIC_FILE).read()
         block = txt.split("```json", 1)[1].split("```", 1)[0]
         tb = json.loads(block)["traffic_bytes"]
-        return {k: tb[s] for k, s in SYMBOLS.items() if s in tb}
+        return {k: tb[s] for k, s in symbols(sched).items() if s in tb}
     except (OSError, IndexError, KeyError, ValueError):
         return {}
 
@@ -365,7 +371,7 @@ def main():
     value = NX * NX * steps / elapsed
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
     dk = kern[dom]
-    traffic = measured_traffic()
+    traffic = measured_traffic(tuple(sched))
     for k in kern:
         kern[k]["hbm_traffic_bytes"] = traffic.get(k)
     S = 16.0 * (NX // 2 + 1) * NX

@@ -114,8 +114,9 @@ __device__ __forceinline__ void ka2_body(const AdjK2Args<T>& a, int blk, cplx<T>
   // forward x-transform of two real rows packed as r0 + i r1, split, stored as flux field f
   auto forward_rows = [&](int f, cplx<T> (&w)[1][P]) {
     fft_forward<T, NX, P, 1>(w, L, t, tw);
+    const MirrorSlots<NX, TPT> ms(t);
 #pragma unroll
-    for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = w[0][q];
+    for (int q = 0; q < P; ++q) L[ms.k(q)] = w[0][q];
     __syncthreads();
     if (active) {
       cplx<T>* o = a.out + (long)f * a.ostride;
@@ -123,7 +124,7 @@ __device__ __forceinline__ void ka2_body(const AdjK2Args<T>& a, int blk, cplx<T>
       for (int i = 0; i < MAXI; ++i) {
         const int kr = t + i * TPT;
         if (kr < a.kout) {
-          const cplx<T> Wk = L[pad(kr)], Wm = L[pad((NX - kr) & (NX - 1))];
+          const cplx<T> Wk = L[ms.k(i)], Wm = L[ms.m(i)];
           at(o, boff32(kr, y0, ny)) = split_row(Wk, Wm, 0);
           at(o, boff32(kr, y0 + 1, ny)) = split_row(Wk, Wm, 1);
         }

@@ -160,10 +160,38 @@ __device__ __forceinline__ void powers(cplx<T> (&w)[R]) {
 
 // LDS slot of point i: one pad slot after every 8 points.  Stockham writes at
 // stride R or NS (multiples of 8 or the first pass's 8j + r) and the unit-stride
-// reads then fall on distinct banks, and every slot of a thread's accesses is
-// its base slot plus a compile-time offset (one address register per pass).
-__device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
+// reads then fall on distinct banks.  pad(a + b) = pad(a) + pad(b) whenever
+// a mod 8 + b mod 8 < 8, which holds for a thread's base point and the power-of-two
+// multiples added to it (base + r NS with base mod NS < NS, t + q TPT with t < TPT), so
+// every slot of a thread's accesses is its base slot plus a compile-time offset: one
+// address register per pass and no per-access shift and mask.
+__host__ __device__ __forceinline__ constexpr int pad(int i) { return i + (i >> 3); }
 template <int N> struct LdsN { static constexpr int value = N + N / 8; };
+
+// Slots of kr = t + i TPT and of its mirror (N - kr) mod N in an N-point buffer, as used by
+// the split of a packed spectrum W = F(a + i b) of two real sequences (W[k] with W[N - k]).
+// For t > 0 the mirror is tm + (N - (i + 1) TPT) with tm = TPT - t; for t = 0 it is
+// (N - i TPT) mod N.  Both are a per-thread base plus a compile-time offset when TPT is a
+// multiple of 8 (pad is then additive over multiples of TPT); smaller TPT computes each slot.
+template <int N, int TPT>
+struct MirrorSlots {
+  int lo, m0, mb;
+  __host__ __device__ explicit MirrorSlots(int t) {
+    lo = pad(t);
+    if constexpr (TPT % 8 == 0) {
+      m0 = t != 0 ? pad(TPT - t) + pad(N - TPT) : 0;
+      mb = t != 0 ? pad(TPT - t) - pad(TPT) : 0;
+    } else {
+      m0 = t;   // the thread index, for the computed form
+      mb = 0;
+    }
+  }
+  __host__ __device__ int k(int i) const { return lo + pad(i * TPT); }
+  __host__ __device__ int m(int i) const {
+    if constexpr (TPT % 8 == 0) return i == 0 ? m0 : mb + pad(N - i * TPT);
+    else return pad((N - m0 - i * TPT) & (N - 1));
+  }
+};
 
 // The y-direction kernels interleave the two columns of a pair across adjacent lanes, each
 // column with its own N-point buffer.  Buffers N + N/8 slots apart start on the same LDS bank,
@@ -219,12 +247,14 @@ __device__ __forceinline__ void stockham(cplx<T> (&v)[C][P], cplx<T>* __restrict
       for (int b = 0; b < M; ++b) {
         const int j = t + b * TPT;
         const int base = (j / NS) * NS * R + (j % NS);
+        cplx<T>* wp = lds + pad(base);   // slot base + r NS is pad(base) + pad(r NS)
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[pad(base + r * NS)] = v[c][b + r * M];
+        for (int r = 0; r < R; ++r) wp[pad(r * NS)] = v[c][b + r * M];
       }
       __syncthreads();
+      const cplx<T>* rp = lds + pad(t);
 #pragma unroll
-      for (int q = 0; q < P; ++q) v[c][q] = lds[pad(t + TPT * q)];
+      for (int q = 0; q < P; ++q) v[c][q] = rp[pad(TPT * q)];
       __syncthreads();
     }
     stockham<T, N, P, C, PASS + 1, NS * R>(v, lds, t, tw);

@@ -302,7 +302,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
       }
       fft_forward<T, NX, P, 1, (NX >= FreshFrom<T>::k2)>(w, L, t, tw);
 #pragma unroll
-      for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = w[0][q];
+      for (int q = 0; q < P; ++q) L[pad(t) + pad(TPT * q)] = w[0][q];
       __syncthreads();
       if (active) {
         cplx<T>* o = a.out + f * a.ostride;
@@ -365,15 +365,16 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
     };
     auto forward_rows = [&](cplx<T>* o, cplx<T> (&w)[1][P]) {
       fft_forward<T, NX, P, 1, (NX >= FreshFrom<T>::k2)>(w, L, t, tw);             // of (row y0 + i row y0 + 1)
+      const MirrorSlots<NX, TPT> ms(t);
 #pragma unroll
-      for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = w[0][q];
+      for (int q = 0; q < P; ++q) L[ms.k(q)] = w[0][q];
       __syncthreads();
       if (active) {
 #pragma unroll
         for (int i = 0; i < MAXI; ++i) {
           const int kr = t + i * TPT;
           if (kr < a.kout) {
-            const cplx<T> Wk = L[pad(kr)], Wm = L[pad((NX - kr) & (NX - 1))];
+            const cplx<T> Wk = L[ms.k(i)], Wm = L[ms.m(i)];
             at(o, boff32(kr, y0, ny)) = split_row(Wk, Wm, 0);
             at(o, boff32(kr, y0 + 1, ny)) = split_row(Wk, Wm, 1);
           }
@@ -497,16 +498,17 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
       // split the packed spectra; rows handled one at a time through LDS
       cplx<T>* oA = a.out + (long)(2 * jl + 0) * a.ostride;
       cplx<T>* oB = a.out + (long)(2 * jl + 1) * a.ostride;
+      const MirrorSlots<NX, TPT> ms(t);
       for (int row = 0; row < 2; ++row) {
 #pragma unroll
-        for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = z[row][q];
+        for (int q = 0; q < P; ++q) L[ms.k(q)] = z[row][q];
         __syncthreads();
         cplx<T> Arow[MAXI], Brow[MAXI];
 #pragma unroll
         for (int i = 0; i < MAXI; ++i) {
           const int kr = t + i * TPT;
           if (kr < a.kout) {
-            const cplx<T> Wk = L[pad(kr)], Wm = L[pad((NX - kr) & (NX - 1))];
+            const cplx<T> Wk = L[ms.k(i)], Wm = L[ms.m(i)];
             Arow[i] = split_row(Wk, Wm, 0);
             Brow[i] = split_row(Wk, Wm, 1);
           }
@@ -515,13 +517,13 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
         if constexpr (FULL) {
           // this row's C = u Qx + v Qy spectrum from the packed (C0 + i C1) transform
 #pragma unroll
-          for (int q = 0; q < P; ++q) L[pad(t + TPT * q)] = z[2][q];
+          for (int q = 0; q < P; ++q) L[ms.k(q)] = z[2][q];
           __syncthreads();
 #pragma unroll
           for (int i = 0; i < MAXI; ++i) {
             const int kr = t + i * TPT;
             if (kr < a.kout) {
-              const cplx<T> Cv = split_row(L[pad(kr)], L[pad((NX - kr) & (NX - 1))], row);
+              const cplx<T> Cv = split_row(L[ms.k(i)], L[ms.m(i)], row);
               const T kv = krwave(g, kr);
               // P' = -i kr (u q)^ - (u Qx + v Qy)^
               Arow[i] = mk(kv * Arow[i].im - Cv.re, -kv * Arow[i].re - Cv.im);
