@@ -31,16 +31,18 @@ KERNELS = ("y_inverse_K1", "x_pass_K2", "y_forward_stage_K3")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "twodns_2048_fp64_etdrk4_hbm_traffic.md")
 
 
-def symbols(sched):
+def symbols(sched, splits):
     """Kernel symbols of the FP64 2048^2 headline path, as rocprofv3 names them, under the radix
-    schedule sched = (P_K1, P_K2, P_K3) (one thread group per workgroup at this size)."""
+    schedule sched = (P_K1, P_K2, P_K3) (one thread group per workgroup at this size) and the
+    workgroup splits (K3 with one column per workgroup carries a sixth template argument)."""
     p1, p2, p3 = sched
+    k3 = f"k3_kernel<double, 2048, {p3}, 1, 1, 1>" if splits[2] and 2048 // p3 >= 64 else f"k3_kernel<double, 2048, {p3}, 1, 1>"
     return {"y_inverse_K1": f"k1_kernel<double, 2048, {p1}, 1, 1, 0, 0>",
             "x_pass_K2": f"k2_kernel<double, 2048, {p2}, 1, 0, 1>",
-            "y_forward_stage_K3": f"k3_kernel<double, 2048, {p3}, 1, 1>"}
+            "y_forward_stage_K3": k3}
 
 
-def measured_traffic(sched):
+def measured_traffic(sched, splits):
     """HBM bytes per launch of each headline kernel from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (scaled by the copy-kernel calibration); a kernel is left out when the
     summary was taken under another schedule, and {} is returned if the file is absent."""
@@ -48,7 +50,7 @@ def measured_traffic(sched):
         txt = open(TRAFFIC_FILE).read()
         block = txt.split("```json", 1)[1].split("```", 1)[0]
         tb = json.loads(block)["traffic_bytes"]
-        return {k: tb[s] for k, s in symbols(sched).items() if s in tb}
+        return {k: tb[s] for k, s in symbols(sched, splits).items() if s in tb}
     except (OSError, IndexError, KeyError, ValueError):
         return {}
 
@@ -87,6 +89,7 @@ def gpu_run(gf, steps, warmup, T, dt=CFG["dt"]):
     lib.gf_timing(prob._ctx, 0)
     finite = bool(np.isfinite(prob.sol).all())
     kern["schedule"] = list(gf.schedule(prob))
+    kern["splits"] = list(gf.splits(prob))
     return elapsed, kern, finite
 
 
@@ -367,11 +370,12 @@ def main():
     steps, warmup = max(1, args.steps), max(0, args.warmup)
     elapsed, kern, finite = gpu_run(gf, steps, warmup, np.float64)
     sched = kern.pop("schedule")
+    splits = kern.pop("splits")
     ms_per_step = elapsed / steps * 1e3
     value = NX * NX * steps / elapsed
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
     dk = kern[dom]
-    traffic = measured_traffic(tuple(sched))
+    traffic = measured_traffic(tuple(sched), tuple(splits))
     for k in kern:
         kern[k]["hbm_traffic_bytes"] = traffic.get(k)
     S = 16.0 * (NX // 2 + 1) * NX
@@ -392,6 +396,7 @@ def main():
         "config": {"workload": "TwoDNavierStokes 2048x2048 L=2pi ETDRK4 dt=0.005 nu=1e-21 nnu=4 mu=0.1 "
                                "ring forcing kf=14 dkf=1.5 eps=0.1", "nx": NX, "ny": NX, "stepper": "ETDRK4",
                    "parallelism": "single GPU", "fft_points_per_thread_k1_k2_k3": sched,
+                   "workgroup_splits_k1_k2_k3": splits,
                    "schedule_source": "geophysicalflows.jl_amd/schedules.json (autotuned on MI355X)"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["achieved_GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dk["achieved_GBps"] / HBM_PEAK_GBS,
@@ -416,10 +421,11 @@ def main():
     if not args.no_fp32:
         e32, k32, f32 = gpu_run(gf, steps, warmup, np.float32, dt=0.001)
         sched32 = k32.pop("schedule")
+        splits32 = k32.pop("splits")
         s32 = 8.0 * (NX // 2 + 1) * NX
         out["fp32"] = {"workload": "the same forced 2048^2 problem in Float32 (dt = 0.001, which stays finite)",
                        "value": NX * NX * steps / e32, "ms_per_step": e32 / steps * 1e3, "state_finite": f32,
-                       "fft_points_per_thread_k1_k2_k3": sched32,
+                       "fft_points_per_thread_k1_k2_k3": sched32, "workgroup_splits_k1_k2_k3": splits32,
                        "step_model_frac_of_peak": 64 * s32 / (e32 / steps) / 1e9 / HBM_PEAK_GBS,
                        "kernels": {k: {"avg_us": v["avg_us"], "achieved_GBps": v["achieved_GBps"]} for k, v in k32.items()}}
     if not args.no_cpu:

@@ -111,9 +111,10 @@ def _load():
     lib.gf_set_schedule.argtypes = [vp, ip, ip, ip]
     lib.gf_get_schedule.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
     lib.gf_autotune.argtypes = [vp, ip, dp]
-    lib.gf_set_splits.argtypes = [vp, ip, ip]
+    lib.gf_set_splits.argtypes = [vp, ip, ip, ip]
     lib.gf_set_forcing_harmonic.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double]
-    lib.gf_get_splits.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.gf_get_splits.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_int)]
     lib.gf_schedules_save.argtypes = [ctypes.c_char_p]
     lib.gf_schedules_load.argtypes = [ctypes.c_char_p]
     lib.gf_schedules_clear.argtypes = []

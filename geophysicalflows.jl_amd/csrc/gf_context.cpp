@@ -280,7 +280,7 @@ struct Ctx final : CtxBase {
     Sched tuned;
     if (nl == 1 && sched_lookup(sched_key(), tuned)) {
       for (int k = 0; k < 3; ++k) sched[k] = tuned.p[k];
-      for (int k = 0; k < 2; ++k) split[k] = tuned.split[k];
+      for (int k = 0; k < 3; ++k) split[k] = tuned.split[k];
     }
     HIPCHECK(hipStreamSynchronize(stream));
   }
@@ -689,7 +689,7 @@ struct Ctx final : CtxBase {
     c.ccplx = ccplx; c.cl_layered = 0; c.dt = (T)d.dt; c.filter = filtered() ? filter.as<T>() : nullptr;
     c.tw = twy.as<C>();
     a.p = sched[0]; b.p = sched[1]; c.p = sched[2];
-    if (nl == 1) { a.fsplit = split[0]; b.hsplit = split[1]; }
+    if (nl == 1) { a.fsplit = split[0]; b.hsplit = split[1]; c.csplit = split[2]; }
     if (ab3()) {
       // the RHS history of the aliased band enters later steps: every AB3 step is computed in full
       c.ab3_start = stepno < 3;
@@ -733,16 +733,18 @@ struct Ctx final : CtxBase {
   // sched[k]: FFT points per thread of K1, K2, K3 (0: the built-in choice); single-layer models.
   int sched[3] = {0, 0, 0};
   // split[0]: K1 transforms one field per workgroup row; split[1]: K2 forms u q and v q in two
-  // workgroup rows. More, shorter workgroups for grids too small to fill the GPU.
-  int split[2] = {0, 0};
+  // workgroup rows: more, shorter workgroups for grids too small to fill the GPU. split[2]: K3
+  // takes one column per workgroup instead of a column pair, where a column fills a wave
+  // (k3_single_column): 683 workgroups at 4 per CU balance 256 CUs better than 342 at 2.
+  int split[3] = {0, 0, 0};
   void set_splits(const int* v) override {
-    if (nl != 1 && (v[0] || v[1])) throw GfError(GF_EINVAL, "workgroup splits are for single-layer problems");
+    if (nl != 1 && (v[0] || v[1] || v[2])) throw GfError(GF_EINVAL, "workgroup splits are for single-layer problems");
     bool changed = false;
-    for (int k = 0; k < 2; ++k) { changed |= split[k] != (v[k] != 0); split[k] = v[k] != 0; }
+    for (int k = 0; k < 3; ++k) { changed |= split[k] != (v[k] != 0); split[k] = v[k] != 0; }
     if (changed)
       for (auto& ge : gexec) if (ge) { HIPCHECK(hipStreamSynchronize(stream)); (void)hipGraphExecDestroy(ge); ge = nullptr; }
   }
-  void get_splits(int* v) const override { v[0] = split[0]; v[1] = split[1]; }
+  void get_splits(int* v) const override { for (int k = 0; k < 3; ++k) v[k] = split[k]; }
   SchedKey sched_key() const { return SchedKey{d.model, nl, d.nx, d.ny, d.dtype}; }
   int sched_len(int k) const { return k == 1 ? d.nx : d.ny; }   // K2 transforms rows, K1/K3 columns
   void set_schedule(const int* p) override {
@@ -759,15 +761,16 @@ struct Ctx final : CtxBase {
     for (int k = 0; k < 3; ++k) p[k] = sched[k] ? sched[k] : (nl == 1 ? sched_default(sched_len(k)) : (d.ny >= 32 && k != 1 ? 4 : sched_default(sched_len(k))));
   }
   // Times whole steps under every offered schedule -- P in {4, 8, 16} for each of K1, K2, K3
-  // and the two workgroup splits on or off, up to 108 combinations -- in two passes (the second
+  // and the three workgroup splits on or off, up to 216 combinations -- in two passes (the second
   // in reverse order, best of the two), keeps the fastest and records it for this (model, size,
   // dtype) so that later contexts start with it. The state, its stepper history and the clock
-  // are restored afterwards. us (nullable) receives 108 step times in microseconds, combination
-  // c = (((i1 3 + i2) 3 + i3) 2 + split_K1) 2 + split_K2 with i = 0, 1, 2 for P = 4, 8, 16,
-  // NaN where a combination is not offered.
+  // are restored afterwards. us (nullable) receives 216 step times in microseconds, combination
+  // c = ((((i1 3 + i2) 3 + i3) 2 + split_K1) 2 + split_K2) 2 + split_K3 with i = 0, 1, 2 for
+  // P = 4, 8, 16, NaN where a combination is not offered (split_K3 = 1 is offered where
+  // k3_single_column holds for P_K3).
   void autotune(int steps, double* us) override {
     if (steps < 1) throw GfError(GF_EINVAL, "gf_autotune: steps must be >= 1");
-    constexpr int NC = 108;
+    constexpr int NC = 216;
     if (us) for (int j = 0; j < NC; ++j) us[j] = std::nan("");
     if (nl != 1) return;   // MultiLayerQG keeps its fixed schedule
     HIPCHECK(hipStreamSynchronize(stream));
@@ -780,27 +783,28 @@ struct Ctx final : CtxBase {
     const int64_t step0 = stepno;
     const bool pv0 = prevsol_valid;
     const int old[3] = {sched[0], sched[1], sched[2]};
-    const int old_split[2] = {split[0], split[1]};
+    const int old_split[3] = {split[0], split[1], split[2]};
     static constexpr int PV[3] = {4, 8, 16};
     auto decode = [](int c, int* q, int* v) {
-      v[1] = c & 1; v[0] = (c >> 1) & 1;
-      int r = c >> 2;
+      v[2] = c & 1; v[1] = (c >> 1) & 1; v[0] = (c >> 2) & 1;
+      int r = c >> 3;
       q[2] = PV[r % 3]; r /= 3;
       q[1] = PV[r % 3]; r /= 3;
       q[0] = PV[r % 3];
     };
     auto apply = [&](int c) {
-      int q[3], v[2];
+      int q[3], v[3];
       decode(c, q, v);
       set_schedule(q);
       set_splits(v);
     };
     std::vector<int> combos;
     for (int c = 0; c < NC; ++c) {
-      int q[3], v[2];
+      int q[3], v[3];
       decode(c, q, v);
       bool ok = true;
       for (int k = 0; k < 3; ++k) ok &= sched_candidate(sched_len(k), q[k]);
+      ok &= !v[2] || k3_single_column(sched_len(2), q[2]);
       if (ok) combos.push_back(c);
     }
     double best_t[NC];
@@ -845,7 +849,7 @@ struct Ctx final : CtxBase {
     apply(bc);
     Sched e;
     for (int k = 0; k < 3; ++k) e.p[k] = sched[k];
-    for (int k = 0; k < 2; ++k) e.split[k] = split[k];
+    for (int k = 0; k < 3; ++k) e.split[k] = split[k];
     e.step_us = best_t[bc];
     sched_store(sched_key(), e);
   }
@@ -1418,6 +1422,7 @@ struct BatchCtx final : CtxBase {
           m.make_args(ins[k], epis[k], full != 0, h[i].k1, h[i].k2, h[i].k3);
           h[i].k1.fsplit = 0;   // the batched launches are one-dimensional
           h[i].k2.hsplit = 0;
+          h[i].k3.csplit = 0;
         }
         items[k][full].alloc(sizeof(BatchItem<T>) * n);
         HIPCHECK(hipMemcpy(items[k][full].p, h.data(), sizeof(BatchItem<T>) * n, hipMemcpyHostToDevice));
@@ -1788,16 +1793,17 @@ int gf_get_schedule(gf_ctx* c, int* p_k1, int* p_k2, int* p_k3) {
   }
   return rc;
 }
-int gf_set_splits(gf_ctx* c, int k1_fields, int k2_halves) {
-  const int v[2] = {k1_fields, k2_halves};
+int gf_set_splits(gf_ctx* c, int k1_fields, int k2_halves, int k3_columns) {
+  const int v[3] = {k1_fields, k2_halves, k3_columns};
   return guarded(c, [&] { c->impl->set_splits(v); });
 }
-int gf_get_splits(gf_ctx* c, int* k1_fields, int* k2_halves) {
-  int v[2] = {0, 0};
+int gf_get_splits(gf_ctx* c, int* k1_fields, int* k2_halves, int* k3_columns) {
+  int v[3] = {0, 0, 0};
   const int rc = guarded(c, [&] { c->impl->get_splits(v); });
   if (rc == GF_OK) {
     if (k1_fields) *k1_fields = v[0];
     if (k2_halves) *k2_halves = v[1];
+    if (k3_columns) *k3_columns = v[2];
   }
   return rc;
 }

@@ -127,6 +127,7 @@ struct K3Args {
   int zero_mean;            // E_STORE: set mode (0,0) to zero
   int nomask;               // MLQG linearised equations: no dealiasing of the state
   int lsplit;               // multi-layer: one layer per workgroup row (blockIdx.y)
+  int csplit;               // single-layer: one column per workgroup where k3_single_column allows
   const cplx<T>* psi_in;    // multi-layer, nullable: the bottom layer's psih written by K1
   int ab3_start;            // AB3: forward Euler start-up step
   // adjoint tape (ETDRK4): nsave receives each stage's N, sol_out the new state of
@@ -225,6 +226,9 @@ void launch_counter(int64_t* c, int64_t v, int add, hipStream_t s);
 inline bool sched_candidate(int n, int p) { return p == 4 ? n >= 16 : (p == 8 ? n >= 32 : (p == 16 ? n >= 1024 : false)); }
 // The built-in choice (PY in the kernels): P = 4 below 32 points, else 8.
 inline int sched_default(int n) { return n < 32 ? 4 : 8; }
+// K3 runs one column per workgroup (csplit) when a column's n/p threads fill at least one wave;
+// otherwise csplit is ignored and the column pairs share workgroups.
+constexpr bool k3_single_column(int n, int p) { return n / p >= 64; }
 bool size_supported(int n);
 
 }  // namespace gf

@@ -564,17 +564,33 @@ void k2_kernel(const K2Args<T> a) {
 // ---------------------------------------------------------------------------
 // K3: flux -> spectral N + stepper stage epilogue, one column / group
 // ---------------------------------------------------------------------------
-// Body of K3 for block blk of column pairs.
-template <typename T, int NY, int P, int G, int NL>
+// Column of single-column workgroup b: the two columns of pair p go to workgroups
+// 16 (p / 8) + (p mod 8) and that + 8, which the dispatcher hands to the same XCD (workgroup b
+// runs on XCD b mod 8), so the second read of each 32-byte piece of the blocked layout (one row
+// of the pair) hits in that XCD's L2.
+__device__ __forceinline__ int single_column(int b) { return 2 * (8 * (b / 16) + b % 8) + (b / 8) % 2; }
+
+// Body of K3 for block blk of column pairs (CPW = 2, G pairs per workgroup) or, with CPW = 1,
+// for the single column single_column(blk) (G = 1).
+template <typename T, int NY, int P, int G, int NL, int CPW = 2>
 __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __restrict__ lds,
                                         const cplx<T>* __restrict__ tw) {
   constexpr int TPT = NY / P;
-  const YMap<TPT> m(threadIdx.x);
-  const int t = m.t;
-  const int kr = 2 * (blk * G + m.slot) + m.c;
+  static_assert(CPW == 2 || G == 1, "single-column workgroups hold one column");
+  int t, kr;
+  cplx<T>* L;
+  if constexpr (CPW == 2) {
+    const YMap<TPT> m(threadIdx.x);
+    t = m.t;
+    kr = 2 * (blk * G + m.slot) + m.c;
+    L = col_buffer<T, NY>(lds, m.slot, m.c);
+  } else {
+    t = threadIdx.x;
+    kr = single_column(blk);
+    L = lds;
+  }
   const Geo<T>& g = a.g;
   const bool active = kr < a.ncols && kr < g.nkr;
-  cplx<T>* L = col_buffer<T, NY>(lds, m.slot, m.c);
   const T kv = krwave(g, kr);
 
   if (a.epi == E_STORE) {
@@ -784,14 +800,14 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
   }
 }
 
-template <typename T, int NY, int P, int G, int NL>
-__global__ __launch_bounds__(2 * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K3_WAVES)))
+template <typename T, int NY, int P, int G, int NL, int CPW = 2>
+__global__ __launch_bounds__(CPW * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K3_WAVES)))
 void k3_kernel(const K3Args<T> a) {
-  __shared__ cplx<T> lds[G * PairLds<T, NY>::value];
+  __shared__ cplx<T> lds[CPW == 2 ? G * PairLds<T, NY>::value : LdsN<NY>::value];
   __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
   load_twiddles<T, NY, P>(tw, a.tw);
   __syncthreads();
-  k3_body<T, NY, P, G, NL>(a, blockIdx.x, lds, tw);
+  k3_body<T, NY, P, G, NL, CPW>(a, blockIdx.x, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -900,6 +916,17 @@ void k2_n(const K2Args<T>& a, hipStream_t s) {
 template <typename T, int N, int NL, int P>
 void k3_np(const K3Args<T>& a, hipStream_t s) {
   constexpr int G = GY<N, P>::G;
+  if constexpr (NL == 1 && k3_single_column(N, P)) {
+    static_assert(G == 1, "a single column fills a wave");
+    if (a.csplit) {
+      // one column per workgroup, a pair's columns 8 workgroups apart (single_column): at 2048^2
+      // 683 workgroups at 4 per CU instead of 342 column pairs at 2 per CU on 256 CUs
+      const int npairs = (a.ncols + 1) / 2;
+      const int nblk = 16 * ((npairs + 7) / 8);
+      if (npairs > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1>), dim3(nblk), dim3(N / P), 0, s, a);
+      return;
+    }
+  }
   const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
   if (nblk > 0)
     hipLaunchKernelGGL((k3_kernel<T, N, P, G, NL>), dim3(nblk, (NL > 1 && a.lsplit) ? NL : 1), dim3(2 * G * (N / P)), 0, s, a);

@@ -1,8 +1,8 @@
 // The schedule table and its JSON form:
 //   {"format": "gf-schedules-1", "entries": [
-//     {"model": 0, "nlayers": 1, "nx": 2048, "ny": 2048, "dtype": 0, "p": [8, 8, 4], "split": [1, 0],
-//      "step_us": 441.2}, ...]}
-// ("split" may be absent: no splits).
+//     {"model": 0, "nlayers": 1, "nx": 2048, "ny": 2048, "dtype": 0, "p": [8, 8, 8], "split": [1, 0, 1],
+//      "step_us": 381.9}, ...]}
+// ("split" may be absent: no splits; a 2-entry "split" leaves K3's column split off).
 #include "gf_schedule.hpp"
 
 #include <cctype>
@@ -131,7 +131,8 @@ void sched_save(const std::string& path) {
       const Sched& e = kv.second;
       o << (first ? "\n" : ",\n") << "  {\"model\": " << k[0] << ", \"nlayers\": " << k[1] << ", \"nx\": " << k[2]
         << ", \"ny\": " << k[3] << ", \"dtype\": " << k[4] << ", \"p\": [" << e.p[0] << ", " << e.p[1] << ", "
-        << e.p[2] << "], \"split\": [" << e.split[0] << ", " << e.split[1] << "], \"step_us\": ";
+        << e.p[2] << "], \"split\": [" << e.split[0] << ", " << e.split[1] << ", " << e.split[2]
+        << "], \"step_us\": ";
       std::snprintf(buf, sizeof buf, "%.3f", e.step_us);
       o << buf << "}";
       first = false;
@@ -170,12 +171,13 @@ int sched_load(const std::string& path) {
     for (int j = 0; j < 3; ++j) s.p[j] = as_int(&p->arr[j], "p");
     const Value* sp = e.get("split");
     if (sp) {
-      if (sp->kind != Value::ARR || sp->arr.size() != 2) throw std::runtime_error("schedule table: 'split' must hold 2 integers");
-      for (int j = 0; j < 2; ++j) {
+      if (sp->kind != Value::ARR || (sp->arr.size() != 2 && sp->arr.size() != 3))
+        throw std::runtime_error("schedule table: 'split' must hold 2 or 3 integers");
+      for (int j = 0; j < (int)sp->arr.size(); ++j) {
         s.split[j] = as_int(&sp->arr[j], "split");
         if (s.split[j] != 0 && s.split[j] != 1) throw std::runtime_error("schedule table: 'split' entries are 0 or 1");
       }
-      if ((s.split[0] || s.split[1]) && k[1] != 1)
+      if ((s.split[0] || s.split[1] || s.split[2]) && k[1] != 1)
         throw std::runtime_error("schedule table: workgroup splits are for single-layer problems");
     }
     const Value* us = e.get("step_us");

@@ -10,7 +10,8 @@ using SchedKey = std::array<int, 5>;   // model, nlayers, nx, ny, dtype
 
 struct Sched {
   int p[3] = {0, 0, 0};         // FFT points per thread of K1, K2, K3 (0: built-in choice)
-  int split[2] = {0, 0};        // K1 one field per workgroup row, K2 u q / v q in two rows
+  int split[3] = {0, 0, 0};     // K1 one field per workgroup row, K2 u q / v q in two rows,
+                                // K3 one column per workgroup
   double step_us = 0;           // measured time of one step under this schedule, microseconds
 };
 

@@ -258,15 +258,16 @@ def schedule(prob):
     return tuple(x.value for x in p)
 
 
-def set_splits(prob, k1_fields=0, k2_halves=0):
+def set_splits(prob, k1_fields=0, k2_halves=0, k3_columns=0):
     """Workgroup splits of a single-layer problem: K1 one field per workgroup row, K2 u q and v q
-    in two rows (more, shorter workgroups for small grids).  No counterpart in the reference."""
-    check(lib.gf_set_splits(prob._ctx, int(k1_fields), int(k2_halves)), prob._ctx)
+    in two rows (more, shorter workgroups for small grids), K3 one column per workgroup instead
+    of a column pair (where a column fills a wave).  No counterpart in the reference."""
+    check(lib.gf_set_splits(prob._ctx, int(k1_fields), int(k2_halves), int(k3_columns)), prob._ctx)
 
 
 def splits(prob):
-    """The workgroup splits in use, ``(k1_fields, k2_halves)``."""
-    v = [ctypes.c_int() for _ in range(2)]
+    """The workgroup splits in use, ``(k1_fields, k2_halves, k3_columns)``."""
+    v = [ctypes.c_int() for _ in range(3)]
     check(lib.gf_get_splits(prob._ctx, *[ctypes.byref(x) for x in v]), prob._ctx)
     return tuple(x.value for x in v)
 
@@ -274,16 +275,16 @@ def splits(prob):
 def autotune(prob, steps=20):
     """Time whole steps of ``prob`` under every offered schedule (radix and splits; state and
     clock restored), apply the fastest and record it for later problems of the same model, size
-    and dtype.  Returns the chosen ``schedule + splits`` 5-tuple and the step time in
+    and dtype.  Returns the chosen ``schedule + splits`` 6-tuple and the step time in
     microseconds of each combination tried, keyed the same way."""
-    us = (ctypes.c_double * 108)()
+    us = (ctypes.c_double * 216)()
     check(lib.gf_autotune(prob._ctx, int(steps), us), prob._ctx)
     times = {}
     pv = (4, 8, 16)
-    for c in range(108):
+    for c in range(216):
         if math.isfinite(us[c]):
-            r = c >> 2
-            key = (pv[r // 9], pv[(r // 3) % 3], pv[r % 3], (c >> 1) & 1, c & 1)
+            r = c >> 3
+            key = (pv[r // 9], pv[(r // 3) % 3], pv[r % 3], (c >> 2) & 1, (c >> 1) & 1, c & 1)
             times[key] = us[c]
     return schedule(prob) + splits(prob), times
 

@@ -43,8 +43,8 @@ def main(argv=None):
         (m.set_ζ if mod == "TwoDNavierStokes" else m.set_q)(p, f)
         gf.stepforward(p, 3)
         chosen, us = gf.autotune(p, steps=args.steps)
-        base = us.get((8, 8, 8, 0, 0)) or us.get((4, 4, 4, 0, 0))
-        best_plain = min(v for k, v in us.items() if k[3:] == (0, 0))
+        base = us.get((8, 8, 8, 0, 0, 0)) or us.get((4, 4, 4, 0, 0, 0))
+        best_plain = min(v for k, v in us.items() if k[3:] == (0, 0, 0))
         print(f"{mod:17s} {n:5d}^2 {np.dtype(T).name:8s} P {chosen[:3]} splits {chosen[3:]}  step {us[chosen]:.2f} us"
               f"  (built-in {base:.2f} us, best without splits {best_plain:.2f} us)", flush=True)
     gf.save_schedules(args.out)

@@ -148,20 +148,22 @@ int gf_set_schedule(gf_ctx* ctx, int p_k1, int p_k2, int p_k3);
 int gf_get_schedule(gf_ctx* ctx, int* p_k1, int* p_k2, int* p_k3);
 
 /* Workgroup splits for single-layer problems (0 off, 1 on), part of the schedule:
- *   k1_fields: K1 gives each of u, v, q its own row of workgroups;
- *   k2_halves: K2 forms u q and v q in two rows of workgroups, each transforming q itself.
- * More and shorter workgroups, for grids too small to fill the GPU; the results are the same. */
-int gf_set_splits(gf_ctx* ctx, int k1_fields, int k2_halves);
-int gf_get_splits(gf_ctx* ctx, int* k1_fields, int* k2_halves);
+ *   k1_fields:  K1 gives each of u, v, q its own row of workgroups;
+ *   k2_halves:  K2 forms u q and v q in two rows of workgroups, each transforming q itself;
+ *   k3_columns: K3 gives each spectral column its own workgroup instead of a column pair, where
+ *               a column's ny / P_K3 threads fill a 64-lane wave (ignored elsewhere).
+ * Workgroups are regrouped only: the results are the same bit for bit. */
+int gf_set_splits(gf_ctx* ctx, int k1_fields, int k2_halves, int k3_columns);
+int gf_get_splits(gf_ctx* ctx, int* k1_fields, int* k2_halves, int* k3_columns);
 
 /* The autotuner. It times `steps` whole steps under every offered schedule on this problem:
- * P in {4, 8, 16} for each of K1, K2, K3, with each workgroup split on or off, up to 108
+ * P in {4, 8, 16} for each of K1, K2, K3, with each workgroup split on or off, up to 216
  * combinations. It runs two passes, the second in reverse order, keeping each combination's
  * best, then applies the fastest. The state, its stepper history and the clock are restored
  * afterwards. The choice is recorded for (model, nlayers, nx, ny, dtype): contexts created later
- * in the process start with it. us (nullable) receives 108 step times in microseconds, indexed
- * (((i1 * 3 + i2) * 3 + i3) * 2 + split_K1) * 2 + split_K2, where i = 0, 1, 2 for P = 4, 8,
- * 16; NaN where a combination is not offered. */
+ * in the process start with it. us (nullable) receives 216 step times in microseconds, indexed
+ * ((((i1 * 3 + i2) * 3 + i3) * 2 + split_K1) * 2 + split_K2) * 2 + split_K3, where i = 0, 1, 2
+ * for P = 4, 8, 16; NaN where a combination is not offered. */
 int gf_autotune(gf_ctx* ctx, int steps, double* us);
 
 /* The process-wide table of tuned schedules as JSON ("gf-schedules-1": one entry per (model,

@@ -103,12 +103,14 @@ def test_concurrent_gradients_equal_sequential_ones(gf):
         assert np.array_equal(a[1], b[1])
 
 
-def test_workgroup_splits_give_the_same_gradient(gf):
+@pytest.mark.parametrize("n", [128, 512])
+def test_workgroup_splits_give_the_same_gradient(gf, n):
     # the splits only regroup the same arithmetic: forward tape and adjoint agree bit for bit
-    prob, z0, target = _setup(gf, 128, 1e-3, 80)
-    gf.set_splits(prob, 0, 0)
+    # (at 512 K3's column split applies: 512 / 8 threads per column)
+    prob, z0, target = _setup(gf, n, 1e-3, 80)
+    gf.set_splits(prob, 0, 0, 0)
     a = gf.TwoDNavierStokes.adjoint_gradient(prob, target, 9, 4)
-    gf.set_splits(prob, 1, 1)
+    gf.set_splits(prob, 1, 1, 1)
     b = gf.TwoDNavierStokes.adjoint_gradient(prob, target, 9, 4)
     assert a[0] == b[0] and a[2] == b[2]
     assert np.array_equal(a[1], b[1])

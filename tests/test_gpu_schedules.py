@@ -20,7 +20,7 @@ def _spec(rng, n):
 
 @pytest.mark.parametrize("n", [64, 256])
 @pytest.mark.parametrize("sched", SCHEDULES)
-@pytest.mark.parametrize("split", [(0, 0), (1, 1), (1, 0)])
+@pytest.mark.parametrize("split", [(0, 0, 0), (1, 1, 1), (1, 0, 0), (0, 0, 1)])
 def test_twodns_every_schedule_matches_oracle(gf, n, sched, split):
     rng = np.random.default_rng(80 + n)
     s = _spec(rng, n)
@@ -54,7 +54,7 @@ def test_slqg_schedules_topography_and_varying_U(gf, sched):
     for kw in (dict(U=0.3, eta=eta), dict(U=Uy, eta=eta)):
         p = gf.SingleLayerQG.Problem(gf.GPU(), nx=n, β=2.0, μ=0.01, dt=0.002, stepper="FilteredRK4", **kw)
         gf.set_schedule(p, *sched)
-        gf.set_splits(p, 1, 1)
+        gf.set_splits(p, 1, 1, 1)
         rk = dict(kw)
         r = osl.Problem(nx=n, beta=2.0, mu=0.01, dt=0.002, stepper="FilteredRK4", **rk)
         s = _spec(np.random.default_rng(90), n)
@@ -62,6 +62,29 @@ def test_slqg_schedules_topography_and_varying_U(gf, sched):
         Nr, sr = np.zeros_like(s), s.copy()
         osl.calcN(Nr, sr, 0.0, r.clock, r.vars, r.params, r.grid)
         assert rel(N, Nr) < 1e-13, kw.keys()
+
+
+@pytest.mark.parametrize("n,dtype", [(512, np.float64), (1024, np.float64), (2048, np.float64), (2048, np.float32)])
+def test_k3_column_split_is_bitwise_the_pair_layout(gf, n, dtype):
+    # one column per workgroup regroups the same arithmetic: calcN and steps agree bit for bit
+    rng = np.random.default_rng(95)
+    T = gf.TwoDNavierStokes
+    kw = dict(nx=n, ν=1e-8, nν=2, μ=0.1, dt=0.001, stepper="ETDRK4", T=dtype,
+              calcF=gf.RingForcing(10.0, 1.5, 0.1, seed=7))
+    z = 0.1 * rng.standard_normal((n, n))
+    out = {}
+    for p3 in (4, 8):
+        for cs in (0, 1):
+            p = T.Problem(gf.GPU(), **kw)
+            gf.set_schedule(p, 8, 8, p3)
+            gf.set_splits(p, 1, 0, cs)
+            assert gf.splits(p) == (1, 0, cs)
+            T.set_ζ(p, z)
+            N, _ = p.calcN(p.sol)
+            gf.stepforward(p, 3)
+            out[p3, cs] = (N, p.sol)
+        assert np.array_equal(out[p3, 0][0], out[p3, 1][0]) and np.array_equal(out[p3, 0][1], out[p3, 1][1])
+        assert np.isfinite(out[p3, 1][1]).all()
 
 
 def test_schedule_not_offered_is_rejected(gf):
@@ -96,7 +119,9 @@ def test_autotune_restores_state_and_records_the_choice(gf, tmp_path):
     assert np.array_equal(p.sol, before) and p.clock.t == t0 and p.clock.step == k0
     assert T.energy_work(p) == w0
     assert all(c in (4, 8) for c in chosen[:3]) and gf.schedule(p) + gf.splits(p) == chosen   # no 16 at 256
-    assert len(us) == 32 and min(us.values()) > 0 and us[chosen] == min(us.values())
+    # 8 P triples x 4 (K1, K2) splits, and the K3 column split where 256 / P_K3 >= 64 (P_K3 = 4)
+    assert len(us) == 48 and min(us.values()) > 0 and us[chosen] == min(us.values())
+    assert all(k[5] == 0 or k[2] == 4 for k in us)
     gf.set_schedule(q, *chosen[:3])
     gf.set_splits(q, *chosen[3:])
     gf.stepforward(p, 7)                       # the history survived: same steps as the twin
@@ -110,7 +135,7 @@ def test_autotune_restores_state_and_records_the_choice(gf, tmp_path):
     gf.save_schedules(path)
     gf._lib.lib.gf_schedules_clear()
     r = T.Problem(gf.GPU(), **kw)
-    assert gf.schedule(r) == (8, 8, 8) and gf.splits(r) == (0, 0)
+    assert gf.schedule(r) == (8, 8, 8) and gf.splits(r) == (0, 0, 0)
     assert gf.load_schedules(path) >= 1
     r = T.Problem(gf.GPU(), **kw)
     assert gf.schedule(r) + gf.splits(r) == chosen
