@@ -34,9 +34,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "twodns_2048_fp64_etdrk4_hbm_traff
 def symbols(sched, splits):
     """Kernel symbols of the FP64 2048^2 headline path, as rocprofv3 names them, under the radix
     schedule sched = (P_K1, P_K2, P_K3) (one thread group per workgroup at this size) and the
-    workgroup splits (K3 with one column per workgroup carries a sixth template argument)."""
+    workgroup splits (K3's sixth template argument is its columns per workgroup, 1 or 2)."""
     p1, p2, p3 = sched
-    k3 = f"k3_kernel<double, 2048, {p3}, 1, 1, 1>" if splits[2] and 2048 // p3 >= 64 else f"k3_kernel<double, 2048, {p3}, 1, 1>"
+    cpw = 1 if splits[2] and 2048 // p3 >= 64 else 2
+    k3 = f"k3_kernel<double, 2048, {p3}, 1, 1, {cpw}>"
     return {"y_inverse_K1": f"k1_kernel<double, 2048, {p1}, 1, 1, 0, 0>",
             "x_pass_K2": f"k2_kernel<double, 2048, {p2}, 1, 0, 1>",
             "y_forward_stage_K3": k3}
