@@ -230,6 +230,8 @@ struct Ctx final : CtxBase {
   // device buffers
   DevBuf sol, s1, s2, n1, n23, racc, nout, mixed, flux, twx, twy, phys, jl, cinp;
   DevBuf coef[6], filter, eta, etah, etax, etay, Uy, Qy1, sinv, Fh, sqrt_spec;
+  DevBuf noise;   // ring forcing: K3's box of this step's forcing terms
+  int nbl = 0;    // rows of that box (ring_box_row)
   DevBuf prevsol, diag_part;
   DevBuf psib;               // multi-layer with drag: the bottom layer's psih, K1 -> K3
   bool prevsol_valid = false;
@@ -650,6 +652,10 @@ struct Ctx final : CtxBase {
         }
       }
     upload_blocked(sqrt_spec, ss);
+    if (fkx >= 0) {
+      nbl = std::min(ny, 2 * fky + 1);
+      noise.alloc(sizeof(C) * (size_t)(fkx + 1) * nbl);
+    }
   }
 
   // ---------------- hot path ----------------
@@ -674,6 +680,7 @@ struct Ctx final : CtxBase {
     c.fscale = cur_fscale;
     c.sqrt_spec = (d.forcing == GF_FORCING_RING && fkx >= 0) ? sqrt_spec.as<T>() : nullptr;
     c.seed = d.seed; c.fstep = stepno; c.sqrt_dt = (T)std::sqrt(d.dt); c.fkx = fkx; c.fky = fky;
+    c.noise = noise.as<C>(); c.nbl = nbl;
     c.instance = d.instance;
     if (d.model == GF_MODEL_MLQG && d.mu != 0) { c.mu_drag = (T)d.mu; c.drag_src = in; c.sinv = sinv.as<T>(); }
     if (nl > 1) {

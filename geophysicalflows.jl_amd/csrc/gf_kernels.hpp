@@ -113,6 +113,9 @@ struct K3Args {
   T fscale;                 // factor on Fh: 1, or cos(omega t + phase) at the stage time
   const T* sqrt_spec;       // ring forcing amplitude (blocked real), nullable
   uint64_t seed; int64_t fstep; T sqrt_dt; int fkx, fky; int instance;
+  // ring forcing: (fkx + 1) x nbl box of this step's forcing terms, written by K3 itself
+  // before its transforms and read back in the stage epilogue (ring_box_row)
+  cplx<T>* noise; int nbl;
   const int64_t* fstep_dev; // nullable: the forcing step is read here (steps replayed from a graph)
   // MLQG bottom drag mu Krsq psi_n
   T mu_drag; const cplx<T>* drag_src;  // calcN input state (for psi_n)
@@ -226,6 +229,9 @@ void launch_counter(int64_t* c, int64_t v, int add, hipStream_t s);
 inline bool sched_candidate(int n, int p) { return p == 4 ? n >= 16 : (p == 8 ? n >= 32 : (p == 16 ? n >= 1024 : false)); }
 // The built-in choice (PY in the kernels): P = 4 below 32 points, else 8.
 inline int sched_default(int n) { return n < 32 ? 4 : 8; }
+// Row of spectral row l in a ring-forcing box of nbl = min(ny, 2 fky + 1) rows holding
+// l <= fky and l >= ny - fky.
+__host__ __device__ inline int ring_box_row(int l, int ny, int fky, int nbl) { return l <= fky ? l : l - ny + nbl; }
 // K3 runs one column per workgroup (csplit) when a column's n/p threads fill at least one wave;
 // otherwise csplit is ignored and the column pairs share workgroups.
 constexpr bool k3_single_column(int n, int p) { return n / p >= 64; }
