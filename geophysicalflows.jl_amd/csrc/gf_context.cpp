@@ -867,8 +867,8 @@ struct Ctx final : CtxBase {
   // one graph launch per step instead of 3 per stage. The ring forcing's step number, the
   // one argument that changes, is read from step_dev, which the graph advances itself.
   static constexpr int GRAPH_BLOCK = 8;   // interior steps in the larger graph
-  // Off unless gf_use_graphs(ctx, 1): for one problem the replay gains at most a few percent
-  // (the steps are bound by kernel time, not by launches).
+  // Off unless gf_use_graphs(ctx, 1): for one problem the replay gains at most 3% at small sizes
+  // and loses about 1% at 2048^2 (the steps are bound by kernel time, not by launches).
   bool graphs = false;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};   // one interior step, GRAPH_BLOCK interior steps
   DevBuf step_dev;
