@@ -34,12 +34,17 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "twodns_2048_fp64_etdrk4_hbm_traff
 def symbols(sched, splits):
     """Kernel symbols of the FP64 2048^2 headline path, as rocprofv3 names them, under the radix
     schedule sched = (P_K1, P_K2, P_K3) (one thread group per workgroup at this size) and the
-    workgroup splits (K3's sixth template argument is its columns per workgroup, 1 or 2)."""
+    workgroup splits (K3's sixth template argument is its columns per workgroup, 1 or 2), as a
+    list per kernel class."""
     p1, p2, p3 = sched
     cpw = 1 if splits[2] and 2048 // p3 >= 64 else 2
-    k3 = f"k3_kernel<double, 2048, {p3}, 1, 1, {cpw}>"
-    return {"y_inverse_K1": f"k1_kernel<double, 2048, {p1}, 1, 1, 0, 0>",
-            "x_pass_K2": f"k2_kernel<double, 2048, {p2}, 1, 0, 1>",
+    if cpw == 1 and p3 == 8:
+        # one kernel per ETDRK4 stage (the seventh argument is the stage's epilogue code)
+        k3 = [f"k3_kernel<double, 2048, 8, 1, 1, 1, {e}>" for e in (2, 3, 4, 5)]
+    else:
+        k3 = [f"k3_kernel<double, 2048, {p3}, 1, 1, {cpw}, -1>"]
+    return {"y_inverse_K1": [f"k1_kernel<double, 2048, {p1}, 1, 1, 0, 0>"],
+            "x_pass_K2": [f"k2_kernel<double, 2048, {p2}, 1, 0, 1>"],
             "y_forward_stage_K3": k3}
 
 
@@ -51,7 +56,10 @@ def measured_traffic(sched, splits):
         txt = open(TRAFFIC_FILE).read()
         block = txt.split("```json", 1)[1].split("```", 1)[0]
         tb = json.loads(block)["traffic_bytes"]
-        return {k: tb[s] for k, s in symbols(sched, splits).items() if s in tb}
+        # a kernel class launched as several kernels (K3's stages, once each per step): the
+        # mean over them is the class's bytes per launch
+        return {k: sum(tb[x] for x in syms) / len(syms) for k, syms in symbols(sched, splits).items()
+                if all(x in tb for x in syms)}
     except (OSError, IndexError, KeyError, ValueError):
         return {}
 

@@ -18,6 +18,7 @@
 // This header holds the templates; gf_inst_*.hip instantiate them per size.
 #pragma once
 #include "gf_kernels.hpp"
+#include <type_traits>
 
 namespace gf {
 
@@ -571,8 +572,9 @@ void k2_kernel(const K2Args<T> a) {
 __device__ __forceinline__ int single_column(int b) { return 2 * (8 * (b / 16) + b % 8) + (b / 8) % 2; }
 
 // Body of K3 for block blk of column pairs (CPW = 2, G pairs per workgroup) or, with CPW = 1,
-// for the single column single_column(blk) (G = 1).
-template <typename T, int NY, int P, int G, int NL, int CPW = 2>
+// for the single column single_column(blk) (G = 1).  EPI >= 0 fixes the stage epilogue at
+// compile time (it must equal a.epi); EPI = -1 switches on a.epi.
+template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1>
 __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __restrict__ lds,
                                         const cplx<T>* __restrict__ tw) {
   constexpr int TPT = NY / P;
@@ -592,8 +594,9 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
   const Geo<T>& g = a.g;
   const bool active = kr < a.ncols && kr < g.nkr;
   const T kv = krwave(g, kr);
+  const int epi = EPI >= 0 ? EPI : a.epi;
 
-  if (a.sqrt_spec != nullptr && a.epi != E_STORE && active && kr <= a.fkx) {
+  if (a.sqrt_spec != nullptr && epi != E_STORE && active && kr <= a.fkx) {
     // Ring forcing (examples/twodnavierstokes_stochasticforcing.jl:78-86: N += sqrt(spectrum) xi
     // / sqrt(dt), xi complex standard normal per mode and step): this column's terms are drawn
     // now, while few registers are live, and read back in the epilogue.  Drawn there, the
@@ -612,7 +615,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
     }
   }
 
-  if (a.epi == E_STORE) {
+  if (epi == E_STORE) {
     // forward y-transform of mixed field(s) -> spectral state (set_q!/set_zeta!)
     for (int jl = 0; jl < g.nlayers; ++jl) {
       cplx<T> w[1][P];
@@ -717,7 +720,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       const uint32_t ci = a.cl_layered ? ix : cf;
       const cplx<T> sv = kept ? at(a.sol, ix) : czero<T>();
       if (a.nsave != nullptr) at(a.nsave, ix) = N;
-      switch (a.epi) {
+      switch (epi) {
         case E_CALCN:
           at(a.nout, ix) = N;
           break;
@@ -758,16 +761,16 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
         case E_RK3:
         case E_RK4: {
           const Coef<T> Lc{a.c0, a.ccplx};
-          const cplx<T> xin = (a.epi == E_RK1) ? sv : (kept ? at(a.s1, ix) : czero<T>());
+          const cplx<T> xin = (epi == E_RK1) ? sv : (kept ? at(a.s1, ix) : czero<T>());
           const cplx<T> R = N + Lc.mul(ci, xin);   // RHS = N + L sol_i
-          if (a.epi == E_RK1) {
+          if (epi == E_RK1) {
             if (kept) at(a.s1, ix) = sv + rmul(a.dt / (T)2, R);
             at(a.racc, ix) = mk(R.re / (T)6, R.im / (T)6);
-          } else if (a.epi == E_RK2) {
+          } else if (epi == E_RK2) {
             if (kept) at(a.s1, ix) = sv + rmul(a.dt / (T)2, R);
             const cplx<T> ra = at(a.racc, ix);
             at(a.racc, ix) = mk(ra.re + R.re / (T)3, ra.im + R.im / (T)3);
-          } else if (a.epi == E_RK3) {
+          } else if (epi == E_RK3) {
             if (kept) at(a.s1, ix) = sv + rmul(a.dt, R);
             const cplx<T> ra = at(a.racc, ix);
             at(a.racc, ix) = mk(ra.re + R.re / (T)3, ra.im + R.im / (T)3);
@@ -814,14 +817,14 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
   }
 }
 
-template <typename T, int NY, int P, int G, int NL, int CPW = 2>
+template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1>
 __global__ __launch_bounds__(CPW * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K3_WAVES)))
 void k3_kernel(const K3Args<T> a) {
   __shared__ cplx<T> lds[CPW == 2 ? G * PairLds<T, NY>::value : LdsN<NY>::value];
   __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
   load_twiddles<T, NY, P>(tw, a.tw);
   __syncthreads();
-  k3_body<T, NY, P, G, NL, CPW>(a, blockIdx.x, lds, tw);
+  k3_body<T, NY, P, G, NL, CPW, EPI>(a, blockIdx.x, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -937,7 +940,21 @@ void k3_np(const K3Args<T>& a, hipStream_t s) {
       // 683 workgroups at 4 per CU instead of 342 column pairs at 2 per CU on 256 CUs
       const int npairs = (a.ncols + 1) / 2;
       const int nblk = 16 * ((npairs + 7) / 8);
-      if (npairs > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1>), dim3(nblk), dim3(N / P), 0, s, a);
+      if (npairs <= 0) return;
+      const dim3 grid(nblk), blk(N / P);
+      if constexpr (std::is_same<T, double>::value && N == 2048 && P == 8) {
+        // FP64 at 2048, P = 8: one kernel per ETDRK4 stage.  Each fits the 4-wave budget of
+        // 128 VGPRs without spilling (120-128); the kernel that switches on the stage needs
+        // 137 and spills 8.
+        switch (a.epi) {
+          case E_ETD1: hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1, E_ETD1>), grid, blk, 0, s, a); return;
+          case E_ETD2: hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1, E_ETD2>), grid, blk, 0, s, a); return;
+          case E_ETD3: hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1, E_ETD3>), grid, blk, 0, s, a); return;
+          case E_ETD4: hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1, E_ETD4>), grid, blk, 0, s, a); return;
+          default: break;
+        }
+      }
+      hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1>), grid, blk, 0, s, a);
       return;
     }
   }
