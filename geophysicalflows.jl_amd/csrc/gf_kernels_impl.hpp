@@ -933,6 +933,14 @@ void k2_n(const K2Args<T>& a, hipStream_t s) {
 template <typename T, int N, int NL, int P>
 void k3_np(const K3Args<T>& a, hipStream_t s) {
   constexpr int G = GY<N, P>::G;
+  if constexpr (NL > 1 && N >= 512 && k3_single_column(N, P)) {
+    // multi-layer problems (fixed schedule): one column per workgroup from 512^2, where it
+    // measured 3.7% faster over a 3-layer 512^2 ETDRK4 step (0.1915 -> 0.1845 ms)
+    const int npairs = (a.ncols + 1) / 2;
+    const int nblk = 16 * ((npairs + 7) / 8);
+    if (npairs > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, 1, NL, 1>), dim3(nblk, a.lsplit ? NL : 1), dim3(N / P), 0, s, a);
+    return;
+  }
   if constexpr (NL == 1 && k3_single_column(N, P)) {
     static_assert(G == 1, "a single column fills a wave");
     if (a.csplit) {

@@ -249,9 +249,9 @@ def test_slqg_reference_nonlinear_advection(gf, topo, bg, vary, ell):
 
 
 # ----------------------------------------------------------------------------- MultiLayerQG
-@pytest.mark.parametrize("nlayers", [2, 3, 4])
-def test_mlqg_calcN_steps_updatevars(gf, nlayers):
-    n = 64
+@pytest.mark.parametrize("nlayers,n", [(2, 64), (3, 64), (4, 64), (2, 512), (3, 512), (4, 512)])
+def test_mlqg_calcN_steps_updatevars(gf, nlayers, n):
+    # from 512^2 the multi-layer K3 runs one spectral column per workgroup
     H = {2: [0.2, 0.8], 3: [0.2, 0.3, 0.5], 4: [0.1, 0.2, 0.3, 0.4]}[nlayers]
     b = {2: [-1.0, -1.2], 3: [-1.0, -1.2, -1.3], 4: [-1.0, -1.1, -1.25, -1.4]}[nlayers]
     U = list(np.linspace(1.0, 0.0, nlayers))
