@@ -817,8 +817,12 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
   }
 }
 
+// Occupancy target of K3.  The multi-layer single-column kernel (2-wave workgroups, 513 of them at
+// 512^2 x 3) needs 169 VGPRs: held to 128 for 4 waves it spilled 40; at 2 waves it does not
+// spill, and a CU still holds 4 of its workgroups.
+template <int NL, int CPW> struct K3Waves { static constexpr int value = (NL > 1 && CPW == 1) ? 2 : GF_K3_WAVES; };
 template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1>
-__global__ __launch_bounds__(CPW * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K3_WAVES)))
+__global__ __launch_bounds__(CPW * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(K3Waves<NL, CPW>::value)))
 void k3_kernel(const K3Args<T> a) {
   __shared__ cplx<T> lds[CPW == 2 ? G * PairLds<T, NY>::value : LdsN<NY>::value];
   __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
