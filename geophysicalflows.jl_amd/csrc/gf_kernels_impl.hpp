@@ -817,10 +817,11 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
   }
 }
 
-// Occupancy target of K3.  The multi-layer single-column kernel (2-wave workgroups, 513 of them at
-// 512^2 x 3) needs 169 VGPRs: held to 128 for 4 waves it spilled 40; at 2 waves it does not
-// spill, and a CU still holds 4 of its workgroups.
-template <int NL, int CPW> struct K3Waves { static constexpr int value = (NL > 1 && CPW == 1) ? 2 : GF_K3_WAVES; };
+// Occupancy target of K3.  The multi-layer kernels need about 170 VGPRs (the bottom drag's psi
+// per point, layered coefficient tables): held to 128 for 4 waves they spilled 30-50.  At 2
+// waves they do not spill, and a CU still holds more of their workgroups than the layered grids
+// supply (513 single-column workgroups at 512^2 x 3).
+template <int NL, int CPW> struct K3Waves { static constexpr int value = NL > 1 ? 2 : GF_K3_WAVES; };
 template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1>
 __global__ __launch_bounds__(CPW * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(K3Waves<NL, CPW>::value)))
 void k3_kernel(const K3Args<T> a) {
