@@ -62,3 +62,20 @@ def test_malformed_tables_are_rejected(gf, tmp_path, text, msg):
 def test_missing_file(gf, tmp_path):
     with pytest.raises(gf._lib.GFError, match="cannot open"):
         gf.load_schedules(tmp_path / "none.json")
+
+
+def test_shipped_table_loads_with_every_entry_offered(gf, tmp_path):
+    # the table beside the library is read at start-up; a malformed or unoffered entry would
+    # leave every problem on the built-in schedules without an error
+    import os
+    path = os.path.join(os.path.dirname(gf.__file__), "schedules.json")
+    ents = json.load(open(path))["entries"]
+    gf._lib.lib.gf_schedules_clear()
+    assert gf.load_schedules(path) == len(ents) > 0
+    head = [e for e in ents if (e["model"], e["nlayers"], e["nx"], e["ny"], e["dtype"]) == (0, 1, 2048, 2048, 0)]
+    assert len(head) == 1 and len(head[0]["split"]) == 3      # the headline workload is tuned
+    out = tmp_path / "again.json"
+    gf.save_schedules(out)
+    assert sorted(json.dumps(e, sort_keys=True) for e in json.loads(out.read_text())["entries"]) == \
+        sorted(json.dumps(e, sort_keys=True) for e in ents)
+    gf._lib.lib.gf_schedules_clear()
