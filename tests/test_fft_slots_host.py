@@ -1,4 +1,5 @@
-"""Host check of the FFT core's split LDS slot arithmetic (no GPU needed).
+"""Host checks of index arithmetic shared by the kernels (no GPU needed): the FFT core's split
+LDS slots and the ring-forcing box rows.
 
 The kernels address the padded LDS buffers as a per-thread base slot plus compile-time offsets
 (``pad(a + b) = pad(a) + pad(b)`` when ``a mod 8 + b mod 8 < 8``).  tests/host/fft_slots_check.cpp
@@ -23,3 +24,13 @@ def test_split_slots_match_direct_slots(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout
     assert r.stdout.strip().endswith("total 0")
+
+
+@pytest.mark.skipif(HIPCC is None, reason="hipcc is not installed")
+def test_ring_box_rows_are_one_to_one(tmp_path):
+    exe = tmp_path / "ring_box_check"
+    subprocess.run([HIPCC, "-O1", "-std=c++17", "-I", CSRC, os.path.join(HERE, "host", "ring_box_check.cpp"),
+                    "-o", str(exe)], check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+    assert r.stdout.strip().endswith("mismatches 0")
