@@ -354,6 +354,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only run the 1 GiB copy used to calibrate counters")
     ap.add_argument("--only-ensemble", action="store_true", help="only run the batched-ensemble measurement")
     ap.add_argument("--only-adjoint", action="store_true", help="only run the adjoint measurement")
+    ap.add_argument("--only-mlqg", action="store_true", help="only run the MultiLayerQG 512^2 x 3 measurement")
+    ap.add_argument("--only-decaying", action="store_true", help="only run the decaying 128^2 measurement")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -364,6 +366,12 @@ def main():
         out = {"adjoint_512": adjoint_run(gf)}
         if not args.no_adjoint_batch:
             out["adjoint_512_batch16"] = adjoint_batch_run(gf)
+        print(json.dumps(out), flush=True)
+        return
+    if args.only_mlqg or args.only_decaying:
+        import __graft_entry__ as ge
+        gf = ge.load_package()
+        out = {"mlqg_512x3": mlqg_run(gf, 200)} if args.only_mlqg else {"decaying_128": decaying_run(gf, 200)}
         print(json.dumps(out), flush=True)
         return
     if args.only_ensemble:
