@@ -156,3 +156,33 @@ def test_sixteen_point_schedules_at_1024(gf, dtype, tol, sched):
     Nr, sr = np.zeros_like(s), s.copy()
     o2d.calcN(Nr, sr, 0.0, r.clock, r.vars, r.params, r.grid)
     assert rel(N, Nr) < tol
+
+
+@pytest.mark.parametrize("nx,ny,Ly", [(128, 512, 2.5), (512, 256, 4.0)])
+def test_k3_column_split_on_rectangular_grids(gf, nx, ny, Ly):
+    # columns number nx/2 + 1 (kept: about nx/3) while a column's length is ny: the split is
+    # offered from ny / P_K3 >= 64 and must agree with the pair layout bit for bit and with the oracle
+    rng = np.random.default_rng(97)
+    T = gf.TwoDNavierStokes
+    kw = dict(nx=nx, ny=ny, Lx=2 * math.pi, Ly=Ly, ν=1e-4, nν=1, μ=0.05, dt=0.002, stepper="ETDRK4")
+    s = rng.standard_normal((ny, nx // 2 + 1)) + 1j * rng.standard_normal((ny, nx // 2 + 1))
+    s *= 0.05
+    s[0, 0] = 0
+    out = []
+    for cs in (0, 1):
+        p = T.Problem(gf.GPU(), **kw)
+        gf.set_schedule(p, 4, 4, 4)
+        gf.set_splits(p, 0, 0, cs)
+        N, _ = p.calcN(s)
+        p.sol = s
+        gf.stepforward(p, 3)
+        out.append((N, p.sol))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    r = o2d.Problem(nx=nx, ny=ny, Lx=2 * math.pi, Ly=Ly, nu=1e-4, nnu=1, mu=0.05, dt=0.002, stepper="ETDRK4")
+    Nr, sr = np.zeros_like(s), s.copy()
+    o2d.calcN(Nr, sr, 0.0, r.clock, r.vars, r.params, r.grid)
+    assert rel(out[1][0], Nr) < 1e-13
+    r.sol[...] = s
+    from oracle import fourierflows as off
+    off.stepforward(r, 3)
+    assert rel(out[1][1], r.sol) < 1e-12
