@@ -30,7 +30,9 @@ template <int N> struct BatchGeom {
   static_assert(TwPlan<N, BATCH_P>::size() <= BATCH_TW, "twiddle table");
 };
 
-template <typename T, int N, int K>
+// PLAIN (K = 2): every member advects without background gradients, so the kernel holds only that
+// x-pass body (122 VGPRs instead of 243 with the K2_ADVECT_FULL body beside it).
+template <typename T, int N, int K, bool PLAIN>
 __device__ __forceinline__ void batch_body(const BatchItem<T>& it, int blk, cplx<T>* lds, cplx<T>* tw) {
   using BG = BatchGeom<N>;
   if constexpr (K == 1) {
@@ -41,8 +43,11 @@ __device__ __forceinline__ void batch_body(const BatchItem<T>& it, int blk, cplx
   } else if constexpr (K == 2) {
     load_twiddles<T, N, BATCH_P>(tw, it.k2.tw);
     __syncthreads();
-    if (it.k2.mode == K2_ADVECT_FULL) k2_body<T, N, BATCH_P, BG::G2, K2_ADVECT_FULL, 1>(it.k2, blk, lds, tw);
-    else k2_body<T, N, BATCH_P, BG::G2, K2_ADVECT, 1>(it.k2, blk, lds, tw);
+    if (!PLAIN && it.k2.mode == K2_ADVECT_FULL) {
+      if constexpr (!PLAIN) k2_body<T, N, BATCH_P, BG::G2, K2_ADVECT_FULL, 1>(it.k2, blk, lds, tw);
+    } else {
+      k2_body<T, N, BATCH_P, BG::G2, K2_ADVECT, 1>(it.k2, blk, lds, tw);
+    }
   } else {
     load_twiddles<T, N, BATCH_P>(tw, it.k3.tw);
     __syncthreads();
@@ -50,7 +55,7 @@ __device__ __forceinline__ void batch_body(const BatchItem<T>& it, int blk, cplx
   }
 }
 
-template <typename T, int K>
+template <typename T, int K, bool PLAIN>
 __global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 void batch_kernel(const BatchItem<T>* __restrict__ items, const BatchWork* __restrict__ work) {
   __shared__ cplx<T> lds[BATCH_LDS];
@@ -58,11 +63,11 @@ void batch_kernel(const BatchItem<T>* __restrict__ items, const BatchWork* __res
   const BatchWork w = work[blockIdx.x];
   const BatchItem<T>& it = items[w.item];
   switch (it.n) {
-    case 32: batch_body<T, 32, K>(it, w.blk, lds, tw); break;
-    case 64: batch_body<T, 64, K>(it, w.blk, lds, tw); break;
-    case 128: batch_body<T, 128, K>(it, w.blk, lds, tw); break;
-    case 256: batch_body<T, 256, K>(it, w.blk, lds, tw); break;
-    case 512: batch_body<T, 512, K>(it, w.blk, lds, tw); break;
+    case 32: batch_body<T, 32, K, PLAIN>(it, w.blk, lds, tw); break;
+    case 64: batch_body<T, 64, K, PLAIN>(it, w.blk, lds, tw); break;
+    case 128: batch_body<T, 128, K, PLAIN>(it, w.blk, lds, tw); break;
+    case 256: batch_body<T, 256, K, PLAIN>(it, w.blk, lds, tw); break;
+    case 512: batch_body<T, 512, K, PLAIN>(it, w.blk, lds, tw); break;
     default: break;
   }
 }
@@ -72,10 +77,12 @@ void batch_kernel(const BatchItem<T>* __restrict__ items, const BatchWork* __res
 #endif
 
 template <typename T, int K>
-void launch_batch(const BatchItem<T>* items, const BatchWork* work, int nwork, hipStream_t s) {
-  if (nwork > 0) hipLaunchKernelGGL((batch_kernel<T, K>), dim3(nwork), dim3(BATCH_THREADS), 0, s, items, work);
+void launch_batch(const BatchItem<T>* items, const BatchWork* work, int nwork, int plain, hipStream_t s) {
+  if (nwork <= 0) return;
+  if (K == 2 && plain) hipLaunchKernelGGL((batch_kernel<T, K, true>), dim3(nwork), dim3(BATCH_THREADS), 0, s, items, work);
+  else hipLaunchKernelGGL((batch_kernel<T, K, false>), dim3(nwork), dim3(BATCH_THREADS), 0, s, items, work);
 }
-template void launch_batch<double, GF_BATCH_K>(const BatchItem<double>*, const BatchWork*, int, hipStream_t);
+template void launch_batch<double, GF_BATCH_K>(const BatchItem<double>*, const BatchWork*, int, int, hipStream_t);
 
 #if GF_BATCH_K == 1
 bool batch_size_supported(int n) { return n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }

@@ -1410,6 +1410,7 @@ struct BatchCtx final : CtxBase {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  bool k2_plain = true;   // every member's x pass is K2_ADVECT (no background gradients)
   void build_tables() {
     const int n = (int)inst.size();
     std::vector<int> ord(n);
@@ -1419,6 +1420,7 @@ struct BatchCtx final : CtxBase {
     int epis[4];
     nst = inst[0]->stages(ins, epis);
     std::vector<BatchItem<T>> h(n);
+    k2_plain = true;
     for (int full = 0; full < 2; ++full) {
       for (int k = 0; k < nst; ++k) {
         for (int i = 0; i < n; ++i) {
@@ -1430,6 +1432,7 @@ struct BatchCtx final : CtxBase {
           h[i].k1.fsplit = 0;   // the batched launches are one-dimensional
           h[i].k2.hsplit = 0;
           h[i].k3.csplit = 0;
+          k2_plain = k2_plain && h[i].k2.mode == K2_ADVECT;
         }
         items[k][full].alloc(sizeof(BatchItem<T>) * n);
         HIPCHECK(hipMemcpy(items[k][full].p, h.data(), sizeof(BatchItem<T>) * n, hipMemcpyHostToDevice));
@@ -1453,13 +1456,13 @@ struct BatchCtx final : CtxBase {
     for (int k = 0; k < nst; ++k) {
       const BatchItem<T>* it = items[k][full].as<BatchItem<T>>();
       tm.rec(0, true, stream);
-      launch_batch<T, 1>(it, work[0][full].as<BatchWork>(), nwork[0][full], stream);
+      launch_batch<T, 1>(it, work[0][full].as<BatchWork>(), nwork[0][full], 0, stream);
       tm.rec(0, false, stream);
       tm.rec(1, true, stream);
-      launch_batch<T, 2>(it, work[1][full].as<BatchWork>(), nwork[1][full], stream);
+      launch_batch<T, 2>(it, work[1][full].as<BatchWork>(), nwork[1][full], k2_plain, stream);
       tm.rec(1, false, stream);
       tm.rec(2, true, stream);
-      launch_batch<T, 3>(it, work[2][full].as<BatchWork>(), nwork[2][full], stream);
+      launch_batch<T, 3>(it, work[2][full].as<BatchWork>(), nwork[2][full], 0, stream);
       tm.rec(2, false, stream);
     }
     HIPCHECK(hipGetLastError());

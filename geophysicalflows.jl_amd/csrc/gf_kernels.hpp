@@ -259,7 +259,9 @@ struct BatchWork {
 
 // Launch kernel class K (1, 2, 3) of one stage for every instance: workgroup b runs
 // block work[b].blk of instance work[b].item (FP64 only).
-template <typename T, int K> void launch_batch(const BatchItem<T>* items, const BatchWork* work, int nwork, hipStream_t s);
+// plain (K = 2): every member's x pass is K2_ADVECT, which selects the kernel without the
+// background-gradient body
+template <typename T, int K> void launch_batch(const BatchItem<T>* items, const BatchWork* work, int nwork, int plain, hipStream_t s);
 // workgroups of kernel class k for an instance of size n: count = columns (k = 1, 3) or ny (k = 2)
 int batch_blocks(int k, int n, int count);
 bool batch_size_supported(int n);
