@@ -276,7 +276,10 @@ __device__ __forceinline__ void stockham(cplx<T> (&v)[C][P], cplx<T>* __restrict
 template <typename T, int N, int P, int C, bool FRESH = false>
 __device__ __forceinline__ void fft_forward(cplx<T> (&v)[C][P], cplx<T>* __restrict__ lds, int t,
                                             const cplx<T>* __restrict__ tw) {
-  if constexpr (FRESH) asm volatile("" : "+v"(t));
+  if constexpr (FRESH) {
+    asm volatile("" : "+v"(t));
+    __builtin_assume(t >= 0 && t < N / P);   // the asm hides t's range from the multiplies below
+  }
   stockham<T, N, P, C, 0, 1>(v, lds, t, tw);
 }
 
