@@ -654,7 +654,7 @@ struct Ctx final : CtxBase {
     upload_blocked(sqrt_spec, ss);
     if (fkx >= 0) {
       nbl = std::min(ny, 2 * fky + 1);
-      noise.alloc(sizeof(C) * (size_t)(fkx + 1) * nbl);
+      noise.alloc(sizeof(C) * (size_t)(fkx + 1) * nbl * nl);   // one slice per layer row (lsplit)
     }
   }
 

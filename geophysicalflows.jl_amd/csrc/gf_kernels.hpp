@@ -113,8 +113,9 @@ struct K3Args {
   T fscale;                 // factor on Fh: 1, or cos(omega t + phase) at the stage time
   const T* sqrt_spec;       // ring forcing amplitude (blocked real), nullable
   uint64_t seed; int64_t fstep; T sqrt_dt; int fkx, fky; int instance;
-  // ring forcing: (fkx + 1) x nbl box of this step's forcing terms, written by K3 itself
-  // before its transforms and read back in the stage epilogue (ring_box_row)
+  // ring forcing: (fkx + 1) x nbl box of this step's forcing terms (one per layer row when
+  // lsplit), written by K3 itself before its transforms and read back in the stage epilogue
+  // (ring_box_row)
   cplx<T>* noise; int nbl;
   const int64_t* fstep_dev; // nullable: the forcing step is read here (steps replayed from a graph)
   // MLQG bottom drag mu Krsq psi_n

@@ -596,12 +596,15 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
   const T kv = krwave(g, kr);
   const int epi = EPI >= 0 ? EPI : a.epi;
 
+  // Ring forcing (examples/twodnavierstokes_stochasticforcing.jl:78-86: N += sqrt(spectrum) xi
+  // / sqrt(dt), xi complex standard normal per mode and step): this column's terms are drawn
+  // now, while few registers are live, and read back in the epilogue.  Drawn there, the Philox
+  // and Box-Muller sequence set the whole kernel's register count (144 VGPRs instead of 118 at
+  // 2048, P = 8), although only the box's few columns execute it.  A workgroup row of its own
+  // per layer (lsplit) has its own slice of the box, so every workgroup reads back only what it
+  // wrote.
+  cplx<T>* const noise = a.noise + ((NL > 1 && a.lsplit) ? (long)blockIdx.y * (a.fkx + 1) * a.nbl : 0);
   if (a.sqrt_spec != nullptr && epi != E_STORE && active && kr <= a.fkx) {
-    // Ring forcing (examples/twodnavierstokes_stochasticforcing.jl:78-86: N += sqrt(spectrum) xi
-    // / sqrt(dt), xi complex standard normal per mode and step): this column's terms are drawn
-    // now, while few registers are live, and read back in the epilogue.  Drawn there, the
-    // Philox and Box-Muller sequence set the whole kernel's register count (144 VGPRs instead of
-    // 118 at 2048, P = 8), although only the box's few columns execute it.
     const int64_t fstep = a.fstep_dev != nullptr ? a.fstep_dev[0] : a.fstep;
     for (int q = 0; q < P; ++q) {
       const int l = t + TPT * q;
@@ -609,7 +612,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
         const T sp = at(a.sqrt_spec, boff32(kr, l, NY));
         if (sp != (T)0) {
           const cplx<T> xi = complex_normal<T>(a.seed, fstep, (uint32_t)(l * g.nkr + kr), (uint32_t)a.instance);
-          a.noise[kr * a.nbl + ring_box_row(l, NY, a.fky, a.nbl)] = mk((xi.re * sp) / a.sqrt_dt, (xi.im * sp) / a.sqrt_dt);
+          noise[kr * a.nbl + ring_box_row(l, NY, a.fky, a.nbl)] = mk((xi.re * sp) / a.sqrt_dt, (xi.im * sp) / a.sqrt_dt);
         }
       }
     }
@@ -710,7 +713,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       if (a.Fh != nullptr) N = N + rmul(a.fscale, at(a.Fh, ix));
       if (a.sqrt_spec != nullptr) {
         if (kr <= a.fkx && (l <= a.fky || l >= NY - a.fky)) {
-          if (at(a.sqrt_spec, idx) != (T)0) N = N + a.noise[kr * a.nbl + ring_box_row(l, NY, a.fky, a.nbl)];
+          if (at(a.sqrt_spec, idx) != (T)0) N = N + noise[kr * a.nbl + ring_box_row(l, NY, a.fky, a.nbl)];
         }
       }
       // The stepper's coefficient tables and the filter depend on kr and Krsq only, so rows l and

@@ -398,3 +398,28 @@ def test_translation_equivariance_512(gf):
     shift = np.exp(-1j * g.Kr * g.dx)
     m = g.dealias_mask()
     assert rel(b.sol * m, a.sol * shift * m) < 1e-12
+
+
+@pytest.mark.parametrize("n", [64, 512])
+def test_mlqg_ring_forcing_matches_oracle(gf, n):
+    # the ring forcing's realisation is added to every layer; with one layer per workgroup row each
+    # row draws it into a box slice of its own (64^2: column pairs, 512^2: one column per workgroup)
+    from oracle import forcing as ofo
+    kf, dkf, eps, seed = 6.0, 1.5, 0.1, 21
+    kw = dict(nx=n, Lx=TWO_PI, H=[0.2, 0.8], b=[-1.0, -1.2], U=[1.0, 0.0], dt=0.002, stepper="ETDRK4")
+    p = gf.MultiLayerQG.Problem(2, gf.GPU(), f0=1.0, β=5.0, μ=0.005, ν=1e-4,
+                                calcFq=gf.RingForcing(kf, dkf, eps, seed=seed), **kw)
+    cache = {}
+
+    def ring(Fqh, sol, t, clock, vars, prm, grid):
+        if "s" not in cache:
+            cache["s"] = np.sqrt(ofo.ring_spectrum(grid, kf, dkf, eps))
+        Fqh[...] = ofo.ring_forcing(grid, cache["s"], seed, clock.step, clock.dt)[None]
+
+    r = oml.Problem(2, f0=1.0, beta=5.0, mu=0.005, nu=1e-4, calcFq=ring, stochastic=True, **kw)
+    s = rand_spec(np.random.default_rng(n), (2, n, n // 2 + 1)) * 0.02
+    p.sol = s
+    r.sol[...] = s
+    gf.stepforward(p, 4)
+    off.stepforward(r, 4)
+    assert rel(p.sol, r.sol) < 1e-12
