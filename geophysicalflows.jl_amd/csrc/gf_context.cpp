@@ -790,6 +790,10 @@ struct Ctx final : CtxBase {
     const int64_t step0 = stepno;
     const bool pv0 = prevsol_valid;
     const int old[3] = {sched[0], sched[1], sched[2]};
+    // plain launches while timing: a schedule change drops the captured graphs, and capturing
+    // again inside a timed block would be charged to the schedule
+    const bool old_graphs = graphs;
+    graphs = false;
     const int old_split[3] = {split[0], split[1], split[2]};
     static constexpr int PV[3] = {4, 8, 16};
     auto decode = [](int c, int* q, int* v) {
@@ -841,9 +845,11 @@ struct Ctx final : CtxBase {
       cleanup();
       set_schedule(old);
       set_splits(old_split);
+      graphs = old_graphs;
       throw;
     }
     cleanup();
+    graphs = old_graphs;
     for (int j = 0; j < 4; ++j)
       if (save[j].p) HIPCHECK(hipMemcpy(keep[j]->p, save[j].p, cb, hipMemcpyDeviceToDevice));
     t = t0; stepno = step0; prevsol_valid = pv0;
@@ -867,8 +873,8 @@ struct Ctx final : CtxBase {
   // one graph launch per step instead of 3 per stage. The ring forcing's step number, the
   // one argument that changes, is read from step_dev, which the graph advances itself.
   static constexpr int GRAPH_BLOCK = 8;   // interior steps in the larger graph
-  // Off unless gf_use_graphs(ctx, 1): for one problem the replay gains at most 3% at small sizes
-  // and loses about 1% at 2048^2 (the steps are bound by kernel time, not by launches).
+  // Off unless gf_use_graphs(ctx, 1): for one problem the replay gains 1-3% up to 512^2 and
+  // loses 0.7% at 2048^2 (the steps are bound by kernel time, not by launches).
   bool graphs = false;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};   // one interior step, GRAPH_BLOCK interior steps
   DevBuf step_dev;
