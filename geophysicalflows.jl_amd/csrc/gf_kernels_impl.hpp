@@ -249,12 +249,18 @@ void k1_kernel(const K1Args<T> a) {
 // Hermitian-extended entry k of a half spectrum (kr = k or NX-k).
 template <typename T, int NX>
 __device__ __forceinline__ cplx<T> herm_load(const cplx<T>* f, int k, int y, int ny, int kin, bool active) {
+  // Branch-free: every lane loads and a select clears what it must not use. With an early return
+  // here the compiler waited for each load before issuing the next one. A lane past the kept
+  // columns re-reads its row's last kept column (a line other lanes fetch anyway; one shared
+  // address for all of them would send every such load in the chip to one L2 channel), and a
+  // lane outside the rows reads element 0, which every field has.
   const int kk = k <= NX / 2 ? k : NX - k;
-  if (!active || kk >= kin) return czero<T>();
-  cplx<T> v = at(f, boff32(kk, y, ny));
-  if (kk == 0 || kk == NX / 2) v.im = (T)0;   // c2r ignores the imaginary DC/Nyquist parts
-  else if (k > NX / 2) v.im = -v.im;
-  return v;
+  const bool ok = active && kk < kin;
+  const uint32_t idx = (active && kin > 0) ? boff32(kk < kin ? kk : kin - 1, y, ny) : 0u;
+  const cplx<T> v = at(f, idx);
+  // c2r ignores the imaginary DC/Nyquist parts
+  const T im = (kk == 0 || kk == NX / 2) ? (T)0 : (k > NX / 2 ? -v.im : v.im);
+  return ok ? mk(v.re, im) : czero<T>();
 }
 
 // Split of a packed spectrum W = F(a + i b) of two real rows:
@@ -358,15 +364,21 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
       // of the x pass's load instructions fetched data another lane had already loaded (at
       // 2048^2 these instructions each touch about 30 pages); the exchange costs two barriers.
       // 2048^2 step 0.3480 -> 0.3435 ms, 1024^2 0.1569 -> 0.1531.
+      // All loads are issued before any is used, so their latencies overlap.
+      constexpr int QH = P / 2 + 1;                     // the q with TPT q <= NX/2
+      cplx<T> A[QH], B[QH];
 #pragma unroll
-      for (int q = 0; q < P; ++q) {
-        if (TPT * q > NX / 2) continue;                 // no lane of this q holds k <= NX/2
+      for (int q = 0; q < QH; ++q) {
+        const int k = t + TPT * q;
+        A[q] = herm_load<T, NX>(f, k, y0, ny, a.kin, active && k <= NX / 2);
+        B[q] = herm_load<T, NX>(f, k, y0 + 1, ny, a.kin, active && k <= NX / 2);
+      }
+#pragma unroll
+      for (int q = 0; q < QH; ++q) {
         const int k = t + TPT * q;
         if (k > NX / 2) continue;
-        const cplx<T> A = herm_load<T, NX>(f, k, y0, ny, a.kin, active);
-        const cplx<T> B = herm_load<T, NX>(f, k, y0 + 1, ny, a.kin, active);
-        w[0][q] = mk(A.re - B.im, -(A.im + B.re));     // conj(A + i B)
-        if (k > 0 && k < NX / 2) L[pad(NX - k)] = mk(A.re + B.im, A.im - B.re);
+        w[0][q] = mk(A[q].re - B[q].im, -(A[q].im + B[q].re));     // conj(A + i B)
+        if (k > 0 && k < NX / 2) L[pad(NX - k)] = mk(A[q].re + B[q].im, A[q].im - B[q].re);
       }
       __syncthreads();
 #pragma unroll
