@@ -360,14 +360,16 @@ struct Ctx final : CtxBase {
     const T a = (T)krw(kr), b = (T)lw(l);
     return (double)(T)(a * a + b * b);
   }
+  // pad: zeroed bytes after the table (the stepper's coefficient tables carry 16, so that the
+  // stage epilogue may read a real table's element i + 1 along with element i)
   template <typename X>
-  void upload_blocked(DevBuf& b, const std::vector<X>& full /* [l][kr] */, int nlay = 1) {
+  void upload_blocked(DevBuf& b, const std::vector<X>& full /* [l][kr] */, int nlay = 1, size_t pad = 0) {
     std::vector<X> h((size_t)fsz * nlay, X{});
     for (int j = 0; j < nlay; ++j)
       for (int l = 0; l < d.ny; ++l)
         for (int kr = 0; kr < g.nkr; ++kr)
           h[(size_t)j * fsz + boff_h(kr, l, d.ny)] = full[((size_t)j * d.ny + l) * g.nkr + kr];
-    b.alloc(sizeof(X) * h.size());
+    b.alloc(sizeof(X) * h.size() + pad);
     HIPCHECK(hipMemcpy(b.p, h.data(), sizeof(X) * h.size(), hipMemcpyHostToDevice));
   }
   // device rfft2 of nf physical fields (x fastest) -> blocked spectral fields
@@ -588,11 +590,11 @@ struct Ctx final : CtxBase {
       if (ccplx) {
         std::vector<C> h(nm);
         for (size_t i = 0; i < nm; ++i) { h[i].re = (T)v[i].real(); h[i].im = (T)v[i].imag(); }
-        upload_blocked(b, h);
+        upload_blocked(b, h, 1, 16);
       } else {
         std::vector<T> h(nm);
         for (size_t i = 0; i < nm; ++i) h[i] = (T)v[i].real();
-        upload_blocked(b, h);
+        upload_blocked(b, h, 1, 16);
       }
     };
     if (etd()) {

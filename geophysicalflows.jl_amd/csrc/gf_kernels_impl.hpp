@@ -81,9 +81,15 @@ __device__ __noinline__ cplx<T> complex_normal(uint64_t seed, int64_t step, uint
 
 template <typename T> struct Coef {
   const void* p; int cplx_;
+  // The same two loads for either kind of table: a complex table's element i, or a real table's
+  // elements i and i + 1 (every table carries 16 bytes of padding), the second then unused. With
+  // a branch on the kind between a load and its use, the compiler waited for each coefficient
+  // before issuing the next load.
   __device__ __forceinline__ cplx<T> mul(uint32_t i, cplx<T> x) const {
-    if (cplx_) return cmul(at((const cplx<T>*)p, i), x);
-    return rmul(at((const T*)p, i), x);
+    const T* e = reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) +
+                                            i * (uint32_t)(cplx_ ? sizeof(cplx<T>) : sizeof(T)));
+    const T re = e[0], im = e[1];
+    return cplx_ ? cmul(mk(re, im), x) : rmul(re, x);
   }
 };
 
@@ -751,34 +757,35 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       // cache and halves the tables' HBM traffic.
       const uint32_t cf = boff32(kr, l <= NY / 2 ? l : NY - l, NY);
       const uint32_t ci = a.cl_layered ? ix : cf;
-      const cplx<T> sv = kept ? at(a.sol, ix) : czero<T>();
+      const cplx<T> svr = at(a.sol, ix);     // every lane of an active column has this point
+      const cplx<T> sv = kept ? svr : czero<T>();
       if (a.nsave != nullptr) at(a.nsave, ix) = N;
       switch (epi) {
         case E_CALCN:
           at(a.nout, ix) = N;
           break;
+        // The stages compute for every lane and store only where kept: loads under a branch on
+        // kept made the compiler wait for each before issuing the next.
         case E_ETD1: {  // sol1 = E2 sol + zeta N1
-          if (kept) {
-            const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
-            at(a.s1, ix) = E2.mul(ci, sv) + Z.mul(ci, N);
-          }
+          const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
+          const cplx<T> r = E2.mul(ci, sv) + Z.mul(ci, N);
+          if (kept) at(a.s1, ix) = r;
           at(a.n1, ix) = N;
         } break;
         case E_ETD2: {  // sol2 = E2 sol + zeta N2
-          if (kept) {
-            const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
-            at(a.s2, ix) = E2.mul(ci, sv) + Z.mul(ci, N);
-          }
+          const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
+          const cplx<T> r = E2.mul(ci, sv) + Z.mul(ci, N);
+          if (kept) at(a.s2, ix) = r;
           at(a.n23, ix) = N;
         } break;
         case E_ETD3: {  // sol2 = E2 sol1 + zeta (2 N3 - N1)
-          if (kept) {
-            const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
-            const cplx<T> s1v = at(a.s1, ix);
-            const cplx<T> n1v = at(a.n1, ix);
-            at(a.s2, ix) = E2.mul(ci, s1v) + Z.mul(ci, mk((T)2 * N.re - n1v.re, (T)2 * N.im - n1v.im));
-          }
-          at(a.n23, ix) = at(a.n23, ix) + N;
+          const Coef<T> E2{a.c0, a.ccplx}, Z{a.c1, a.ccplx};
+          const cplx<T> s1v = at(a.s1, ix);
+          const cplx<T> n1v = at(a.n1, ix);
+          const cplx<T> n23v = at(a.n23, ix);
+          const cplx<T> r = E2.mul(ci, s1v) + Z.mul(ci, mk((T)2 * N.re - n1v.re, (T)2 * N.im - n1v.im));
+          if (kept) at(a.s2, ix) = r;
+          at(a.n23, ix) = n23v + N;
         } break;
         case E_ETD4: {  // sol = E sol + alpha N1 + 2 beta (N2 + N3) + Gamma N4
           const Coef<T> E{a.c2, a.ccplx}, Al{a.c3, a.ccplx}, Be{a.c4, a.ccplx}, Ga{a.c5, a.ccplx};
