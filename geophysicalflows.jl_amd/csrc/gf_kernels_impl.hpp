@@ -172,6 +172,13 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
     // src/multilayerqg.jl:495-509)
     T iK[P];
     cplx<T> ps[P];
+    // A point's state is read in a loop of its own, before any is used: read where it is used,
+    // each read's branch on the dealias mask made the compiler wait for it before the next.
+    cplx<T> svs[P];
+    if constexpr (FS != FS_RAW && MODEL != M_TWODNS && NL == 1) {
+#pragma unroll
+      for (int q = 0; q < P; ++q) svs[q] = sval(jl, t + TPT * q);
+    }
     if constexpr (FS != FS_RAW) {
 #pragma unroll
       for (int q = 0; q < P; ++q) {
@@ -191,7 +198,7 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
           // the bottom layer's psih, for K3's bottom drag of the same stage
           if (FS == FS_CALCN && jl == NL - 1 && a.psi_out != nullptr && active) at(a.psi_out, boff32(kr, l, NY)) = acc;
         } else {
-          const cplx<T> sv = sval(jl, l);
+          const cplx<T> sv = svs[q];
           if (a.eb) ps[q] = mk(-sv.re / (K2 + a.ell2inv), -sv.im / (K2 + a.ell2inv));
           else ps[q] = rmul(-((kr == 0 && l == 0) ? (T)0 : (T)1 / K2), sv);
         }
@@ -206,16 +213,20 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
       }
       const int code = FSET::code(f);
       cplx<T> w[1][P];
+      if (code == F_RAW || code == F_Q || MODEL == M_TWODNS) {
+#pragma unroll
+        for (int q = 0; q < P; ++q) svs[q] = sval(jl, t + TPT * q);
+      }
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         const int l = t + TPT * q;
         const T lv = lwave(g, l);
         cplx<T> r;
         if (code == F_RAW || code == F_Q) {
-          r = sval(jl, l);
+          r = svs[q];
         } else if constexpr (MODEL == M_TWODNS) {
           // uh = i l invK sol, vh = -i kr invK sol (src/twodnavierstokes.jl:254-255)
-          const cplx<T> sv = sval(jl, l);
+          const cplx<T> sv = svs[q];
           if (code == F_U) { const T aa = lv * iK[q]; r = mk(-aa * sv.im, aa * sv.re); }
           else if (code == F_V) { const T bb = kv * iK[q]; r = mk(bb * sv.im, -bb * sv.re); }
           else { r = rmul(-iK[q], sv); }
