@@ -46,20 +46,30 @@ __device__ __forceinline__ void ka1_body(const AdjK1Args<T>& a, int blk, cplx<T>
   for (int f = 0; f < 5; ++f) {
     if (a.fsplit && f != (int)blockIdx.y) continue;   // one field per workgroup row (uniform)
     cplx<T> w[1][P];
+    // the field's reads first, all of them, then their use: read where used, each read's branch
+    // on the dealias mask made the compiler wait for it before issuing the next
+    cplx<T> vs[P];
+    T cs[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) vs[q] = masked(f < 3 ? a.s : a.x, t + TPT * q);
+    if (f >= 3 && a.lamc != nullptr) {
+#pragma unroll
+      for (int q = 0; q < P; ++q) cs[q] = at(a.lamc, active ? boff32(kr, t + TPT * q, NY) : 0u);
+    }
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int l = t + TPT * q;
       const T lv = lwave(g, l);
       cplx<T> r;
       if (f < 3) {
-        const cplx<T> sv = masked(a.s, l);
+        const cplx<T> sv = vs[q];
         const T iK = (kr == 0 && l == 0) ? (T)0 : (T)1 / (kv * kv + lv * lv);
         if (f == 0) { const T aa = lv * iK; r = mk(-aa * sv.im, aa * sv.re); }   // u: i l s/|k|^2
         else if (f == 1) { const T bb = kv * iK; r = mk(bb * sv.im, -bb * sv.re); }  // v: -i kr s/|k|^2
         else r = sv;                                                                // zeta
       } else {
-        cplx<T> lam = masked(a.x, l);
-        if (a.lamc != nullptr && active && krk && kept_l(g, l)) lam = rmul(at(a.lamc, boff32(kr, l, NY)), lam);
+        cplx<T> lam = vs[q];
+        if (a.lamc != nullptr && active && krk && kept_l(g, l)) lam = rmul(cs[q], lam);
         const T fac = (f == 3 ? kv : lv) * wi;                                      // i kr/w, i l/w
         r = mk(-fac * lam.im, fac * lam.re);
       }
