@@ -11,8 +11,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .problem import GPU, Problem as _Problem, attach_forcing, device_diagnostic, dptr, make_desc, stepforward  # noqa: F401
-from .spectral import parsevalsum
+from .problem import GPU, Grid, Problem as _Problem, attach_forcing, device_diagnostic, dptr, make_desc, stepforward  # noqa: F401
 
 
 class Params:
@@ -37,11 +36,10 @@ def _U_layers(U, n, ny):
 def Problem(nlayers, dev=None, *, nx=128, ny=None, Lx=2 * math.pi, Ly=None, f0=1.0, β=0.0, U=None, H=None,
             b=None, eta=None, topographic_pv_gradient=(0.0, 0.0), μ=0.0, ν=0.0, nν=1, dt=0.01, stepper="RK4",
             calcFq=None, stochastic=False, linear=False, aliased_fraction=1 / 3, T=np.float64):
-    from .grid import make_grid
     n = int(nlayers)
     ny = nx if ny is None else ny
     Ly = Lx if Ly is None else Ly
-    grid = make_grid(nx, ny, Lx, Ly, aliased_fraction, T)
+    grid = Grid(int(nx), int(ny), Lx, Ly, aliased_fraction, T)
     d = make_desc(_lib.GF_MODEL_MLQG, nx, ny, Lx, Ly, T, stepper, dt, aliased_fraction)
     d.nlayers = n
     H = np.ones(n) / n if H is None else H

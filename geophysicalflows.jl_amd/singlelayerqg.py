@@ -10,7 +10,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .problem import GPU, Problem as _Problem, attach_forcing, device_diagnostic, dptr, make_desc, stepforward  # noqa: F401
+from .problem import GPU, Grid, Problem as _Problem, attach_forcing, device_diagnostic, dptr, make_desc, stepforward  # noqa: F401
 
 
 class Params:
@@ -24,10 +24,9 @@ class SLQGProblem(_Problem):
 def Problem(dev=None, *, nx=256, ny=None, Lx=2 * math.pi, Ly=None, β=0.0, deformation_radius=math.inf,
             U=0.0, eta=None, ν=0.0, nν=1, μ=0.0, dt=0.01, stepper="RK4", calcF=None, stochastic=False,
             aliased_fraction=1 / 3, T=np.float64):
-    from .grid import make_grid
     ny = nx if ny is None else ny
     Ly = Lx if Ly is None else Ly
-    grid = make_grid(nx, ny, Lx, Ly, aliased_fraction, T)
+    grid = Grid(int(nx), int(ny), Lx, Ly, aliased_fraction, T)
     d = make_desc(_lib.GF_MODEL_SLQG, nx, ny, Lx, Ly, T, stepper, dt, aliased_fraction)
     d.beta, d.mu, d.nu, d.nnu = float(β), float(μ), float(ν), int(nν)
     ell = deformation_radius

@@ -11,7 +11,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .problem import GPU, Problem as _Problem, attach_forcing, device_diagnostic, make_desc, stepforward  # noqa: F401
+from .problem import GPU, Grid, Problem as _Problem, attach_forcing, device_diagnostic, make_desc, stepforward  # noqa: F401
 
 
 class Params:
@@ -25,13 +25,12 @@ class TwoDNSProblem(_Problem):
 
 def Problem(dev=None, *, nx=256, ny=None, Lx=2 * math.pi, Ly=None, ν=0.0, nν=1, μ=0.0, nμ=0,
             dt=0.01, stepper="RK4", calcF=None, stochastic=False, aliased_fraction=1 / 3, T=np.float64):
-    from .grid import make_grid
     ny = nx if ny is None else ny
     Ly = Lx if Ly is None else Ly
     d = make_desc(_lib.GF_MODEL_TWODNS, nx, ny, Lx, Ly, T, stepper, dt, aliased_fraction)
     d.nu, d.nnu, d.mu, d.nmu = float(ν), int(nν), float(μ), int(nμ)
     Fh = attach_forcing(d, calcF)
-    p = TwoDNSProblem(d, make_grid(nx, ny, Lx, Ly, aliased_fraction, T), Params(float(ν), int(nν), float(μ), int(nμ), calcF, stepper))
+    p = TwoDNSProblem(d, Grid(int(nx), int(ny), Lx, Ly, aliased_fraction, T), Params(float(ν), int(nν), float(μ), int(nμ), calcF, stepper))
     if Fh is not None:
         p.set_forcing(Fh)
     return p

@@ -10,6 +10,9 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library through the C ABI)")
+    # the oracle's 2D FFTs on up to 16 host threads (full-size checks at 2048^2)
+    from oracle import fourierflows
+    fourierflows.set_fft_workers(min(16, os.cpu_count() or 1))
 
 
 @pytest.fixture(scope="session")

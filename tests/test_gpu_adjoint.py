@@ -46,6 +46,32 @@ def test_gradient_matches_oracle(gf, n, nsteps, ckpt):
     assert dnu == pytest.approx(dnur, rel=1e-9)
 
 
+def test_gradient_512_two_checkpoint_segments_matches_oracle(gf):
+    """The adjoint workload's grid (512^2, nu = 1e-3, nnu = 1, dt = 0.005) over 100 steps
+    with a checkpoint every 50: the first segment is recomputed from its checkpoint on the
+    second stream while the sweep runs the second.  Against the oracle's adjoint (gradient
+    1e-10, dJ/dnu 1e-9, J 1e-12), then central differences of the device loss on 8 random
+    directions (h = 1e-4, 1e-6 relative)."""
+    n, nsteps, ckpt = 512, 100, 50
+    prob, z0, target = _setup(gf, n, 1e-3, 512)
+    J, grad, dnu = gf.TwoDNavierStokes.adjoint_gradient(prob, target, nsteps, ckpt)
+    g = off.TwoDGrid(n, 2 * np.pi)
+    Jr, gr, dnur = oad.loss_and_gradient(z0, target, g, 1e-3, 1, 0.005, nsteps, checkpoint=ckpt)
+    assert J == pytest.approx(Jr, rel=1e-12)
+    assert np.linalg.norm(grad - gr) <= 1e-10 * np.linalg.norm(gr)
+    assert dnu == pytest.approx(dnur, rel=1e-9)
+    rng = np.random.default_rng(8)
+    h = 1e-4
+    for _ in range(8):
+        v = rng.standard_normal((n, n))
+        Js = []
+        for sgn in (1, -1):
+            gf.TwoDNavierStokes.set_ζ(prob, z0 + sgn * h * v)
+            Js.append(gf.TwoDNavierStokes.adjoint_gradient(prob, target, nsteps, nsteps, nu=False)[0])
+        fd = (Js[0] - Js[1]) / (2 * h)
+        assert fd == pytest.approx(float(np.sum(grad * v)), rel=1e-6)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient(gf):
     prob, z0, target = _setup(gf, 64, 1e-3, 5)
     base = gf.TwoDNavierStokes.adjoint_gradient(prob, target, 13, 13)

@@ -359,9 +359,11 @@ def test_mlqg_other_steppers(gf, stepper):
 
 
 # ----------------------------------------------------------------------------- full size
-def test_full_size_2048_forced_step_matches_oracle(gf):
-    """The benchmark workload (2048^2 FP64 ETDRK4, ring forcing): one step from a
-    random state against the oracle, and bit-exact determinism of the device path."""
+def test_full_size_2048_forced_steps_match_oracle(gf):
+    """The benchmark workload (2048^2 FP64 ETDRK4, ring forcing): 5 steps from a random
+    state in one call (4 interior steps on the retained columns, the step counter on the
+    device, then the full last step) against the oracle at 1e-12, then bit-exact
+    determinism of the device path."""
     n = 2048
     kw = dict(nx=n, Lx=TWO_PI, dt=0.005, stepper="ETDRK4")
     rng = np.random.default_rng(20)
@@ -371,14 +373,51 @@ def test_full_size_2048_forced_step_matches_oracle(gf):
     s0 = p.sol
     r = o2d.Problem(nu=1e-21, nnu=4, mu=0.1, nmu=0, calcF=("ring", 14.0, 1.5, 0.1, 2), **kw)
     r.sol[...] = s0
-    gf.stepforward(p, 1)
-    off.stepforward(r, 1)
+    gf.stepforward(p, 5)
+    off.stepforward(r, 5)
     assert rel(p.sol, r.sol) < 1e-12
-    s1 = p.sol
+    s5 = p.sol
     p.sol = s0
     p.set_clock(0.0, 0)
-    gf.stepforward(p, 1)
-    assert np.array_equal(p.sol, s1)
+    gf.stepforward(p, 5)
+    assert np.array_equal(p.sol, s5)
+
+
+def test_full_size_2048_fp32_shipped_schedule_matches_fp64_oracle(gf):
+    """The Float32 variant of the benchmark workload under the shipped tuned schedule
+    (P = (8, 8, 8), K1 field split with the per-transform address refresh that FP32 K1 uses
+    from 2048 points, K3 column split): 3 steps in one call against the FP64 oracle started
+    from the same (FP32-rounded) state.  Tolerance 2e-5 relative L2, as for the other FP32
+    checks: a few hundred FP32 roundings deep (eps 6e-8) per step."""
+    n = 2048
+    kw = dict(nx=n, Lx=TWO_PI, dt=0.005, stepper="ETDRK4")
+    z = np.random.default_rng(22).standard_normal((n, n))
+    p = gf.TwoDNavierStokes.Problem(gf.GPU(), ν=1e-21, nν=4, μ=0.1, nμ=0, T=np.float32,
+                                    calcF=gf.RingForcing(14.0, 1.5, 0.1, seed=2), **kw)
+    assert gf.schedule(p) == (8, 8, 8) and gf.splits(p) == (1, 0, 1)
+    gf.TwoDNavierStokes.set_ζ(p, z)
+    s0 = p.sol.astype(np.complex128)
+    r = o2d.Problem(nu=1e-21, nnu=4, mu=0.1, nmu=0, calcF=("ring", 14.0, 1.5, 0.1, 2), **kw)
+    r.sol[...] = s0
+    gf.stepforward(p, 3)
+    off.stepforward(r, 3)
+    assert np.isfinite(p.sol).all()
+    assert rel(p.sol, r.sol) < 2e-5
+
+
+@pytest.mark.parametrize("n", [16, 64])
+@pytest.mark.parametrize("f", [1 / 3, 1 / 2])
+def test_device_calcN_retained_modes_are_alias_free(gf, n, f):
+    """On every retained mode the device N equals the exact convolution of the retained
+    modes (tests/alias_free.py); at n = 16 and 64 with f = 1/3 this fails if the l band is
+    one index too narrow at its top."""
+    from tests.alias_free import exact_calcN
+    s = np.fft.rfft2(np.random.default_rng(n).standard_normal((n, n)))
+    p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, stepper="ETDRK4", aliased_fraction=f)
+    N, sm = p.calcN(s)
+    m = off.TwoDGrid(n, TWO_PI, aliased_fraction=f).dealias_mask()
+    E = exact_calcN(sm, n, n)
+    assert rel(N * m, E * m) < 1e-13
 
 
 def test_translation_equivariance_512(gf):

@@ -267,7 +267,9 @@ def test_mlqg_energies_closed_form():
     psi = np.array([np.cos(2 * x) * np.cos(2 * y), 0.5 * np.cos(2 * x) * np.cos(2 * y)])
     ML.set_psi(prob, psi)
     KE, PE = ML.energies(prob)
-    assert abs(KE[0] - 1 / 4) < 1e-13 and abs(KE[1] - 3 / 16) < 1e-13 and abs(PE[0] - 1 / 160) < 1e-13
+    ke1, ke2, pe = (float(Fraction(v)) for v in GOLD["mlqg_KE1_KE2_PE"]["value"])
+    assert (ke1, ke2, pe) == (1 / 4, 3 / 16, 1 / 160)
+    assert abs(KE[0] - ke1) < 1e-13 and abs(KE[1] - ke2) < 1e-13 and abs(PE[0] - pe) < 1e-13
 
 
 def test_mlqg_rossby_wave_single_layer():
