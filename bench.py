@@ -245,7 +245,7 @@ def adjoint_run(gf, nsteps=2000, checkpoint=50):
     return out
 
 
-def adjoint_batch_run(gf, nb=16, nsteps=2000, checkpoint=50, fd_dirs=8):
+def adjoint_batch_run(gf, nb=16, nsteps=2000, checkpoint=50, fd_dirs=8, graphs=False):
     """Config 5 as a batch: 16 independent 512^2 problems (own contexts and streams), their
     2000-step gradients computed concurrently from 16 host threads, against the same 16 forward
     runs done concurrently.  Then a central-difference check of one gradient on fd_dirs random
@@ -259,6 +259,8 @@ def adjoint_batch_run(gf, nb=16, nsteps=2000, checkpoint=50, fd_dirs=8):
         z0s.append(gf.peakedisotropicspectrum(grid, 6, 0.5, rng=rng))
         targets.append(gf.peakedisotropicspectrum(grid, 6, 0.5, rng=rng))
         probs.append(gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, ν=1e-3, nν=1, dt=0.005, stepper="ETDRK4"))
+        if graphs:
+            probs[-1].use_graphs(True)
     pool = ThreadPoolExecutor(max_workers=nb)
 
     def fwd(b):
@@ -348,6 +350,8 @@ def main():
     ap.add_argument("--no-adjoint-batch", action="store_true",
                     help="with --only-adjoint: the single-problem gradient only (no 16-thread leg)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the Float32 variant")
+    ap.add_argument("--graphs", action="store_true",
+                    help="with --only-adjoint: graph replay on for the 16 concurrent problems (each thread captures)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--no-stable", action="store_true", help="skip the dt = 0.001 run")
@@ -365,7 +369,7 @@ def main():
         gf = ge.load_package()
         out = {"adjoint_512": adjoint_run(gf)}
         if not args.no_adjoint_batch:
-            out["adjoint_512_batch16"] = adjoint_batch_run(gf)
+            out["adjoint_512_batch16"] = adjoint_batch_run(gf, graphs=args.graphs)
         print(json.dumps(out), flush=True)
         return
     if args.only_mlqg or args.only_decaying:

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -104,23 +105,75 @@ void etd_coeff_derivs(cd dtL, double dt, cd& dzeta, cd& dalpha, cd& dbeta, cd& d
   dzeta = f * sz; dalpha = f * sa; dbeta = f * sb; dgamma = f * sg;
 }
 
+// Device-wide runtime calls and stream capture.
+//
+// hipMalloc, hipFree, hipHostMalloc/hipHostFree, the synchronous hipMemcpy and the null-stream
+// hipMemset/hipStreamSynchronize, stream creation/destruction and graph-exec destruction may
+// synchronise the device or take runtime-wide locks. Contexts may be driven from several host
+// threads, each capturing its own stream into a HIP graph, and such a call on one thread must not
+// interleave with a capture on another. Every such call goes through the wrappers below, which
+// hold g_devwide shared; a capture (begin, the recorded launches, end, instantiate) holds it
+// exclusively (CaptureLock). Captures use hipStreamCaptureModeThreadLocal, so an unsafe call made
+// by the capturing thread itself fails instead of being recorded or blocking.
+// (The raw runtime calls appear only between the two DEVWIDE markers; a host test checks that.)
+static std::shared_mutex g_devwide;
+// DEVWIDE-BEGIN
+namespace dw {
+using SL = std::shared_lock<std::shared_mutex>;
+hipError_t malloc(void** p, size_t n) { SL l(g_devwide); return hipMalloc(p, n); }
+hipError_t free(void* p) { SL l(g_devwide); return hipFree(p); }
+hipError_t host_malloc(void** p, size_t n) { SL l(g_devwide); return hipHostMalloc(p, n, hipHostMallocDefault); }
+hipError_t host_free(void* p) { SL l(g_devwide); return hipHostFree(p); }
+hipError_t copy(void* dst, const void* src, size_t n, hipMemcpyKind k) { SL l(g_devwide); return hipMemcpy(dst, src, n, k); }
+// zero n bytes and wait: the contexts' streams are non-blocking, so a null-stream memset is not
+// ordered before their work (a late memset would clear what a kernel wrote)
+hipError_t zero_sync(void* p, size_t n) {
+  SL l(g_devwide);
+  hipError_t e = hipMemset(p, 0, n);
+  return e == hipSuccess ? hipStreamSynchronize(nullptr) : e;
+}
+hipError_t stream_create(hipStream_t* s) { SL l(g_devwide); return hipStreamCreateWithFlags(s, hipStreamNonBlocking); }
+hipError_t stream_destroy(hipStream_t s) { SL l(g_devwide); return hipStreamDestroy(s); }
+hipError_t exec_destroy(hipGraphExec_t g) { SL l(g_devwide); return hipGraphExecDestroy(g); }
+// stream -> graph exec, holding g_devwide exclusively; rec() records the launches
+template <typename F>
+hipGraphExec_t capture(hipStream_t s, F&& rec) {
+  std::unique_lock<std::shared_mutex> l(g_devwide);
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) throw GfError(GF_EHIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  try {
+    rec();
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &gr);
+    if (gr) (void)hipGraphDestroy(gr);
+    throw;
+  }
+  e = hipStreamEndCapture(s, &gr);
+  if (e != hipSuccess) throw GfError(GF_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  if (e != hipSuccess) throw GfError(GF_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return ex;
+}
+}  // namespace dw
+// DEVWIDE-END
+
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
   void alloc(size_t bytes) {
-    if (p) (void)hipFree(p);
+    if (p) (void)dw::free(p);
     p = nullptr; n = bytes;
     if (bytes) {
-      hipError_t e = hipMalloc(&p, bytes);
+      hipError_t e = dw::malloc(&p, bytes);
       if (e != hipSuccess) throw GfError(GF_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-      // the contexts' streams are non-blocking: the null-stream memset is not ordered before
-      // their work, so wait for it here (a late memset would clear what a kernel wrote)
-      e = hipMemset(p, 0, bytes);
-      if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+      e = dw::zero_sync(p, bytes);
       if (e != hipSuccess) throw GfError(GF_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
     }
   }
-  ~DevBuf() { if (p) (void)hipFree(p); }
+  ~DevBuf() { if (p) (void)dw::free(p); }
   template <typename X> X* as() const { return (X*)p; }
 };
 
@@ -203,6 +256,9 @@ struct CtxBase {
   virtual void adjoint_grad(const double*, int, int, double*, double*, double*) {
     throw GfError(GF_EUNSUPPORTED, "the adjoint is provided for TwoDNavierStokes FP64 ETDRK4 problems");
   }
+  // a step since the state was last set produced a non-finite value (read without waiting:
+  // reflects the steps that have finished)
+  virtual bool nonfinite() const { return false; }
   double t = 0;
   int64_t stepno = 0;
 };
@@ -245,12 +301,19 @@ struct Ctx final : CtxBase {
   double last_fscale = 1;
   std::vector<double> Hlay, gprime;   // MLQG layer depths and reduced gravities
   KernelTimer tm;
+  // non-finite flag of the state (pinned host memory the final stage of K3 writes 1 into); cleared
+  // whenever the state is set
+  int* nf_host = nullptr;
+  bool nonfinite() const override { return *(volatile const int*)nf_host != 0; }
+  void clear_nonfinite() { *(volatile int*)nf_host = 0; }
 
   bool own_stream = true;
   explicit Ctx(const gf_problem_desc& desc, hipStream_t shared = nullptr) : d(desc) {
     if (shared) { stream = shared; own_stream = false; }
-    else HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    else HIPCHECK(dw::stream_create(&stream));
     nl = d.model == GF_MODEL_MLQG ? d.nlayers : 1;
+    HIPCHECK(dw::host_malloc((void**)&nf_host, 64));
+    clear_nonfinite();
     const int nx = d.nx, ny = d.ny, nkr = nx / 2 + 1;
     g.nx = nx; g.ny = ny; g.nkr = nkr; g.nkr2 = (nkr + 1) & ~1; g.nlayers = nl;
     int lo_full, hi_full, lo_r, dum1, dum2;
@@ -287,19 +350,20 @@ struct Ctx final : CtxBase {
     HIPCHECK(hipStreamSynchronize(stream));
   }
   ~Ctx() override {
-    if (stream) { (void)hipStreamSynchronize(stream); if (own_stream) (void)hipStreamDestroy(stream); }
-    for (auto& ge : gexec) if (ge) (void)hipGraphExecDestroy(ge);
+    if (stream) { (void)hipStreamSynchronize(stream); if (own_stream) (void)dw::stream_destroy(stream); }
+    for (auto& ge : gexec) if (ge) (void)dw::exec_destroy(ge);
     if (rstream) {
       (void)hipStreamSynchronize(rstream);
-      (void)hipStreamDestroy(rstream);
+      (void)dw::stream_destroy(rstream);
       (void)hipEventDestroy(fwd_done);
       for (int k = 0; k < 2; ++k) { (void)hipEventDestroy(adj_done[k]); (void)hipEventDestroy(rec_done[k]); }
     }
     if (copy_stream) {
       (void)hipStreamSynchronize(copy_stream);
       for (int k = 0; k < 2; ++k) { (void)hipEventDestroy(snap_ready[k]); (void)hipEventDestroy(snap_done[k]); }
-      (void)hipStreamDestroy(copy_stream);
+      (void)dw::stream_destroy(copy_stream);
     }
+    if (nf_host) (void)dw::host_free(nf_host);
   }
 
   // ---------------- snapshots ----------------
@@ -316,7 +380,7 @@ struct Ctx final : CtxBase {
   int64_t snapshot(void* dst) override {
     const size_t bytes = sizeof(C) * (size_t)g.nkr * d.ny * nl;
     if (!copy_stream) {
-      HIPCHECK(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+      HIPCHECK(dw::stream_create(&copy_stream));
       for (int k = 0; k < 2; ++k) {
         HIPCHECK(hipEventCreateWithFlags(&snap_ready[k], hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&snap_done[k], hipEventDisableTiming));
@@ -348,7 +412,7 @@ struct Ctx final : CtxBase {
       const long double a = -2.0L * 3.141592653589793238462643383279502884L * i / n;
       h[i].re = (T)std::cos(a); h[i].im = (T)std::sin(a);
     }
-    HIPCHECK(hipMemcpy(b.p, h.data(), sizeof(C) * n, hipMemcpyHostToDevice));
+    HIPCHECK(dw::copy(b.p, h.data(), sizeof(C) * n, hipMemcpyHostToDevice));
   }
   double krw(int kr) const { return (double)kr * ((2 * M_PI / d.Lx * d.nx) / d.nx); }
   double lw(int l) const {
@@ -370,7 +434,7 @@ struct Ctx final : CtxBase {
         for (int kr = 0; kr < g.nkr; ++kr)
           h[(size_t)j * fsz + boff_h(kr, l, d.ny)] = full[((size_t)j * d.ny + l) * g.nkr + kr];
     b.alloc(sizeof(X) * h.size() + pad);
-    HIPCHECK(hipMemcpy(b.p, h.data(), sizeof(X) * h.size(), hipMemcpyHostToDevice));
+    HIPCHECK(dw::copy(b.p, h.data(), sizeof(X) * h.size(), hipMemcpyHostToDevice));
   }
   // device rfft2 of nf physical fields (x fastest) -> blocked spectral fields
   void fwd_device(const T* phys_dev, C* spec_dev, int nf, int zero_mean) {
@@ -406,11 +470,11 @@ struct Ctx final : CtxBase {
     const size_t np = (size_t)d.nx * d.ny;
     std::vector<T> ft(np);
     for (size_t i = 0; i < np; ++i) ft[i] = (T)f[i];
-    HIPCHECK(hipMemcpy(phys.p, ft.data(), sizeof(T) * np, hipMemcpyHostToDevice));
+    HIPCHECK(dw::copy(phys.p, ft.data(), sizeof(T) * np, hipMemcpyHostToDevice));
     fwd_device(phys.as<T>(), nout.as<C>(), 1, 0);
     std::vector<C> h(fsz);
     HIPCHECK(hipStreamSynchronize(stream));
-    HIPCHECK(hipMemcpy(h.data(), nout.p, sizeof(C) * fsz, hipMemcpyDeviceToHost));
+    HIPCHECK(dw::copy(h.data(), nout.p, sizeof(C) * fsz, hipMemcpyDeviceToHost));
     for (int l = 0; l < d.ny; ++l)
       for (int kr = 0; kr < g.nkr; ++kr) {
         C& z = h[boff_h(kr, l, d.ny)];
@@ -421,10 +485,10 @@ struct Ctx final : CtxBase {
         else { const T m2 = -(m * m); r.re = m2 * z.re; r.im = m2 * z.im; }  // -l^2
         z = r;
       }
-    HIPCHECK(hipMemcpy(nout.p, h.data(), sizeof(C) * fsz, hipMemcpyHostToDevice));
+    HIPCHECK(dw::copy(nout.p, h.data(), sizeof(C) * fsz, hipMemcpyHostToDevice));
     inv_device(nout.as<C>(), phys.as<T>(), 1);
     HIPCHECK(hipStreamSynchronize(stream));
-    HIPCHECK(hipMemcpy(ft.data(), phys.p, sizeof(T) * np, hipMemcpyDeviceToHost));
+    HIPCHECK(dw::copy(ft.data(), phys.p, sizeof(T) * np, hipMemcpyDeviceToHost));
     std::vector<double> out(np);
     for (size_t i = 0; i < np; ++i) out[i] = (double)ft[i];
     return out;
@@ -433,7 +497,7 @@ struct Ctx final : CtxBase {
     std::vector<T> h(v.size());
     for (size_t i = 0; i < v.size(); ++i) h[i] = (T)v[i];
     b.alloc(sizeof(T) * h.size());
-    HIPCHECK(hipMemcpy(b.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    HIPCHECK(dw::copy(b.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
   }
 
   std::vector<cd> Lop;   // [l][kr], one layer (MLQG L is the same in every layer)
@@ -481,7 +545,7 @@ struct Ctx final : CtxBase {
         // etah = rfft(eta): the -i kr U etah term and the enstrophy (src/singlelayerqg.jl:190)
         std::vector<T> ft(np);
         for (size_t i = 0; i < np; ++i) ft[i] = (T)etav[i];
-        HIPCHECK(hipMemcpy(phys.p, ft.data(), sizeof(T) * np, hipMemcpyHostToDevice));
+        HIPCHECK(dw::copy(phys.p, ft.data(), sizeof(T) * np, hipMemcpyHostToDevice));
         etah.alloc(sizeof(C) * fsz);
         fwd_device(phys.as<T>(), etah.as<C>(), 1, 0);
         HIPCHECK(hipStreamSynchronize(stream));
@@ -696,6 +760,7 @@ struct Ctx final : CtxBase {
     c.racc = racc.as<C>(); c.nout = nout.as<C>();
     c.c0 = coef[0].p; c.c1 = coef[1].p; c.c2 = coef[2].p; c.c3 = coef[3].p; c.c4 = coef[4].p; c.c5 = coef[5].p;
     c.ccplx = ccplx; c.cl_layered = 0; c.dt = (T)d.dt; c.filter = filtered() ? filter.as<T>() : nullptr;
+    c.nonfinite = nf_host;
     c.tw = twy.as<C>();
     a.p = sched[0]; b.p = sched[1]; c.p = sched[2];
     if (nl == 1) { a.fsplit = split[0]; b.hsplit = split[1]; c.csplit = split[2]; }
@@ -751,7 +816,7 @@ struct Ctx final : CtxBase {
     bool changed = false;
     for (int k = 0; k < 3; ++k) { changed |= split[k] != (v[k] != 0); split[k] = v[k] != 0; }
     if (changed)
-      for (auto& ge : gexec) if (ge) { HIPCHECK(hipStreamSynchronize(stream)); (void)hipGraphExecDestroy(ge); ge = nullptr; }
+      for (auto& ge : gexec) if (ge) { HIPCHECK(hipStreamSynchronize(stream)); (void)dw::exec_destroy(ge); ge = nullptr; }
   }
   void get_splits(int* v) const override { for (int k = 0; k < 3; ++k) v[k] = split[k]; }
   SchedKey sched_key() const { return SchedKey{d.model, nl, d.nx, d.ny, d.dtype}; }
@@ -764,7 +829,7 @@ struct Ctx final : CtxBase {
     bool changed = false;
     for (int k = 0; k < 3; ++k) { changed |= sched[k] != p[k]; sched[k] = p[k]; }
     if (changed)   // the captured steps hold the old launches
-      for (auto& ge : gexec) if (ge) { HIPCHECK(hipStreamSynchronize(stream)); (void)hipGraphExecDestroy(ge); ge = nullptr; }
+      for (auto& ge : gexec) if (ge) { HIPCHECK(hipStreamSynchronize(stream)); (void)dw::exec_destroy(ge); ge = nullptr; }
   }
   void get_schedule(int* p) const override {
     for (int k = 0; k < 3; ++k) p[k] = sched[k] ? sched[k] : (nl == 1 ? sched_default(sched_len(k)) : (d.ny >= 32 && k != 1 ? 4 : sched_default(sched_len(k))));
@@ -787,10 +852,11 @@ struct Ctx final : CtxBase {
     DevBuf save[4];
     DevBuf* keep[4] = {&sol, &racc, &n1, &prevsol};
     for (int j = 0; j < 4; ++j)
-      if (keep[j]->p) { save[j].alloc(cb); HIPCHECK(hipMemcpy(save[j].p, keep[j]->p, cb, hipMemcpyDeviceToDevice)); }
+      if (keep[j]->p) { save[j].alloc(cb); HIPCHECK(dw::copy(save[j].p, keep[j]->p, cb, hipMemcpyDeviceToDevice)); }
     const double t0 = t;
     const int64_t step0 = stepno;
     const bool pv0 = prevsol_valid;
+    const int nf0 = *nf_host;
     const int old[3] = {sched[0], sched[1], sched[2]};
     // plain launches while timing: a schedule change drops the captured graphs, and capturing
     // again inside a timed block would be charged to the schedule
@@ -853,8 +919,9 @@ struct Ctx final : CtxBase {
     cleanup();
     graphs = old_graphs;
     for (int j = 0; j < 4; ++j)
-      if (save[j].p) HIPCHECK(hipMemcpy(keep[j]->p, save[j].p, cb, hipMemcpyDeviceToDevice));
+      if (save[j].p) HIPCHECK(dw::copy(keep[j]->p, save[j].p, cb, hipMemcpyDeviceToDevice));
     t = t0; stepno = step0; prevsol_valid = pv0;
+    *(volatile int*)nf_host = nf0;
     int bc = -1;
     for (int c : combos) {
       if (us) us[c] = best_t[c];
@@ -895,19 +962,9 @@ struct Ctx final : CtxBase {
   void build_graph(int which) {
     const bool ring = d.forcing == GF_FORCING_RING;
     if (ring && !step_dev.p) step_dev.alloc(sizeof(int64_t));
-    hipGraph_t gr = nullptr;
-    HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
-    try {
+    gexec[which] = dw::capture(stream, [&] {
       for (int s = 0; s < (which ? GRAPH_BLOCK : 1); ++s) interior_launches(ring ? step_dev.as<int64_t>() : nullptr);
-    } catch (...) {
-      (void)hipStreamEndCapture(stream, &gr);
-      if (gr) (void)hipGraphDestroy(gr);
-      throw;
-    }
-    HIPCHECK(hipStreamEndCapture(stream, &gr));
-    const hipError_t e = hipGraphInstantiate(&gexec[which], gr, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(gr);
-    if (e != hipSuccess) { gexec[which] = nullptr; throw GfError(GF_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+    });
   }
   void set_graphs(int enable) override { graphs = enable != 0; }
 
@@ -961,6 +1018,7 @@ struct Ctx final : CtxBase {
     }
     launch_from_julia<T>(g, jl.as<C>(), sol.as<C>(), nl, stream);
     HIPCHECK(hipStreamSynchronize(stream));
+    clear_nonfinite();
   }
   void get_state(void* dst, hipMemcpyKind kind) override {
     const size_t bytes = sizeof(C) * (size_t)g.nkr * d.ny * nl;
@@ -974,6 +1032,7 @@ struct Ctx final : CtxBase {
     fwd_device(phys.as<T>(), sol.as<C>(), nl, d.model != GF_MODEL_SLQG);
     launch_mask<T>(g, sol.as<C>(), nl, stream);   // updatevars!' dealias!
     HIPCHECK(hipStreamSynchronize(stream));
+    clear_nonfinite();
   }
   int nvars() const override { return d.model == GF_MODEL_TWODNS ? 3 : 4; }
   void updatevars(void* out) override {
@@ -1213,7 +1272,7 @@ struct Ctx final : CtxBase {
     make_args(s2, E_ETD3, false, a, b, k);
     k.s1 = s1; k.s2 = s3; k.nsave = slot(i, 6, tp); run();
     make_args(s3, E_ETD4, false, a, b, k);
-    k.sol = s0; k.sol_out = next; k.nsave = slot(i, 7, tp); run();
+    k.sol = s0; k.sol_out = next; k.nsave = slot(i, 7, tp); k.nonfinite = nullptr; run();
     HIPCHECK(hipGetLastError());
   }
 
@@ -1290,7 +1349,7 @@ struct Ctx final : CtxBase {
         tape2_steps = seg;
       }
       if (pipe && !rstream) {
-        HIPCHECK(hipStreamCreateWithFlags(&rstream, hipStreamNonBlocking));
+        HIPCHECK(dw::stream_create(&rstream));
         HIPCHECK(hipEventCreateWithFlags(&fwd_done, hipEventDisableTiming));
         for (int k = 0; k < 2; ++k) {
           HIPCHECK(hipEventCreateWithFlags(&adj_done[k], hipEventDisableTiming));
@@ -1399,23 +1458,23 @@ struct BatchCtx final : CtxBase {
   KernelTimer tm;
 
   BatchCtx(const gf_problem_desc* ds, int n) {
-    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(dw::stream_create(&stream));
     try {
       inst.reserve(n);
       for (int i = 0; i < n; ++i) inst.emplace_back(new Ctx<T>(ds[i], stream));
       build_tables();
     } catch (...) {
       inst.clear();
-      (void)hipStreamDestroy(stream);
+      (void)dw::stream_destroy(stream);
       stream = nullptr;
       throw;
     }
   }
   ~BatchCtx() override {
     if (stream) (void)hipStreamSynchronize(stream);
-    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (gexec) (void)dw::exec_destroy(gexec);
     inst.clear();
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream) (void)dw::stream_destroy(stream);
   }
 
   bool k2_plain = true;   // every member's x pass is K2_ADVECT (no background gradients)
@@ -1443,7 +1502,7 @@ struct BatchCtx final : CtxBase {
           k2_plain = k2_plain && h[i].k2.mode == K2_ADVECT;
         }
         items[k][full].alloc(sizeof(BatchItem<T>) * n);
-        HIPCHECK(hipMemcpy(items[k][full].p, h.data(), sizeof(BatchItem<T>) * n, hipMemcpyHostToDevice));
+        HIPCHECK(dw::copy(items[k][full].p, h.data(), sizeof(BatchItem<T>) * n, hipMemcpyHostToDevice));
       }
       for (int kc = 0; kc < 3; ++kc) {
         std::vector<BatchWork> w;
@@ -1455,7 +1514,7 @@ struct BatchCtx final : CtxBase {
         }
         nwork[kc][full] = (int)w.size();
         work[kc][full].alloc(sizeof(BatchWork) * w.size());
-        HIPCHECK(hipMemcpy(work[kc][full].p, w.data(), sizeof(BatchWork) * w.size(), hipMemcpyHostToDevice));
+        HIPCHECK(dw::copy(work[kc][full].p, w.data(), sizeof(BatchWork) * w.size(), hipMemcpyHostToDevice));
       }
     }
   }
@@ -1480,19 +1539,7 @@ struct BatchCtx final : CtxBase {
   hipGraphExec_t gexec = nullptr;
   void set_graphs(int enable) override { graphs = enable != 0; }
   void build_graph() {
-    hipGraph_t gr = nullptr;
-    HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
-    try {
-      step_launches(0);
-    } catch (...) {
-      (void)hipStreamEndCapture(stream, &gr);
-      if (gr) (void)hipGraphDestroy(gr);
-      throw;
-    }
-    HIPCHECK(hipStreamEndCapture(stream, &gr));
-    const hipError_t e = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(gr);
-    if (e != hipSuccess) { gexec = nullptr; throw GfError(GF_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+    gexec = dw::capture(stream, [&] { step_launches(0); });
   }
 
   void step(int n) override {
@@ -1763,9 +1810,16 @@ int gf_set_forcing(gf_ctx* c, const void* F) {
   if (!F) return GF_EINVAL;
   return guarded(c, [&] { c->impl->set_forcing(F); });
 }
+static void check_finite(CtxBase* x) {
+  for (int i = 0; i < x->ninstances(); ++i)
+    if (x->instance(i)->nonfinite())
+      throw GfError(GF_ENOTFINITE, x->ninstances() > 1 ? "the state of ensemble member " + std::to_string(i) +
+                                                             " has non-finite values"
+                                                       : std::string("the state has non-finite values"));
+}
 int gf_step(gf_ctx* c, int n) {
   if (n < 0) return GF_EINVAL;
-  return guarded(c, [&] { c->impl->step(n); });
+  return guarded(c, [&] { c->impl->step(n); check_finite(c->impl); });
 }
 int gf_clock(const gf_ctx* c, double* t, int64_t* step) {
   if (!c) return GF_EINVAL;
@@ -1778,7 +1832,7 @@ int gf_set_clock(gf_ctx* c, double t, int64_t step) {
   c->impl->set_clock(t, step);
   return GF_OK;
 }
-int gf_synchronize(gf_ctx* c) { return guarded(c, [&] { c->impl->sync(); }); }
+int gf_synchronize(gf_ctx* c) { return guarded(c, [&] { c->impl->sync(); check_finite(c->impl); }); }
 int gf_timing(gf_ctx* c, int enable) { return guarded(c, [&] { c->impl->timing(enable); }); }
 int gf_timing_read(gf_ctx* c, int k, int64_t* n, double* ms) {
   if (!n || !ms) return GF_EINVAL;
@@ -1849,27 +1903,27 @@ int gf_snapshot_wait(gf_ctx* c, int64_t ticket) {
 }
 void* gf_host_alloc(size_t bytes) {
   void* p = nullptr;
-  if (bytes == 0 || hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+  if (bytes == 0 || dw::host_malloc(&p, bytes) != hipSuccess) return nullptr;
   return p;
 }
 void gf_host_free(void* p) {
-  if (p) (void)hipHostFree(p);
+  if (p) (void)dw::host_free(p);
 }
 void* gf_device_alloc(size_t bytes) {
   void* p = nullptr;
-  if (bytes == 0 || hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-  if (hipMemset(p, 0, bytes) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
-    (void)hipFree(p);
+  if (bytes == 0 || dw::malloc(&p, bytes) != hipSuccess) return nullptr;
+  if (dw::zero_sync(p, bytes) != hipSuccess) {
+    (void)dw::free(p);
     return nullptr;
   }
   return p;
 }
 void gf_device_free(void* p) {
-  if (p) (void)hipFree(p);
+  if (p) (void)dw::free(p);
 }
 int gf_device_copy(void* dst, const void* src, size_t bytes) {
   if (!dst || !src) return GF_EINVAL;
-  return hipMemcpy(dst, src, bytes, hipMemcpyDefault) == hipSuccess ? GF_OK : GF_EHIP;
+  return dw::copy(dst, src, bytes, hipMemcpyDefault) == hipSuccess ? GF_OK : GF_EHIP;
 }
 
 int gf_diagnostic_async(gf_ctx* c, int kind, double xi, int nxi, double* out_dev, int nout) {

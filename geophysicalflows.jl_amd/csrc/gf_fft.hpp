@@ -25,6 +25,7 @@ struct alignas(2 * sizeof(T)) cplx {
 
 template <typename T> __device__ __forceinline__ cplx<T> mk(T a, T b) { cplx<T> z; z.re = a; z.im = b; return z; }
 template <typename T> __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return mk(a.re + b.re, a.im + b.im); }
+template <typename T> __device__ __forceinline__ bool finite(cplx<T> a) { return __builtin_isfinite(a.re) && __builtin_isfinite(a.im); }
 template <typename T> __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return mk(a.re - b.re, a.im - b.im); }
 // complex product in the (ac - bd, ad + bc) form
 template <typename T> __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {

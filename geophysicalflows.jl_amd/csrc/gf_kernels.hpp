@@ -138,6 +138,9 @@ struct K3Args {
   // E_ETD4 (both nullable: no tape / update sol in place)
   cplx<T>* nsave;
   cplx<T>* sol_out;
+  // nullable: the final stage of a step writes 1 here (a plain store from the lanes that saw one)
+  // when a value of the new state is not finite; host memory, read by the context after its stream
+  int* nonfinite;
   const cplx<T>* tw;
 };
 

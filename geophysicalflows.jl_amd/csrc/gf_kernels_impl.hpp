@@ -728,6 +728,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
     fft_forward<T, NY, P, 1, (NY >= FreshFrom<T>::k3)>(B, L, t, tw);
     if (!active) continue;
     const long lo = (long)jl * g.lstride;
+    bool bad = false;   // a non-finite value in the new state (final stages)
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int l = t + TPT * q;
@@ -806,6 +807,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
           r = r + Ga.mul(ci, N);
           if (a.filter) r = rmul(at(a.filter, cf), r);
           at(a.sol_out != nullptr ? a.sol_out : a.sol, ix) = r;
+          bad |= !finite(r);
         } break;
         case E_RK1:
         case E_RK2:
@@ -831,6 +833,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
             cplx<T> r = sv + rmul(a.dt, acc);
             if (a.filter) r = rmul(at(a.filter, cf), r);
             at(a.sol, ix) = r;
+            bad |= !finite(r);
           }
         } break;
         case E_FE: {
@@ -838,6 +841,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
           cplx<T> r = sv + rmul(a.dt, N + Lc.mul(ci, sv));
           if (a.filter) r = rmul(at(a.filter, cf), r);
           at(a.sol, ix) = r;
+          bad |= !finite(r);
         } break;
         case E_AB3: {
           // RHS = N + L sol; forward Euler start-up, then 23/12 RHS - 16/12 RHS_1 + 5/12 RHS_2;
@@ -857,6 +861,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
           at(a.sol, ix) = r;
           at(a.n1, ix) = R1;
           at(a.racc, ix) = R;
+          bad |= !finite(r);
         } break;
         default: break;
       }
@@ -865,6 +870,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       // points in flight; with 4 the registers allow all points' loads to be in flight at once.
       if constexpr (P >= 8) __builtin_amdgcn_sched_barrier(0);
     }
+    if (bad && a.nonfinite != nullptr) *a.nonfinite = 1;
   }
 }
 

@@ -75,7 +75,7 @@ typedef struct gf_problem_desc {
   const double* H;           /* nlayers                                             */
   const double* b;           /* nlayers (Boussinesq buoyancy)                       */
   double topo_grad_x, topo_grad_y;   /* topographic_pv_gradient                     */
-  int linear;                /* MLQG linear equations (not supported: GF_EUNSUPPORTED) */
+  int linear;                /* MLQG linearised equations (src/multilayerqg.jl:358-362, calcNlinear!) */
   /* forcing (calcF / calcFq) */
   int forcing;               /* GF_FORCING_*                                        */
   double kf, dkf, eps;       /* ring spectrum (examples/twodnavierstokes_stochasticforcing.jl:53-66) */
@@ -130,6 +130,11 @@ int gf_set_forcing_harmonic(gf_ctx* ctx, const void* Fh, double omega, double ph
 
 /* [FF] stepforward!(prob, nsteps): nsteps of the configured stepper, queued on the context's
  * stream (the call returns before they finish; every read of the state waits for them).
+ * Failure detection: the last stage of every step flags a non-finite value of the new state.
+ * gf_step returns GF_ENOTFINITE (after queuing its steps) when a step that has already finished
+ * was flagged; gf_synchronize waits and then returns GF_ENOTFINITE if any step since the state
+ * was last set (gf_set_state, gf_set_state_dev, gf_set_physical) was flagged. No counterpart in
+ * the reference, which steps non-finite states on silently.
  * With graph replay on, every step but the call's last replays a HIP graph of the whole step
  * (all stages, or all stages of every ensemble member), captured at the first such step. */
 int gf_step(gf_ctx* ctx, int nsteps);
@@ -183,7 +188,7 @@ const char* gf_schedules_error(void);
 int gf_clock(const gf_ctx* ctx, double* t, int64_t* step);
 int gf_set_clock(gf_ctx* ctx, double t, int64_t step);
 
-/* Wait for all work queued on the context's stream. */
+/* Wait for all work queued on the context's stream; GF_ENOTFINITE as described at gf_step. */
 int gf_synchronize(gf_ctx* ctx);
 
 /* Per-kernel timing of gf_step with HIP events on the context's stream:

@@ -462,3 +462,29 @@ def test_mlqg_ring_forcing_matches_oracle(gf, n):
     gf.stepforward(p, 4)
     off.stepforward(r, 4)
     assert rel(p.sol, r.sol) < 1e-12
+
+
+# ----------------------------------------------------------------------------- failure detection
+def test_nonfinite_state_is_reported(gf):
+    """An over-large dt on a small grid drives the state to non-finite values: gf_synchronize
+    (and gf_step, once a finished step has flagged it) returns GF_ENOTFINITE; setting the state
+    clears the flag, and a stable run reports nothing."""
+    n = 32
+    z = 50.0 * np.random.default_rng(30).standard_normal((n, n))
+    p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, dt=5.0, stepper="RK4")
+    gf.TwoDNavierStokes.set_ζ(p, z)
+    with pytest.raises(gf._lib.GFError) as e:
+        for _ in range(200):
+            gf.stepforward(p, 1)
+        p.synchronize()
+    assert e.value.code == gf._lib.GF_ENOTFINITE
+    assert not np.isfinite(p.sol).all()
+    with pytest.raises(gf._lib.GFError):
+        p.synchronize()                     # the flag stays until the state is set
+    gf.TwoDNavierStokes.set_ζ(p, 1e-3 * z)
+    p.synchronize()
+    q = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=n, dt=1e-3, stepper="ETDRK4")
+    gf.TwoDNavierStokes.set_ζ(q, z / 50)
+    gf.stepforward(q, 20)
+    q.synchronize()
+    assert np.isfinite(q.sol).all()
