@@ -64,6 +64,26 @@ def measured_traffic(sched, splits):
         return {}
 
 
+# Steps before the untimed warm-up steps, so that the timed steps do not start on a GPU whose
+# clocks are still ramping up from idle (about 0.1 s of work at 2048^2)
+CLOCK_RAMP_STEPS = 300
+
+
+def _step(gf, prob, n):
+    """gf_step that accepts a non-finite state: the headline configuration (dt = 0.005 at 2048^2)
+    exceeds the advective CFL limit within about 200 steps, and the kernels' timing does not
+    depend on the values (the library reports it as GF_ENOTFINITE)."""
+    rc = gf._lib.lib.gf_step(prob._ctx, int(n))
+    if rc not in (gf._lib.GF_OK, gf._lib.GF_ENOTFINITE):
+        gf._lib.check(rc, prob._ctx)
+
+
+def _sync(gf, prob):
+    rc = gf._lib.lib.gf_synchronize(prob._ctx)
+    if rc not in (gf._lib.GF_OK, gf._lib.GF_ENOTFINITE):
+        gf._lib.check(rc, prob._ctx)
+
+
 def gpu_run(gf, steps, warmup, T, dt=CFG["dt"]):
     lib = gf._lib.lib
     prob = gf.TwoDNavierStokes.Problem(
@@ -71,18 +91,23 @@ def gpu_run(gf, steps, warmup, T, dt=CFG["dt"]):
         dt=dt, stepper="ETDRK4", T=T,
         calcF=gf.RingForcing(CFG["kf"], CFG["dkf"], CFG["eps"], seed=CFG["seed"]))
     gf.TwoDNavierStokes.set_ζ(prob, np.zeros((NX, NX)))
+    _step(gf, prob, CLOCK_RAMP_STEPS)
+    _sync(gf, prob)
+    gf.TwoDNavierStokes.set_ζ(prob, np.zeros((NX, NX)))
+    prob.set_clock(0.0, 0)
     if warmup:
-        gf.stepforward(prob, warmup)
-    prob.synchronize()
+        _step(gf, prob, warmup)
+    _sync(gf, prob)
     t0 = time.perf_counter()
-    gf.stepforward(prob, steps)
-    prob.synchronize()
+    _step(gf, prob, steps)
+    _sync(gf, prob)
     t1 = time.perf_counter()
     elapsed = t1 - t0
+    finite = bool(np.isfinite(prob.sol).all())
     # second pass with per-launch HIP events on the library's stream
     lib.gf_timing(prob._ctx, 1)
-    gf.stepforward(prob, steps)
-    prob.synchronize()
+    _step(gf, prob, steps)
+    _sync(gf, prob)
     kern = {}
     for k, name in enumerate(KERNELS):
         n = ctypes.c_int64()
@@ -96,7 +121,6 @@ def gpu_run(gf, steps, warmup, T, dt=CFG["dt"]):
         kern[name] = dict(launches=n.value, avg_us=avg_s * 1e6, bytes_per_launch=per_launch,
                           achieved_GBps=per_launch / avg_s / 1e9)
     lib.gf_timing(prob._ctx, 0)
-    finite = bool(np.isfinite(prob.sol).all())
     kern["schedule"] = list(gf.schedule(prob))
     kern["splits"] = list(gf.splits(prob))
     return elapsed, kern, finite
@@ -302,11 +326,24 @@ def adjoint_batch_run(gf, nb=16, nsteps=2000, checkpoint=50, fd_dirs=8, graphs=F
             "grads_finite": all(bool(np.isfinite(r[1]).all()) for r in res)}
 
 
-def cpu_baseline(nsteps=3):
-    """The oracle (NumPy + SciPy pocketfft with one worker per core) on the same workload."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(nsteps=20):
+    """The oracle (NumPy + SciPy pocketfft with one worker per core) on the same workload: 20
+    ETDRK4 steps after one warm-up step.  Workers: the cores this process may run on, at most 16
+    (a one-GPU box's share of its host, whatever nproc reports there)."""
     from oracle import fourierflows as off
     from oracle import twodnavierstokes as o2d
-    cores = min(os.cpu_count() or 1, 16)
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = min(nproc, 16)
     off.set_fft_workers(cores)
     r = o2d.Problem(nx=NX, Lx=CFG["Lx"], nu=CFG["nu"], nnu=CFG["nnu"], mu=CFG["mu"], nmu=CFG["nmu"], dt=CFG["dt"],
                     stepper="ETDRK4", calcF=("ring", CFG["kf"], CFG["dkf"], CFG["eps"], CFG["seed"]))
@@ -317,7 +354,7 @@ def cpu_baseline(nsteps=3):
     return dict(value=NX * NX * nsteps / dt, unit="grid-pt-steps/s", cores=cores, kind="port",
                 sample=f"{nsteps} ETDRK4 steps of the 2048^2 FP64 forced workload after 1 warm-up step; "
                        f"scipy.fft rfft2/irfft2 with workers={cores}, NumPy broadcasts single-threaded",
-                s_per_step=dt / nsteps)
+                s_per_step=dt / nsteps, nproc=nproc, os_cpu_count=os.cpu_count(), cpu_model=_cpu_model())
 
 
 def copy_probe(nbytes=1 << 30, reps=3):
@@ -427,6 +464,7 @@ def main():
         "step_model": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
                        "frac_of_peak": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "state_finite": finite,
+        "clock_ramp_steps": CLOCK_RAMP_STEPS,
     }
     if not args.no_stable:
         # at dt = 0.005 the forced 2048^2 flow exceeds the advective CFL limit within a few hundred

@@ -18,7 +18,7 @@ STEPPERS = {"ETDRK4": 0, "FilteredETDRK4": 1, "RK4": 2, "FilteredRK4": 3,
 GF_FORCING_NONE, GF_FORCING_FIXED, GF_FORCING_RING = 0, 1, 2
 (GF_DIAG_ENERGY, GF_DIAG_KINETIC_ENERGY, GF_DIAG_POTENTIAL_ENERGY, GF_DIAG_ENSTROPHY, GF_DIAG_PALINSTROPHY,
  GF_DIAG_ENERGY_DISSIPATION, GF_DIAG_ENSTROPHY_DISSIPATION, GF_DIAG_ENERGY_DRAG, GF_DIAG_ENSTROPHY_DRAG,
- GF_DIAG_ENERGY_WORK, GF_DIAG_ENSTROPHY_WORK, GF_DIAG_LAYER_ENERGIES) = range(12)
+ GF_DIAG_ENERGY_WORK, GF_DIAG_ENSTROPHY_WORK, GF_DIAG_LAYER_ENERGIES, GF_DIAG_FLUXES) = range(13)
 
 EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string",
            "gf_set_state", "gf_get_state", "gf_set_state_dev", "gf_get_state_dev",
@@ -30,7 +30,7 @@ EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string"
            "gf_device_alloc", "gf_device_free", "gf_device_copy", "gf_use_graphs",
            "gf_set_schedule", "gf_get_schedule", "gf_autotune", "gf_schedules_save", "gf_schedules_load",
            "gf_schedules_clear", "gf_schedules_error", "gf_set_splits", "gf_get_splits",
-           "gf_set_forcing_harmonic")
+           "gf_set_forcing_harmonic", "gf_set_psi")
 
 
 class ProblemDesc(ctypes.Structure):
@@ -79,6 +79,7 @@ def _load():
     for name in ("gf_set_state", "gf_get_state", "gf_set_state_dev", "gf_get_state_dev",
                  "gf_set_physical", "gf_updatevars", "gf_set_forcing"):
         getattr(lib, name).argtypes = [vp, vp]
+    lib.gf_set_psi.argtypes = [vp, vp]
     lib.gf_num_vars.argtypes = [vp]
     lib.gf_calcN.argtypes = [vp, vp, vp, ctypes.c_double]
     lib.gf_step.argtypes = [vp, ip]

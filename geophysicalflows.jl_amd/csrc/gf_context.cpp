@@ -224,6 +224,7 @@ struct CtxBase {
   virtual void set_state(const void* src, hipMemcpyKind kind) = 0;
   virtual void get_state(void* dst, hipMemcpyKind kind) = 0;
   virtual void set_physical(const void* f) = 0;
+  virtual void set_psi(const void*) { throw GfError(GF_EUNSUPPORTED, "set_psi is a MultiLayerQG entry point"); }
   virtual void updatevars(void* out) = 0;
   virtual void calcN(void* N, void* sol, double t) = 0;
   virtual void set_forcing(const void* Fh) = 0;
@@ -300,6 +301,7 @@ struct Ctx final : CtxBase {
   T cur_fscale = 1;
   double last_fscale = 1;
   std::vector<double> Hlay, gprime;   // MLQG layer depths and reduced gravities
+  std::vector<double> Fcpl;           // MLQG layer coupling F (n x n, row-major): S = -Krsq I + F
   KernelTimer tm;
   // non-finite flag of the state (pinned host memory the final stage of K3 writes 1 into); cleared
   // whenever the state is set
@@ -343,7 +345,7 @@ struct Ctx final : CtxBase {
     if (d.forcing == GF_FORCING_FIXED) Fh.alloc(sizeof(C) * fsz * nl);   // zero until gf_set_forcing
     if (nl > 1 && d.mu != 0) psib.alloc(sizeof(C) * fsz);
     Sched tuned;
-    if (nl == 1 && sched_lookup(sched_key(), tuned)) {
+    if (tunable() && sched_lookup(sched_key(), tuned)) {
       for (int k = 0; k < 3; ++k) sched[k] = tuned.p[k];
       for (int k = 0; k < 3; ++k) split[k] = tuned.split[k];
     }
@@ -572,7 +574,10 @@ struct Ctx final : CtxBase {
     for (int l = 0; l < ny; ++l)
       for (int kr = 0; kr < nkr; ++kr) Lop[(size_t)l * nkr + kr] = cd(-d.nu * ipow(krsq(kr, l), d.nnu), 0);
     Lop[0] = 0;
-    std::vector<double> H(d.H, d.H + n), b(d.b, d.b + n), gp(n - 1), Fm(n - 1), Fp(n - 1);
+    // one layer (SingleLayerParams, src/multilayerqg.jl:193-219): no H, b or coupling
+    std::vector<double> H = n > 1 ? std::vector<double>(d.H, d.H + n) : std::vector<double>{1.0};
+    std::vector<double> b = n > 1 ? std::vector<double>(d.b, d.b + n) : std::vector<double>{0.0};
+    std::vector<double> gp(n - 1), Fm(n - 1), Fp(n - 1);
     Hlay = H;
     for (int j = 0; j < n - 1; ++j) {
       gp[j] = b[j] - b[j + 1];
@@ -586,6 +591,7 @@ struct Ctx final : CtxBase {
       if (j < n - 1) F[j * n + j + 1] = Fp[j];
       if (j > 0) F[j * n + j - 1] = Fm[j - 1];
     }
+    Fcpl = F;
     // S^-1 = I / (-k^2 I + F), k^2 = 0 -> 1, S^-1[1,1] = 0
     std::vector<double> S((size_t)fsz * n * n, 0.0);
     for (int l = 0; l < ny; ++l)
@@ -627,7 +633,8 @@ struct Ctx final : CtxBase {
       for (int y = 0; y < ny; ++y) {
         double q = d.beta - Uyy[(size_t)j * ny + y];
         auto Uj = [&](int jj) { return U[(size_t)jj * ny + y]; };
-        if (j == 0) q -= Fp[0] * (Uj(1) - Uj(0));
+        if (n == 1) {}
+        else if (j == 0) q -= Fp[0] * (Uj(1) - Uj(0));
         else if (j == n - 1) q -= Fm[n - 2] * (Uj(n - 2) - Uj(n - 1));
         else q = q - Fp[j] * (Uj(j + 1) - Uj(j)) - Fm[j - 1] * (Uj(j - 1) - Uj(j));
         Qy[(size_t)j * ny + y] = q;
@@ -763,7 +770,7 @@ struct Ctx final : CtxBase {
     c.nonfinite = nf_host;
     c.tw = twy.as<C>();
     a.p = sched[0]; b.p = sched[1]; c.p = sched[2];
-    if (nl == 1) { a.fsplit = split[0]; b.hsplit = split[1]; c.csplit = split[2]; }
+    if (tunable()) { a.fsplit = split[0]; b.hsplit = split[1]; c.csplit = split[2]; }
     if (ab3()) {
       // the RHS history of the aliased band enters later steps: every AB3 step is computed in full
       c.ab3_start = stepno < 3;
@@ -811,8 +818,11 @@ struct Ctx final : CtxBase {
   // takes one column per workgroup instead of a column pair, where a column fills a wave
   // (k3_single_column): 683 workgroups at 4 per CU balance 256 CUs better than 342 at 2.
   int split[3] = {0, 0, 0};
+  // TwoDNavierStokes and SingleLayerQG take radix schedules and workgroup splits; MultiLayerQG
+  // (any number of layers) has a fixed schedule
+  bool tunable() const { return d.model != GF_MODEL_MLQG; }
   void set_splits(const int* v) override {
-    if (nl != 1 && (v[0] || v[1] || v[2])) throw GfError(GF_EINVAL, "workgroup splits are for single-layer problems");
+    if (!tunable() && (v[0] || v[1] || v[2])) throw GfError(GF_EINVAL, "workgroup splits are for TwoDNavierStokes and SingleLayerQG problems");
     bool changed = false;
     for (int k = 0; k < 3; ++k) { changed |= split[k] != (v[k] != 0); split[k] = v[k] != 0; }
     if (changed)
@@ -823,7 +833,7 @@ struct Ctx final : CtxBase {
   int sched_len(int k) const { return k == 1 ? d.nx : d.ny; }   // K2 transforms rows, K1/K3 columns
   void set_schedule(const int* p) override {
     for (int k = 0; k < 3; ++k)
-      if (p[k] != 0 && !(nl == 1 && sched_candidate(sched_len(k), p[k])))
+      if (p[k] != 0 && !(tunable() && sched_candidate(sched_len(k), p[k])))
         throw GfError(GF_EINVAL, "radix schedule: P = " + std::to_string(p[k]) + " is not offered for kernel " +
                                  std::to_string(k + 1) + " of this problem");
     bool changed = false;
@@ -846,7 +856,7 @@ struct Ctx final : CtxBase {
     if (steps < 1) throw GfError(GF_EINVAL, "gf_autotune: steps must be >= 1");
     constexpr int NC = 216;
     if (us) for (int j = 0; j < NC; ++j) us[j] = std::nan("");
-    if (nl != 1) return;   // MultiLayerQG keeps its fixed schedule
+    if (!tunable()) return;   // MultiLayerQG keeps its fixed schedule
     HIPCHECK(hipStreamSynchronize(stream));
     const size_t cb = sizeof(C) * fsz * nl;
     DevBuf save[4];
@@ -1034,6 +1044,25 @@ struct Ctx final : CtxBase {
     HIPCHECK(hipStreamSynchronize(stream));
     clear_nonfinite();
   }
+  // set_psi! (src/multilayerqg.jl:873-889): psih = rfft(psi); qh = S psih (pvfromstreamfunction!,
+  // 520-552); q = irfft(qh); then set_q!(q): rfft, qh[1, 1, :] = 0, sol = qh, updatevars!'s
+  // dealias!.  s1 and s2 (stage scratch between steps) hold psih and qh.
+  void set_psi(const void* f) override {
+    if (d.model != GF_MODEL_MLQG) throw GfError(GF_EUNSUPPORTED, "set_psi is a MultiLayerQG entry point");
+    const size_t np = (size_t)d.nx * d.ny * nl;
+    HIPCHECK(hipMemcpyAsync(phys.p, f, sizeof(T) * np, hipMemcpyHostToDevice, stream));
+    fwd_device(phys.as<T>(), s1.as<C>(), nl, 0);
+    PvArgs<T> a{};
+    a.g = g; a.nl = nl; a.psi = s1.as<C>(); a.q = s2.as<C>();
+    for (int j = 0; j < nl; ++j)
+      for (int k = 0; k < nl; ++k) a.F[j * MAXL + k] = (T)Fcpl[(size_t)j * nl + k];
+    launch_pv_from_psi<T>(a, stream);
+    inv_device(s2.as<C>(), phys.as<T>(), nl);
+    fwd_device(phys.as<T>(), sol.as<C>(), nl, 1);
+    launch_mask<T>(g, sol.as<C>(), nl, stream);
+    HIPCHECK(hipStreamSynchronize(stream));
+    clear_nonfinite();
+  }
   int nvars() const override { return d.model == GF_MODEL_TWODNS ? 3 : 4; }
   void updatevars(void* out) override {
     launch_mask<T>(g, sol.as<C>(), nl, stream);
@@ -1139,12 +1168,14 @@ struct Ctx final : CtxBase {
     const bool slqg = d.model == GF_MODEL_SLQG, twodns = d.model == GF_MODEL_TWODNS;
     auto bad = [&](const char* m) { throw GfError(GF_EUNSUPPORTED, m); };
     if (d.model == GF_MODEL_MLQG) {
-      if (kind != DIAG_LAYER_ENERGIES) bad("MultiLayerQG provides GF_DIAG_LAYER_ENERGIES");
+      if (kind != DIAG_LAYER_ENERGIES && kind != DIAG_FLUXES) bad("MultiLayerQG provides GF_DIAG_LAYER_ENERGIES and GF_DIAG_FLUXES");
       a.nout = 2 * nl - 1;
+      if (kind == DIAG_FLUXES) { scale = 1; return; }   // the factors are applied per point
       double Hs = 0;
       for (double h : Hlay) Hs += h;
       const double V = d.Lx * d.Ly * Hs;
       for (int j = 0; j < nl; ++j) a.lay_ke[j] = Hlay[j] / (2 * V);
+      if (nl == 1) a.lay_ke[0] = 1 / (2 * d.Lx * d.Ly);   // src/multilayerqg.jl:953-960
       for (int j = 0; j + 1 < nl; ++j) a.lay_pe[j] = d.f0 * d.f0 / (2 * V * gprime[j]);
       a.sinv = sinv.as<T>();
       scale = d.Lx * d.Ly / np2;
@@ -1181,9 +1212,41 @@ struct Ctx final : CtxBase {
   int diag_launch(const DiagArgs<T>& a) {
     const int nblocks = (int)std::min<long>(1024, (fsz + 255) / 256);
     if (!diag_part.p) diag_part.alloc(sizeof(double) * 1024 * DIAG_MAXOUT);
+    if (a.kind == DIAG_FLUXES && d.model == GF_MODEL_MLQG) {
+      flux_launch(diag_part.as<double>(), nblocks);
+      return nblocks;
+    }
     launch_diag<T>(a, diag_part.as<double>(), nblocks, stream);
     HIPCHECK(hipGetLastError());
     return nblocks;
+  }
+  // MultiLayerQG fluxes (src/multilayerqg.jl:977-1022) as per-block partials: lateral_1..n, then
+  // vertical_{3/2}..{n-1/2}.  fluxes calls updatevars!, which dealiases sol in place; the spectra
+  // of psi, v and du/dy go to s1, s2 and n23 (stage scratch between steps), their y transforms to
+  // the mixed fields, and the x sums are taken there by Parseval along x.
+  void flux_launch(double* part, int nblocks) {
+    launch_mask<T>(g, sol.as<C>(), nl, stream);
+    FluxArgs<T> f{};
+    f.g = g; f.nl = nl; f.sol = sol.as<C>(); f.sinv = nl > 1 ? sinv.as<T>() : nullptr;
+    f.f[0] = s1.as<C>(); f.f[1] = s2.as<C>(); f.f[2] = n23.as<C>(); f.fstride = fsz; f.Uy = Uy.as<T>();
+    double Hs = 0;
+    for (double h : Hlay) Hs += h;
+    const double dxdy = (d.Lx / d.nx) * (d.Ly / d.ny);
+    const double V = nl > 1 ? d.Lx * d.Ly * Hs : d.Lx * d.Ly;
+    for (int j = 0; j < nl; ++j) f.fac[j] = (nl > 1 ? Hlay[j] : 1.0) * dxdy / V * d.nx;
+    for (int j = 0; j + 1 < nl; ++j) f.fac[nl + j] = d.f0 * d.f0 / gprime[j] * dxdy / V * d.nx;
+    launch_flux_fields<T>(f, stream);
+    for (int k = 0; k < 3; ++k)
+      for (int j = 0; j < nl; ++j) {
+        K1Args<T> a{};
+        a.g = g; a.g.nlayers = 1; a.model = GF_MODEL_TWODNS; a.fieldset = FS_RAW;
+        a.sol = f.f[k] + (size_t)j * fsz; a.out = mixed.as<C>() + (size_t)(k * nl + j) * fsz; a.fstride = fsz;
+        a.ncols = g.nkr; a.tw = twy.as<C>();
+        launch_k1<T>(d.ny, a, stream);
+      }
+    for (int k = 0; k < 3; ++k) f.f[k] = mixed.as<C>() + (size_t)k * nl * fsz;
+    launch_flux<T>(f, part, nblocks, stream);
+    HIPCHECK(hipGetLastError());
   }
   int diagnostic_async(int kind, double xi, int nxi, double* out_dev, int nout) override {
     DiagArgs<T> a{};
@@ -1598,8 +1661,8 @@ int validate(const gf_problem_desc* d, std::string& msg) {
   if (d->stepper < 0 || d->stepper > 7) { msg = "unknown stepper"; return GF_EINVAL; }
   if (d->aliased_fraction < 0 || d->aliased_fraction >= 1) { msg = "aliased_fraction must be in [0, 1)"; return GF_EINVAL; }
   if (d->model == GF_MODEL_MLQG) {
-    if (d->nlayers < 2 || d->nlayers > 4) { msg = "MultiLayerQG: nlayers must be 2..4 in this build"; return GF_EUNSUPPORTED; }
-    if (!d->H || !d->b) { msg = "MultiLayerQG: H and b are required"; return GF_EINVAL; }
+    if (d->nlayers < 1 || d->nlayers > MAXL) { msg = "MultiLayerQG: nlayers must be 1..8 in this build"; return GF_EUNSUPPORTED; }
+    if (d->nlayers > 1 && (!d->H || !d->b)) { msg = "MultiLayerQG: H and b are required"; return GF_EINVAL; }
     for (int j = 0; j + 1 < d->nlayers; ++j)
       if (d->b[j] - d->b[j + 1] == 0) { msg = "MultiLayerQG: equal buoyancy in adjacent layers"; return GF_EINVAL; }
   } else if (d->nlayers != 1 && d->nlayers != 0) {
@@ -1792,6 +1855,10 @@ int gf_get_state_dev(gf_ctx* c, void* s) {
 int gf_set_physical(gf_ctx* c, const void* f) {
   if (!f) return GF_EINVAL;
   return guarded(c, [&] { c->impl->set_physical(f); });
+}
+int gf_set_psi(gf_ctx* c, const void* psi) {
+  if (!c || !psi) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->set_psi(psi); });
 }
 int gf_updatevars(gf_ctx* c, void* out) {
   if (!out) return GF_EINVAL;

@@ -150,7 +150,128 @@ void launch_diag_finish(const double* part, int nblocks, int nout, double scale,
   hipLaunchKernelGGL(diag_finish_kernel, dim3(1), dim3(64), 0, s, part, nblocks, nout, scale, out);
 }
 
+// ---- MultiLayerQG fluxes (src/multilayerqg.jl:977-1022) ----
+
+// Spectra of psi, v and du/dy per layer from the dealiased state (fluxes calls updatevars!,
+// which dealiases sol): psih = S^-1 qh (one layer: -invKrsq qh), vh = i kr psih, uh = -i l psih,
+// (du/dy)h = i l uh.
+template <typename T>
+__global__ __launch_bounds__(DIAG_THREADS) void flux_fields_kernel(const FluxArgs<T> a) {
+  const Geo<T>& g = a.g;
+  const int ny = g.ny, n = a.nl;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.fstride) return;
+  const int kr = 2 * (int)(i / (2 * ny)) + (int)(i & 1);
+  const int l = (int)((i >> 1) % ny);
+  if (kr >= g.nkr) return;
+  const T kv = krwave(g, kr), lv = lwave(g, l);
+  const T K2 = kv * kv + lv * lv;
+  const T iK = (kr == 0 && l == 0) ? (T)0 : (T)1 / K2;
+  for (int j = 0; j < n; ++j) {
+    cplx<T> ps = czero<T>();
+    if (a.sinv == nullptr) {
+      ps = rmul(-iK, a.sol[i]);
+    } else {
+      const T* m = a.sinv + i * (n * n) + j * n;
+      for (int k = 0; k < n; ++k) ps = ps + rmul(m[k], a.sol[k * a.fstride + i]);
+    }
+    const cplx<T> vh = mk(-kv * ps.im, kv * ps.re);
+    const cplx<T> uh = mk(lv * ps.im, -lv * ps.re);
+    const cplx<T> uyh = mk(-lv * uh.im, lv * uh.re);
+    a.f[0][(long)j * a.fstride + i] = ps;
+    a.f[1][(long)j * a.fstride + i] = vh;
+    a.f[2][(long)j * a.fstride + i] = uyh;
+  }
+}
+
+// x sums by Parseval along x: for real rows f, g with mixed spectra (y transformed, x not)
+// m_f(kr, y), sum_x f g = nx sum_kr w_kr Re(m_f conj(m_g)), w = 2 for 0 < kr < nx/2; the kr = 0 and
+// kr = nx/2 columns enter through their real parts only, as the c2r transform reads them.
+template <typename T>
+__global__ __launch_bounds__(DIAG_THREADS) void flux_kernel(const FluxArgs<T> a, double* __restrict__ part) {
+  const Geo<T>& g = a.g;
+  const int ny = g.ny, n = a.nl, nout = 2 * n - 1;
+  double acc[DIAG_MAXOUT];
+#pragma unroll
+  for (int k = 0; k < DIAG_MAXOUT; ++k) acc[k] = 0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.fstride; i += (long)gridDim.x * blockDim.x) {
+    const int kr = 2 * (int)(i / (2 * ny)) + (int)(i & 1);
+    const int y = (int)((i >> 1) % ny);
+    if (kr >= g.nkr) continue;
+    const bool edge = kr == 0 || 2 * kr == g.nx;
+    const double w = edge ? 1.0 : 2.0;
+    auto prod = [&](int fa, int ja, int fb, int jb) -> double {
+      const cplx<T> p = a.f[fa][(long)ja * a.fstride + i], q = a.f[fb][(long)jb * a.fstride + i];
+      return edge ? (double)p.re * (double)q.re : rdot(p, q);
+    };
+    for (int j = 0; j < n; ++j)   // lateral_j: U_j v_j du_j/dy
+      acc[j] += w * a.fac[j] * (double)a.Uy[j * ny + y] * prod(1, j, 2, j);
+    for (int j = 0; j + 1 < n; ++j)   // vertical_{j+1/2}: (U_j - U_{j+1}) v_{j+1} psi_j
+      acc[n + j] += w * a.fac[n + j] * ((double)a.Uy[j * ny + y] - (double)a.Uy[(j + 1) * ny + y]) *
+                    prod(1, j + 1, 0, j);
+  }
+  __shared__ double red[DIAG_THREADS];
+  for (int k = 0; k < nout; ++k) {
+    red[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = DIAG_THREADS / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long)blockIdx.x * nout + k] = red[0];
+    __syncthreads();
+  }
+}
+
+template <typename T>
+void launch_flux_fields(const FluxArgs<T>& a, hipStream_t s) {
+  hipLaunchKernelGGL(flux_fields_kernel<T>, dim3((unsigned)((a.fstride + DIAG_THREADS - 1) / DIAG_THREADS)),
+                     dim3(DIAG_THREADS), 0, s, a);
+}
+template <typename T>
+void launch_flux(const FluxArgs<T>& a, double* part, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(flux_kernel<T>, dim3(nblocks), dim3(DIAG_THREADS), 0, s, a, part);
+}
+
+// ---- pvfromstreamfunction! (src/multilayerqg.jl:520-552) ----
+// q_j = sum_k S[j, k] psi_k, S = -Krsq I + F, summed in k order as the static-matrix product does;
+// one layer: q = -Krsq psi.
+template <typename T>
+__global__ __launch_bounds__(DIAG_THREADS) void pv_from_psi_kernel(const PvArgs<T> a) {
+  const Geo<T>& g = a.g;
+  const int ny = g.ny, n = a.nl;
+  const long fs = g.lstride;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= fs) return;
+  const int kr = 2 * (int)(i / (2 * ny)) + (int)(i & 1);
+  const int l = (int)((i >> 1) % ny);
+  if (kr >= g.nkr) return;
+  const T kv = krwave(g, kr), lv = lwave(g, l);
+  const T K2 = kv * kv + lv * lv;
+  cplx<T> ps[MAXL];
+  for (int k = 0; k < n; ++k) ps[k] = a.psi[k * fs + i];
+  for (int j = 0; j < n; ++j) {
+    cplx<T> q = czero<T>();
+    for (int k = 0; k < n; ++k) {
+      const T s = k == j ? -K2 + a.F[j * MAXL + k] : a.F[j * MAXL + k];
+      q = q + rmul(s, ps[k]);
+    }
+    a.q[j * fs + i] = q;
+  }
+}
+template <typename T>
+void launch_pv_from_psi(const PvArgs<T>& a, hipStream_t s) {
+  hipLaunchKernelGGL(pv_from_psi_kernel<T>, dim3((unsigned)((a.g.lstride + DIAG_THREADS - 1) / DIAG_THREADS)),
+                     dim3(DIAG_THREADS), 0, s, a);
+}
+
 template void launch_diag<double>(const DiagArgs<double>&, double*, int, hipStream_t);
 template void launch_diag<float>(const DiagArgs<float>&, double*, int, hipStream_t);
+template void launch_flux_fields<double>(const FluxArgs<double>&, hipStream_t);
+template void launch_flux_fields<float>(const FluxArgs<float>&, hipStream_t);
+template void launch_flux<double>(const FluxArgs<double>&, double*, int, hipStream_t);
+template void launch_flux<float>(const FluxArgs<float>&, double*, int, hipStream_t);
+template void launch_pv_from_psi<double>(const PvArgs<double>&, hipStream_t);
+template void launch_pv_from_psi<float>(const PvArgs<float>&, hipStream_t);
 
 }  // namespace gf

@@ -1,27 +1,16 @@
-// Instantiation unit: K1 for float (sizes 16..2048, 1..4 layers).
+// Instantiation unit: K1 for float (sizes 16..2048, 1..4 layers; 5..8 layers in gf_k1_fm.hip).
 #include "gf_kernels_impl.hpp"
+#include "gf_inst.hpp"
 namespace gf {
-template <int NL>
-static void k1_nl_f(int n, const K1Args<float>& a, hipStream_t s) {
-  switch (n) {
-    case 16: k1_n<float, 16, NL>(a, s); break;
-    case 32: k1_n<float, 32, NL>(a, s); break;
-    case 64: k1_n<float, 64, NL>(a, s); break;
-    case 128: k1_n<float, 128, NL>(a, s); break;
-    case 256: k1_n<float, 256, NL>(a, s); break;
-    case 512: k1_n<float, 512, NL>(a, s); break;
-    case 1024: k1_n<float, 1024, NL>(a, s); break;
-    case 2048: k1_n<float, 2048, NL>(a, s); break;
-    default: break;
-  }
-}
+void k1_deep_f(int nl, int n, const K1Args<float>& a, hipStream_t s);
 template <> void launch_k1<float>(int n, const K1Args<float>& a, hipStream_t s) {
-  switch (a.model == M_MLQG ? a.g.nlayers : 1) {
-    case 1: k1_nl_f<1>(n, a, s); break;
-    case 2: k1_nl_f<2>(n, a, s); break;
-    case 3: k1_nl_f<3>(n, a, s); break;
-    case 4: k1_nl_f<4>(n, a, s); break;
-    default: break;
+  const int nl = a.model == M_MLQG ? a.g.nlayers : 1;
+  switch (nl) {
+    case 1: GF_SIZES(k1_n, float, 1, n, a, s); break;
+    case 2: GF_SIZES(k1_n, float, 2, n, a, s); break;
+    case 3: GF_SIZES(k1_n, float, 3, n, a, s); break;
+    case 4: GF_SIZES(k1_n, float, 4, n, a, s); break;
+    default: k1_deep_f(nl, n, a, s); break;
   }
 }
 }  // namespace gf

@@ -1,29 +1,17 @@
-// Instantiation unit: K2 for double (sizes 16..2048, 1..4 layers).
+// Instantiation unit: K2 for double (sizes 16..2048, 1..4 layers; 5..8 layers in gf_k2_dm.hip).
 #include "gf_kernels_impl.hpp"
+#include "gf_inst.hpp"
 namespace gf {
-template <int NL>
-static void k2_nl_d(int n, const K2Args<double>& a, hipStream_t s) {
-  switch (n) {
-    case 16: k2_n<double, 16, NL>(a, s); break;
-    case 32: k2_n<double, 32, NL>(a, s); break;
-    case 64: k2_n<double, 64, NL>(a, s); break;
-    case 128: k2_n<double, 128, NL>(a, s); break;
-    case 256: k2_n<double, 256, NL>(a, s); break;
-    case 512: k2_n<double, 512, NL>(a, s); break;
-    case 1024: k2_n<double, 1024, NL>(a, s); break;
-    case 2048: k2_n<double, 2048, NL>(a, s); break;
-    default: break;
-  }
-}
+void k2_deep_d(int nl, int n, const K2Args<double>& a, hipStream_t s);
 // Inverse-out and forward-in treat every layer as one more field (NL = 1).
 template <> void launch_k2<double>(int n, const K2Args<double>& a, hipStream_t s) {
-  const bool adv = a.mode == K2_ADVECT || a.mode == K2_ADVECT_FULL;
-  switch (adv ? a.g.nlayers : 1) {
-    case 1: k2_nl_d<1>(n, a, s); break;
-    case 2: k2_nl_d<2>(n, a, s); break;
-    case 3: k2_nl_d<3>(n, a, s); break;
-    case 4: k2_nl_d<4>(n, a, s); break;
-    default: break;
+  const int nl = (a.mode == K2_ADVECT || a.mode == K2_ADVECT_FULL) ? a.g.nlayers : 1;
+  switch (nl) {
+    case 1: GF_SIZES(k2_n, double, 1, n, a, s); break;
+    case 2: GF_SIZES(k2_n, double, 2, n, a, s); break;
+    case 3: GF_SIZES(k2_n, double, 3, n, a, s); break;
+    case 4: GF_SIZES(k2_n, double, 4, n, a, s); break;
+    default: k2_deep_d(nl, n, a, s); break;
   }
 }
 }  // namespace gf

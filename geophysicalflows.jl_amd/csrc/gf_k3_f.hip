@@ -1,27 +1,16 @@
-// Instantiation unit: K3 for float (sizes 16..2048, 1..4 layers).
+// Instantiation unit: K3 for float (sizes 16..2048, 1..4 layers; 5..8 layers in gf_k3_fm.hip).
 #include "gf_kernels_impl.hpp"
+#include "gf_inst.hpp"
 namespace gf {
-template <int NL>
-static void k3_nl_f(int n, const K3Args<float>& a, hipStream_t s) {
-  switch (n) {
-    case 16: k3_n<float, 16, NL>(a, s); break;
-    case 32: k3_n<float, 32, NL>(a, s); break;
-    case 64: k3_n<float, 64, NL>(a, s); break;
-    case 128: k3_n<float, 128, NL>(a, s); break;
-    case 256: k3_n<float, 256, NL>(a, s); break;
-    case 512: k3_n<float, 512, NL>(a, s); break;
-    case 1024: k3_n<float, 1024, NL>(a, s); break;
-    case 2048: k3_n<float, 2048, NL>(a, s); break;
-    default: break;
-  }
-}
+void k3_deep_f(int nl, int n, const K3Args<float>& a, hipStream_t s);
 template <> void launch_k3<float>(int n, const K3Args<float>& a, hipStream_t s) {
-  switch (a.model == M_MLQG ? a.g.nlayers : 1) {
-    case 1: k3_nl_f<1>(n, a, s); break;
-    case 2: k3_nl_f<2>(n, a, s); break;
-    case 3: k3_nl_f<3>(n, a, s); break;
-    case 4: k3_nl_f<4>(n, a, s); break;
-    default: break;
+  const int nl = a.model == M_MLQG ? a.g.nlayers : 1;
+  switch (nl) {
+    case 1: GF_SIZES(k3_n, float, 1, n, a, s); break;
+    case 2: GF_SIZES(k3_n, float, 2, n, a, s); break;
+    case 3: GF_SIZES(k3_n, float, 3, n, a, s); break;
+    case 4: GF_SIZES(k3_n, float, 4, n, a, s); break;
+    default: k3_deep_f(nl, n, a, s); break;
   }
 }
 }  // namespace gf

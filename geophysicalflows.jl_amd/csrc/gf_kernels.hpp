@@ -199,7 +199,7 @@ enum DiagKind {
   DIAG_ENERGY = 0, DIAG_KINETIC_ENERGY = 1, DIAG_POTENTIAL_ENERGY = 2, DIAG_ENSTROPHY = 3,
   DIAG_PALINSTROPHY = 4, DIAG_ENERGY_DISSIPATION = 5, DIAG_ENSTROPHY_DISSIPATION = 6,
   DIAG_ENERGY_DRAG = 7, DIAG_ENSTROPHY_DRAG = 8, DIAG_ENERGY_WORK = 9, DIAG_ENSTROPHY_WORK = 10,
-  DIAG_LAYER_ENERGIES = 11
+  DIAG_LAYER_ENERGIES = 11, DIAG_FLUXES = 12
 };
 constexpr int DIAG_MAXOUT = 2 * MAXL - 1;
 
@@ -221,6 +221,39 @@ struct DiagArgs {
 };
 template <typename T> void launch_diag(const DiagArgs<T>& a, double* part, int nblocks, hipStream_t s);
 void launch_diag_finish(const double* part, int nblocks, int nout, double scale, double* out, hipStream_t s);
+
+// MultiLayerQG fluxes (src/multilayerqg.jl:977-1022).  Step 1 (flux_fields): from the dealiased
+// state, per layer j, the spectra psih_j (S^-1 q), vh_j = i kr psih_j and (du/dy)h_j = i l uh_j with
+// uh_j = -i l psih_j, written to layer j of f[0], f[1], f[2].  Step 2: those fields are
+// transformed along y (K1, raw).  Step 3 (flux_kernel): the x sums of the products by Parseval
+// along x on the mixed fields, weighted by U(y): lateral_j ~ U_j v_j du_j/dy, vertical_{j+1/2} ~
+// (U_j - U_{j+1}) v_{j+1} psi_j; fac[o] holds each output's constant factor.
+template <typename T>
+struct FluxArgs {
+  Geo<T> g;
+  int nl;
+  const cplx<T>* sol;       // dealiased state
+  const T* sinv;            // S^-1 per mode (n n reals), nullable for one layer (psih = -invKrsq q)
+  cplx<T>* f[3];            // psi, v, du/dy, nl layers each at stride fstride: spectral fields
+                            // (flux_fields) or their y transforms (flux_kernel)
+  long fstride;
+  const T* Uy;              // nl ny
+  double fac[DIAG_MAXOUT];
+};
+template <typename T> void launch_flux_fields(const FluxArgs<T>& a, hipStream_t s);
+template <typename T> void launch_flux(const FluxArgs<T>& a, double* part, int nblocks, hipStream_t s);
+
+// pvfromstreamfunction! (src/multilayerqg.jl:520-552): q_j = sum_k S[j, k] psi_k per mode with
+// S = -Krsq I + F (calcS!, 603-613); one layer: q = -Krsq psi.  Blocked spectral fields.
+template <typename T>
+struct PvArgs {
+  Geo<T> g;
+  int nl;
+  const cplx<T>* psi;
+  cplx<T>* q;
+  T F[MAXL * MAXL];         // layer coupling, row-major
+};
+template <typename T> void launch_pv_from_psi(const PvArgs<T>& a, hipStream_t s);
 
 template <typename T> void launch_k1(int ny, const K1Args<T>& a, hipStream_t s);
 template <typename T> void launch_k2(int nx, const K2Args<T>& a, hipStream_t s);

@@ -149,7 +149,7 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
   const Geo<T>& g = a.g;
   const bool active = kr < a.ncols && kr < g.nkr;
   // the linearised MultiLayerQG calcN! does not dealias its input (src/multilayerqg.jl:675-683)
-  const bool masked = FS != FS_RAW && !(NL > 1 && FS == FS_CALCN && a.nomask);
+  const bool masked = FS != FS_RAW && !((NL > 1 || MODEL == M_SLQG) && FS == FS_CALCN && a.nomask);
   const bool krk = kr < g.kmax || !masked;
   const T kv = krwave(g, kr);
   cplx<T>* L = col_buffer<T, NY>(lds, m.slot, m.c);
@@ -620,8 +620,10 @@ __device__ __forceinline__ int single_column(int b) { return 2 * (8 * (b / 16) +
 
 // Body of K3 for block blk of column pairs (CPW = 2, G pairs per workgroup) or, with CPW = 1,
 // for the single column single_column(blk) (G = 1).  EPI >= 0 fixes the stage epilogue at
-// compile time (it must equal a.epi); EPI = -1 switches on a.epi.
-template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1>
+// compile time (it must equal a.epi); EPI = -1 switches on a.epi.  ML1: a one-layer MultiLayerQG
+// problem (SingleLayerParams), whose bottom drag mu Krsq psih uses psih = -invKrsq qh of the calcN
+// input (src/multilayerqg.jl:658, 547-551) and whose linearised equations step every mode.
+template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1, bool ML1 = false>
 __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __restrict__ lds,
                                         const cplx<T>* __restrict__ tw) {
   constexpr int TPT = NY / P;
@@ -732,7 +734,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int l = t + TPT * q;
-      const bool kept = (kept_l(g, l) && kr < g.kmax) || (NL > 1 && a.nomask);
+      const bool kept = (kept_l(g, l) && kr < g.kmax) || ((NL > 1 || ML1) && a.nomask);
       if (!kept && !a.full) continue;
       const T lv = lwave(g, l);
       const uint32_t idx = boff32(kr, l, NY);
@@ -756,6 +758,16 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
           // bottom drag N_n += mu Krsq psih_n (src/multilayerqg.jl:658)
           const T K2 = kv * kv + lv * lv;
           N = N + rmul(a.mu_drag * K2, psin[q]);
+        }
+      }
+      if constexpr (ML1) {
+        if (a.drag_src != nullptr) {
+          // one layer: psih = -invKrsq qh of the (masked) calcN input, read before this point of
+          // the in-place stages is written; N += mu Krsq psih
+          const T K2 = kv * kv + lv * lv;
+          const T iK = (kr == 0 && l == 0) ? (T)0 : (T)1 / K2;
+          const cplx<T> ps = kept ? rmul(-iK, at(a.drag_src, ix)) : czero<T>();
+          N = N + rmul(a.mu_drag * K2, ps);
         }
       }
       if (a.Fh != nullptr) N = N + rmul(a.fscale, at(a.Fh, ix));
@@ -870,7 +882,10 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       // points in flight; with 4 the registers allow all points' loads to be in flight at once.
       if constexpr (P >= 8) __builtin_amdgcn_sched_barrier(0);
     }
-    if (bad && a.nonfinite != nullptr) *a.nonfinite = 1;
+    // one store per wave (the lowest lane that saw a non-finite value): with a non-finite state
+    // every lane would otherwise write the flag in host memory
+    const unsigned long long badm = __ballot(bad);
+    if (badm != 0 && a.nonfinite != nullptr && (int)(threadIdx.x & 63) == __ffsll(badm) - 1) *a.nonfinite = 1;
   }
 }
 
@@ -879,14 +894,14 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
 // waves they do not spill, and a CU still holds more of their workgroups than the layered grids
 // supply (513 single-column workgroups at 512^2 x 3).
 template <int NL, int CPW> struct K3Waves { static constexpr int value = NL > 1 ? 2 : GF_K3_WAVES; };
-template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1>
+template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1, bool ML1 = false>
 __global__ __launch_bounds__(CPW * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(K3Waves<NL, CPW>::value)))
 void k3_kernel(const K3Args<T> a) {
   __shared__ cplx<T> lds[CPW == 2 ? G * PairLds<T, NY>::value : LdsN<NY>::value];
   __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
   load_twiddles<T, NY, P>(tw, a.tw);
   __syncthreads();
-  k3_body<T, NY, P, G, NL, CPW, EPI>(a, blockIdx.x, lds, tw);
+  k3_body<T, NY, P, G, NL, CPW, EPI, ML1>(a, blockIdx.x, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -1002,6 +1017,13 @@ void k3_np(const K3Args<T>& a, hipStream_t s) {
     const int nblk = 16 * ((npairs + 7) / 8);
     if (npairs > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, 1, NL, 1>), dim3(nblk, a.lsplit ? NL : 1), dim3(N / P), 0, s, a);
     return;
+  }
+  if constexpr (NL == 1 && P == PY<N>::P) {
+    if (a.model == M_MLQG) {   // one-layer MultiLayerQG: fixed schedule, column pairs
+      const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
+      if (nblk > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, G, 1, 2, -1, true>), dim3(nblk), dim3(2 * G * (N / P)), 0, s, a);
+      return;
+    }
   }
   if constexpr (NL == 1 && k3_single_column(N, P)) {
     static_assert(G == 1, "a single column fills a wave");

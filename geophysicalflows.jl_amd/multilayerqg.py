@@ -1,10 +1,11 @@
 """MultiLayerQG on the device (mirror of ``src/multilayerqg.jl``).
 
-``Problem(nlayers, dev; ...)`` takes the reference keywords (``src/multilayerqg.jl:87-114``).
-The per-wavenumber ``S^-1`` inversion (495-509, 563-583), the layered Jacobian with
-background gradients ``Qx``, ``Qy`` (695-731) and the bottom drag (658) run in the
-HIP library.  ``U`` may be a scalar, one value per layer, or an ``(ny, nlayers)``
-array as in the reference.
+``Problem(nlayers, dev; ...)`` takes the reference keywords (``src/multilayerqg.jl:87-114``),
+for 1 to 8 layers (one layer: ``SingleLayerParams``, 193-219).  The per-wavenumber ``S^-1``
+inversion (495-509, 563-583) and ``S`` product (520-552), the layered Jacobian with
+background gradients ``Qx``, ``Qy`` (695-731), the bottom drag (658), the energies and the
+fluxes (918-1022) run in the HIP library.  ``U`` may be a scalar, one value per layer, or an
+``(ny, nlayers)`` array as in the reference.
 """
 import math
 
@@ -45,9 +46,11 @@ def Problem(nlayers, dev=None, *, nx=128, ny=None, Lx=2 * math.pi, Ly=None, f0=1
     H = np.ones(n) / n if H is None else H
     b = -(1 + np.arange(n) / n) if b is None else b
     U = np.zeros(n) if U is None else U
-    Ha = np.ascontiguousarray(np.asarray(H, dtype=np.float64))
-    ba = np.ascontiguousarray(np.asarray(b, dtype=np.float64))
+    Ha = np.ascontiguousarray(np.asarray(H, dtype=np.float64).reshape(-1))
+    ba = np.ascontiguousarray(np.asarray(b, dtype=np.float64).reshape(-1))
     Ua = np.ascontiguousarray(_U_layers(U, n, ny))
+    if n > 1 and (Ha.size != n or ba.size != n):
+        raise ValueError("H and b need one value per layer")
     keep = [Ha, ba, Ua]
     d.H, d.b, d.Uy = dptr(Ha), dptr(ba), dptr(Ua)
     d.f0, d.beta, d.mu, d.nu, d.nnu, d.linear = float(f0), float(β), float(μ), float(ν), int(nν), int(bool(linear))
@@ -76,15 +79,41 @@ def set_q(prob, q):
     prob._updatevars()
 
 
+def set_ψ(prob, ψ):
+    """``set_ψ!`` (src/multilayerqg.jl:873-889): ``qh = S ψh`` per wavenumber on the device
+    (``pvfromstreamfunction!``, 520-552), then ``set_q!`` of ``irfft(qh)``."""
+    a = np.ascontiguousarray(np.asarray(ψ, dtype=prob._rtype).reshape(prob._pshape))
+    _lib.check(_lib.lib.gf_set_psi(prob._ctx, a.ctypes.data), prob._ctx)
+    prob._updatevars()
+
+
+set_psi = set_ψ
+
+
 def updatevars(prob):
     prob._updatevars()
 
 
-@device_diagnostic(lambda p: (_lib.GF_DIAG_LAYER_ENERGIES, 0.0, 0, 2 * p.nlayers - 1),
-                   unpack=lambda r: (np.array(r[: (len(r) + 1) // 2]), np.array(r[(len(r) + 1) // 2:])))
+def _split(r):
+    if len(r) == 1:
+        return r[0]
+    return np.array(r[: (len(r) + 1) // 2]), np.array(r[(len(r) + 1) // 2:])
+
+
+@device_diagnostic(lambda p: (_lib.GF_DIAG_LAYER_ENERGIES, 0.0, 0, 2 * p.nlayers - 1), unpack=_split)
 def energies(prob):
     """Layer kinetic energies KE_j and interface potential energies PE_{j+1/2}
-    (src/multilayerqg.jl:918-951), computed on the device: returns (KE, PE) arrays."""
+    (src/multilayerqg.jl:918-960), computed on the device: returns (KE, PE) arrays, or the
+    energy alone for one layer."""
     n = prob.nlayers
-    v = prob._diagnostic(_lib.GF_DIAG_LAYER_ENERGIES, nout=2 * n - 1)
-    return np.array(v[:n]), np.array(v[n:])
+    return _split(prob._diagnostic(_lib.GF_DIAG_LAYER_ENERGIES, nout=2 * n - 1))
+
+
+@device_diagnostic(lambda p: (_lib.GF_DIAG_FLUXES, 0.0, 0, 2 * p.nlayers - 1), unpack=_split)
+def fluxes(prob):
+    """Lateral eddy fluxes per layer and vertical eddy fluxes per interface
+    (src/multilayerqg.jl:977-1022), computed on the device: returns (lateralfluxes,
+    verticalfluxes), or the lateral flux alone for one layer.  Dealiases the state in place,
+    as the reference's call of ``updatevars!`` does."""
+    n = prob.nlayers
+    return _split(prob._diagnostic(_lib.GF_DIAG_FLUXES, nout=2 * n - 1))

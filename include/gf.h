@@ -56,7 +56,7 @@ enum { GF_FORCING_NONE = 0, GF_FORCING_FIXED = 1, GF_FORCING_RING = 2 };
 typedef struct gf_problem_desc {
   int model;                 /* GF_MODEL_*                                         */
   int dtype;                 /* GF_F64 | GF_F32  (keyword T)                        */
-  int nx, ny, nlayers;       /* powers of two, 16..2048; nlayers: MLQG 2..4, else 1 */
+  int nx, ny, nlayers;       /* powers of two, 16..2048; nlayers: MLQG 1..8, else 1 */
   double Lx, Ly;
   double aliased_fraction;   /* default 1/3                                         */
   int stepper;               /* GF_ETDRK4 ...                                       */
@@ -72,8 +72,8 @@ typedef struct gf_problem_desc {
                                 MLQG: U per layer, nlayers*ny (y fastest), or NULL  */
   /* MultiLayerQG */
   double f0;
-  const double* H;           /* nlayers                                             */
-  const double* b;           /* nlayers (Boussinesq buoyancy)                       */
+  const double* H;           /* nlayers (unused, nullable, for one layer)           */
+  const double* b;           /* nlayers (Boussinesq buoyancy; as H)                 */
   double topo_grad_x, topo_grad_y;   /* topographic_pv_gradient                     */
   int linear;                /* MLQG linearised equations (src/multilayerqg.jl:358-362, calcNlinear!) */
   /* forcing (calcF / calcFq) */
@@ -104,6 +104,13 @@ int gf_get_state_dev(gf_ctx* ctx, void* sol_dev);
  * src/multilayerqg.jl:845-851): forward rfft of the physical field on the
  * device, zero mean for TwoDNS and MLQG, then updatevars!'s in-place dealias. */
 int gf_set_physical(gf_ctx* ctx, const void* field);
+
+/* MultiLayerQG set_psi! (src/multilayerqg.jl:873-889): psih = rfft(psi) per layer, qh = S psih
+ * per wavenumber (pvfromstreamfunction!, 520-552; one layer: qh = -Krsq psih), then set_q! of
+ * irfft(qh): rfft, zero mean, updatevars!'s dealias!.  psi holds nlayers*nx*ny values of the
+ * problem's precision, layer slowest, x fastest; host memory.  GF_EUNSUPPORTED for the other
+ * models. */
+int gf_set_psi(gf_ctx* ctx, const void* psi);
 
 /* updatevars! (src/twodnavierstokes.jl:331-345, src/singlelayerqg.jl:502-518,
  * src/multilayerqg.jl:819-831): dealias sol in place, write the physical fields
@@ -232,8 +239,12 @@ int gf_ctx_instance(gf_ctx* ensemble, int i, gf_ctx** member);
  *      energy/enstrophy_dissipation (xi = nu, nxi = nnu);
  *      energy/enstrophy_drag (xi = mu);
  *      energy/enstrophy_work.
- *  - MultiLayerQG energies (src/multilayerqg.jl:918-951): GF_DIAG_LAYER_ENERGIES writes
- *    KE_1..KE_n, then PE_{3/2}..PE_{n-1/2}.
+ *  - MultiLayerQG energies (src/multilayerqg.jl:918-960): GF_DIAG_LAYER_ENERGIES writes
+ *    KE_1..KE_n, then PE_{3/2}..PE_{n-1/2} (one layer: the energy only).
+ *  - MultiLayerQG fluxes (src/multilayerqg.jl:977-1022): GF_DIAG_FLUXES writes the lateral
+ *    eddy fluxes lateralfluxes_1..n, then the vertical fluxes verticalfluxes_{3/2}..{n-1/2}
+ *    (one layer: the lateral flux only). As the reference's fluxes calls updatevars!, it
+ *    dealiases the state in place first.
  * For stochastic forcing the work uses (prevsol + sol)/2 and the last step's forcing.
  * Returns the number of values written (>= 1) or a negative GF_E* code; out holds nout
  * doubles. */
@@ -241,7 +252,7 @@ enum {
   GF_DIAG_ENERGY = 0, GF_DIAG_KINETIC_ENERGY = 1, GF_DIAG_POTENTIAL_ENERGY = 2, GF_DIAG_ENSTROPHY = 3,
   GF_DIAG_PALINSTROPHY = 4, GF_DIAG_ENERGY_DISSIPATION = 5, GF_DIAG_ENSTROPHY_DISSIPATION = 6,
   GF_DIAG_ENERGY_DRAG = 7, GF_DIAG_ENSTROPHY_DRAG = 8, GF_DIAG_ENERGY_WORK = 9, GF_DIAG_ENSTROPHY_WORK = 10,
-  GF_DIAG_LAYER_ENERGIES = 11
+  GF_DIAG_LAYER_ENERGIES = 11, GF_DIAG_FLUXES = 12
 };
 int gf_diagnostic(gf_ctx* ctx, int kind, double xi, int nxi, double* out, int nout);
 

@@ -9,7 +9,7 @@
 * ``streamfunctionfrompv``/``pv_streamfunction_kernel!``: 495-509, 563-595.
 * ``calcN``: 651-663 (dealias P1, bottom drag ``N[n] += mu Krsq psih[n]``);
   ``calcN_advection``: 695-731; ``calcNlinear``/``calcN_linearadvection``: 675-683, 744-779.
-* ``updatevars``/``set_q``/``set_psi``: 819-894; ``energies``: 918-951.
+* ``updatevars``/``set_q``/``set_psi``: 819-894; ``energies``: 918-960; ``fluxes``: 977-1022.
 
 Array layout: spectral ``(nlayers, nl, nkr)``, physical ``(nlayers, ny, nx)`` -- the
 reference's ``(nkr, nl, nlayers)``/``(nx, ny, nlayers)`` column-major arrays.
@@ -237,3 +237,22 @@ def energies(prob):
     PE = np.array([1 / (2 * V) * p.f0 ** 2 / p.gp[j] *
                    parsevalsum(np.abs(vars.psih[j] - vars.psih[j + 1]) ** 2, grid) for j in range(n - 1)])
     return KE, PE
+
+
+def fluxes(prob):
+    """Lateral eddy fluxes per layer and vertical eddy fluxes per interface
+    (src/multilayerqg.jl:977-1022): ``updatevars!`` (which dealiases sol), then
+    ``du/dy = irfft(i l uh)`` and the physical-space sums times ``dx dy / V``."""
+    vars, p, grid = prob.vars, prob.params, prob.grid
+    n = p.nlayers
+    updatevars(prob)
+    dudy = grid.irfft(1j * grid.Lw * vars.uh)
+    U = p.U[:, :, None]                                           # (n, ny, 1)
+    dxdy = grid.dx * grid.dy
+    if n == 1:
+        return float(np.sum(U[0] * vars.v[0] * dudy[0])) * dxdy / (grid.Lx * grid.Ly)
+    V = grid.Lx * grid.Ly * np.sum(p.H)
+    lateral = p.H * np.array([np.sum(U[j] * vars.v[j] * dudy[j]) for j in range(n)]) * dxdy / V
+    vertical = np.array([np.sum(p.f0 ** 2 / p.gp[j] * (U[j] - U[j + 1]) * vars.v[j + 1] * vars.psi[j])
+                         for j in range(n - 1)]) * dxdy / V
+    return lateral, vertical

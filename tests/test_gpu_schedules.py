@@ -98,8 +98,14 @@ def test_schedule_not_offered_is_rejected(gf):
     m = gf.MultiLayerQG.Problem(2, gf.GPU(), nx=64, f0=1, H=[1.0, 1.0], b=[0, -1.0], dt=0.01, stepper="RK4")
     with pytest.raises(gf._lib.GFError, match="not offered"):
         gf.set_schedule(m, 4, 4, 4)
-    with pytest.raises(gf._lib.GFError, match="single-layer"):
+    with pytest.raises(gf._lib.GFError, match="TwoDNavierStokes and SingleLayerQG"):
         gf.set_splits(m, 1, 0)
+    # a one-layer MultiLayerQG problem keeps the fixed schedule too
+    m1 = gf.MultiLayerQG.Problem(1, gf.GPU(), nx=64, dt=0.01, stepper="RK4")
+    with pytest.raises(gf._lib.GFError, match="not offered"):
+        gf.set_schedule(m1, 4, 4, 4)
+    with pytest.raises(gf._lib.GFError, match="TwoDNavierStokes and SingleLayerQG"):
+        gf.set_splits(m1, 1, 0)
 
 
 def test_autotune_restores_state_and_records_the_choice(gf, tmp_path):

@@ -285,3 +285,98 @@ def test_mlqg_rossby_wave_single_layer():
     ML.updatevars(prob)
     qth = ampl * np.cos(kw * (x - omega / kw * prob.clock.t)) * np.cos(lw * y)
     assert np.linalg.norm(prob.vars.q[0] - qth) / np.linalg.norm(qth) < n * n * 20 * 1e-12
+
+
+def _fluxes_problem(nlayers, nx=128, ny=126):
+    """test/test_multilayerqg.jl:567-625: a 2- (or 3-) layer problem with U1 = sech(y/0.2)^2."""
+    g = ff.TwoDGrid(nx, 2 * np.pi, ny, 2 * np.pi)
+    if nlayers == 2:
+        H, b = [0.2, 0.8], [-1.0, -1.2]
+    else:
+        H, b = [0.2, 0.35, 0.45], [-1.0, -1.2, -1.2 + 1e-9]
+    U = np.zeros((ny, nlayers))
+    U[:, 0] = 1 / np.cosh(g.y / 0.2) ** 2
+    prob = ML.Problem(nlayers, nx=nx, ny=ny, Lx=2 * np.pi, Ly=2 * np.pi, f0=1, H=H, b=b, U=U)
+    x, y = prob.grid.gridpoints()
+    psi1 = np.cos(x) * np.cos(y) + np.sin(x)
+    psi2 = np.cos(x + np.pi / 10) * np.cos(y)
+    psi = np.array([psi1, psi2] + ([psi2] if nlayers == 3 else []))
+    return prob, psi
+
+
+@pytest.mark.parametrize("nx,ny", [(128, 126), (128, 128)])
+@pytest.mark.parametrize("nlayers", [2, 3])
+def test_mlqg_fluxes_reference_values(nlayers, nx, ny):
+    """test/test_multilayerqg.jl:613-621 on the reference's 128 x 126 grid and on 128 x 128 (the
+    sums are trapezoid rules of an analytic periodic integrand: both grids give the same values)."""
+    prob, psi = _fluxes_problem(nlayers, nx, ny)
+    ML.set_psi(prob, psi)
+    lat, ver = ML.fluxes(prob)
+    ref = GOLD["mlqg_fluxes_2layer"]
+    rtol, az = ref["rtol"], ref["atol_zero"]
+    assert abs(lat[0] - ref["value"]["lateral1"]) <= rtol * abs(ref["value"]["lateral1"])
+    assert abs(lat[1]) <= az
+    assert abs(ver[0] - ref["value"]["vertical1"]) <= rtol * abs(ref["value"]["vertical1"])
+    if nlayers == 3:
+        a3 = GOLD["mlqg_fluxes_3layer_zeros"]["atol"]
+        assert abs(lat[2]) <= a3 and abs(ver[1]) <= a3
+
+
+@pytest.mark.parametrize("nx,ny", [(128, 126), (128, 128)])
+def test_mlqg_fluxes_single_layer_reference_value(nx, ny):
+    """test/test_multilayerqg.jl:632-655: one layer, U = sech(y/0.2)^2."""
+    g = ff.TwoDGrid(nx, 2 * np.pi, ny, 2 * np.pi)
+    prob = ML.Problem(1, nx=nx, ny=ny, Lx=2 * np.pi, Ly=2 * np.pi, U=1 / np.cosh(g.y / 0.2) ** 2)
+    x, y = prob.grid.gridpoints()
+    ML.set_psi(prob, (np.cos(x) * np.cos(y) + np.sin(x))[None])
+    ref = GOLD["mlqg_fluxes_single_layer"]
+    assert abs(ML.fluxes(prob) - ref["value"]) <= ref["atol"]
+
+
+def test_mlqg_fluxes_planted_sign_error_fails():
+    """The reference values catch a vertical flux with the shear reversed, (U_{j+1} - U_j)."""
+    prob, psi = _fluxes_problem(2)
+    ML.set_psi(prob, psi)
+    prob.params.U = prob.params.U[::-1].copy()   # swaps U_1 and U_2: the shear changes sign
+    _, ver = ML.fluxes(prob)
+    ref = GOLD["mlqg_fluxes_2layer"]["value"]["vertical1"]
+    assert abs(ver[0] - ref) > 1e-3
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 66), (64, 64)])
+def test_mlqg_energies_via_set_psi(nx, ny):
+    """test/test_multilayerqg.jl:502-536 entering through set_psi!, on the reference's 64 x 66 grid
+    and on 64 x 64."""
+    prob = ML.Problem(2, nx=nx, ny=ny, Lx=2 * np.pi, Ly=2 * np.pi, f0=1, H=[2.5, 7.5], b=[-1.0, -1.5])
+    x, y = prob.grid.gridpoints()
+    ML.set_psi(prob, np.array([np.cos(2 * x) * np.cos(2 * y), 0.5 * np.cos(2 * x) * np.cos(2 * y)]))
+    KE, PE = ML.energies(prob)
+    ke1, ke2, pe = (float(Fraction(v)) for v in GOLD["mlqg_KE1_KE2_PE"]["value"])
+    assert abs(KE[0] - ke1) < 1e-13 and abs(KE[1] - ke2) < 1e-13 and abs(PE[0] - pe) < 1e-13
+
+
+def test_mlqg_energy_single_layer():
+    """test/test_multilayerqg.jl:538-560: one layer, Ly = 3 pi, energy 29/9."""
+    nx, ny = 64, 64
+    prob = ML.Problem(1, nx=nx, Lx=2 * np.pi, ny=ny, Ly=3 * np.pi, U=np.zeros(ny), stepper="ForwardEuler")
+    x, y = prob.grid.gridpoints()
+    k0, l0 = 2 * np.pi / (2 * np.pi), 2 * np.pi / (3 * np.pi)
+    q0 = (-((2 * k0) ** 2 + (2 * l0) ** 2) * np.sin(2 * k0 * x) * np.cos(2 * l0 * y)
+          - (k0 ** 2 + (3 * l0) ** 2) * 2 * np.sin(k0 * x) * np.cos(3 * l0 * y))
+    ML.set_q(prob, q0[None])
+    e = float(Fraction(GOLD["mlqg_energy_single_layer_64x64_2pi_3pi"]["value"]))
+    assert abs(ML.energies(prob) - e) < 1e-13 * e
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 34), (32, 32)])
+def test_mlqg_set_q_set_psi_roundtrip(nx, ny):
+    """test/test_multilayerqg.jl:663-701: set_psi! then S^-1 gives psi back; set_q! gives q back."""
+    prob = ML.Problem(2, nx=nx, ny=ny, Lx=2 * np.pi, f0=1, H=[0.2, 0.8], b=[-1.0, -1.2])
+    x, y = prob.grid.gridpoints()
+    f = np.array([2 * np.cos(x) * np.cos(y), np.cos(x + np.pi / 10) * np.cos(2 * y)])
+    ML.set_psi(prob, f)
+    psih = np.zeros_like(prob.sol)
+    ML.streamfunctionfrompv(psih, prob.sol.copy(), prob.params, prob.grid)
+    assert np.allclose(prob.grid.irfft(psih), f, rtol=RTOL, atol=1e-14)
+    ML.set_q(prob, f)
+    assert np.allclose(prob.grid.irfft(prob.sol.copy()), f, rtol=RTOL, atol=1e-14)
