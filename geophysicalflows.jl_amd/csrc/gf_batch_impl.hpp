@@ -1,4 +1,5 @@
-// Batched ensembles of independent small instances of uneven size.
+// Batched ensembles of independent small instances of uneven size (FP64 or FP32, one precision
+// per ensemble).
 //
 // One launch per kernel class (K1, K2, K3) and stage covers every member of the
 // ensemble: the grid is a flat table of (instance, block) work entries, ordered
@@ -16,17 +17,20 @@ namespace gf {
 
 constexpr int BATCH_THREADS = 256;
 constexpr int BATCH_P = 8;
-// G1 PairLds<N> (2 G1 LdsN<N> plus one column skew per pair) and G2 LdsN<N>: at most
-// 2304 + 32 skews of 4 slots (N = 32) for every N
-constexpr int BATCH_LDS = BATCH_THREADS * BATCH_P * 9 / 8 + 32 * 4;
+// G1 PairLds<T, N> (2 G1 LdsN<N> plus one column skew per pair) and G2 LdsN<N>: at most
+// 2304 slots plus 32 skews (N = 32, 32 column pairs per workgroup) for every N.  The skew is 64
+// bytes: 4 slots in FP64, 8 in FP32.
+template <typename T> struct BatchLds {
+  static constexpr int value = BATCH_THREADS * BATCH_P * 9 / 8 + 32 * ColSkew<T>::value;
+};
 constexpr int BATCH_TW = 128;
 
-template <int N> struct BatchGeom {
+template <typename T, int N> struct BatchGeom {
   static constexpr int TPT = N / BATCH_P;
   static constexpr int G1 = BATCH_THREADS / (2 * TPT);   // column pairs per K1/K3 workgroup
   static constexpr int G2 = BATCH_THREADS / TPT;         // row pairs per K2 workgroup
   static_assert(PY<N>::P == BATCH_P, "batched sizes use 8 points per thread");
-  static_assert(G1 * PairLds<double, N>::value <= BATCH_LDS && G2 * LdsN<N>::value <= BATCH_LDS, "LDS");
+  static_assert(G1 * PairLds<T, N>::value <= BatchLds<T>::value && G2 * LdsN<N>::value <= BatchLds<T>::value, "LDS");
   static_assert(TwPlan<N, BATCH_P>::size() <= BATCH_TW, "twiddle table");
 };
 
@@ -34,7 +38,7 @@ template <int N> struct BatchGeom {
 // x-pass body (122 VGPRs instead of 243 with the K2_ADVECT_FULL body beside it).
 template <typename T, int N, int K, bool PLAIN>
 __device__ __forceinline__ void batch_body(const BatchItem<T>& it, int blk, cplx<T>* lds, cplx<T>* tw) {
-  using BG = BatchGeom<N>;
+  using BG = BatchGeom<T, N>;
   if constexpr (K == 1) {
     load_twiddles<T, N, BATCH_P>(tw, it.k1.tw);
     __syncthreads();
@@ -51,14 +55,15 @@ __device__ __forceinline__ void batch_body(const BatchItem<T>& it, int blk, cplx
   } else {
     load_twiddles<T, N, BATCH_P>(tw, it.k3.tw);
     __syncthreads();
-    k3_body<T, N, BATCH_P, BG::G1, 1>(it.k3, blk, lds, tw);
+    if (it.k3.model == M_MLQG) k3_body<T, N, BATCH_P, BG::G1, 1, 2, -1, true>(it.k3, blk, lds, tw);   // one layer
+    else k3_body<T, N, BATCH_P, BG::G1, 1>(it.k3, blk, lds, tw);
   }
 }
 
 template <typename T, int K, bool PLAIN>
 __global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 void batch_kernel(const BatchItem<T>* __restrict__ items, const BatchWork* __restrict__ work) {
-  __shared__ cplx<T> lds[BATCH_LDS];
+  __shared__ cplx<T> lds[BatchLds<T>::value];
   __shared__ cplx<T> tw[BATCH_TW];
   const BatchWork w = work[blockIdx.x];
   const BatchItem<T>& it = items[w.item];
@@ -83,6 +88,7 @@ void launch_batch(const BatchItem<T>* items, const BatchWork* work, int nwork, i
   else hipLaunchKernelGGL((batch_kernel<T, K, false>), dim3(nwork), dim3(BATCH_THREADS), 0, s, items, work);
 }
 template void launch_batch<double, GF_BATCH_K>(const BatchItem<double>*, const BatchWork*, int, int, hipStream_t);
+template void launch_batch<float, GF_BATCH_K>(const BatchItem<float>*, const BatchWork*, int, int, hipStream_t);
 
 #if GF_BATCH_K == 1
 bool batch_size_supported(int n) { return n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }

@@ -254,6 +254,7 @@ struct CtxBase {
   virtual int diagnostic_async(int, double, int, double*, int) {
     throw GfError(GF_EINVAL, "diagnostics are computed per problem; address one member through gf_ctx_instance");
   }
+  virtual void set_tape_budget(size_t) { throw GfError(GF_EUNSUPPORTED, "the adjoint's tape is set per problem"); }
   virtual void adjoint_grad(const double*, int, int, double*, double*, double*) {
     throw GfError(GF_EUNSUPPORTED, "the adjoint is provided for TwoDNavierStokes FP64 ETDRK4 problems");
   }
@@ -358,8 +359,9 @@ struct Ctx final : CtxBase {
       (void)hipStreamSynchronize(rstream);
       (void)dw::stream_destroy(rstream);
       (void)hipEventDestroy(fwd_done);
-      for (int k = 0; k < 2; ++k) { (void)hipEventDestroy(adj_done[k]); (void)hipEventDestroy(rec_done[k]); }
     }
+    for (hipEvent_t e : adj_done) (void)hipEventDestroy(e);
+    for (hipEvent_t e : rec_done) (void)hipEventDestroy(e);
     if (copy_stream) {
       (void)hipStreamSynchronize(copy_stream);
       for (int k = 0; k < 2; ++k) { (void)hipEventDestroy(snap_ready[k]); (void)hipEventDestroy(snap_done[k]); }
@@ -734,8 +736,9 @@ struct Ctx final : CtxBase {
   // ---------------- hot path ----------------
   void rec(int k, bool start) { tm.rec(k, start, stream); }
 
-  // argument blocks of the three kernels for one calcN! evaluation with stage epilogue epi
-  void make_args(const C* in, int epi, bool full, K1Args<T>& a, K2Args<T>& b, K3Args<T>& c) const {
+  // argument blocks of the three kernels for one calcN! evaluation with stage epilogue epi;
+  // ab3_start >= 0 overrides the AB3 start-up flag the clock gives (ensemble tables)
+  void make_args(const C* in, int epi, bool full, K1Args<T>& a, K2Args<T>& b, K3Args<T>& c, int ab3_start = -1) const {
     a = K1Args<T>{};
     a.g = g; a.model = d.model; a.fieldset = FS_CALCN; a.sol = in; a.out = mixed.as<C>(); a.fstride = fsz;
     a.ncols = g.kmax; a.eb = eb; a.ell2inv = (T)ell2inv; a.sinv = sinv.as<T>(); a.tw = twy.as<C>();
@@ -773,7 +776,7 @@ struct Ctx final : CtxBase {
     if (tunable()) { a.fsplit = split[0]; b.hsplit = split[1]; c.csplit = split[2]; }
     if (ab3()) {
       // the RHS history of the aliased band enters later steps: every AB3 step is computed in full
-      c.ab3_start = stepno < 3;
+      c.ab3_start = ab3_start >= 0 ? ab3_start : stepno < 3;
       b.kout = g.nkr; c.ncols = g.nkr; c.full = 1;
     }
     if (d.model == GF_MODEL_MLQG && d.linear) {
@@ -1276,15 +1279,24 @@ struct Ctx final : CtxBase {
   }
 
   // ---------------- adjoint (TwoDNavierStokes, ETDRK4, FP64) ----------------
-  DevBuf adj_mixed, adj_flux, lamb, l0b, l1b, lN1b, lN2b, lN3b, nu_part, dtab[6], tape, tape2, ckpts;
-  // With checkpoints the sweep is pipelined over two streams: segment s - 1 is recomputed into
-  // one tape on rstream while the adjoint of segment s reads the other tape on the main stream.
+  DevBuf adj_mixed, adj_flux, lamb, l0b, l1b, lN1b, lN2b, lN3b, nu_part, dtab[6], pool, ckpts;
+  // The tape is a pool of segment slots. Segment s of a run is taped into slot s mod P, so after
+  // the forward pass the last P segments are still taped, and the backward sweep recomputes only
+  // the earlier ones from their checkpoints. P is 2 (the two-stream pipeline) unless a tape
+  // budget (gf_adjoint_tape_budget) affords more. With recomputation the sweep is pipelined over
+  // two streams: segment s - 1 is recomputed into its slot on rstream while the adjoint of
+  // segment s reads its own slot on the main stream.
   hipStream_t rstream = nullptr;
-  hipEvent_t adj_done[2] = {}, rec_done[2] = {}, fwd_done = nullptr;
-  long tape2_steps = 0;
-  long tape_steps = 0;       // steps the tape holds (grown on demand, never shrunk:
-  long ckpt_slots = 0;       // freeing device memory synchronises the whole device)
+  std::vector<hipEvent_t> adj_done, rec_done;   // per slot
+  hipEvent_t fwd_done = nullptr;
+  long pool_seg = 0;         // steps per slot of the pool (grown on demand, never shrunk:
+  int pool_slots = 0;        // freeing device memory synchronises the whole device)
+  long ckpt_slots = 0;
+  size_t tape_budget = 0;    // bytes the pool may take; 0: two slots
   int ka3_blocks = 0;
+  // A tape field holds only the columns kr < kmax (rounded up to a pair): every reader masks,
+  // and in the blocked layout those columns are a prefix of the field.
+  long kstride() const { return (long)((g.kmax + 1) & ~1) * d.ny; }
 
   void adjoint_setup() {
     if (adj_mixed.p) return;
@@ -1313,8 +1325,11 @@ struct Ctx final : CtxBase {
     for (int k = 0; k < 6; ++k) upload_blocked(dtab[k], t6[k]);
   }
 
-  // slot i, field f of tape tp (0: tape, 1: tape2)
-  C* slot(long i, int f, int tp = 0) const { return (tp ? tape2 : tape).as<C>() + ((size_t)i * 8 + f) * fsz; }
+  // step i, field f of pool slot tp (fields: s0, s1, s2, s3, N1, N2, N3, N4; step i + 1's s0 follows)
+  C* slot(long i, int f, int tp = 0) const {
+    return pool.as<C>() + ((size_t)tp * (8 * pool_seg + 1) + (size_t)i * 8 + f) * kstride();
+  }
+  void set_tape_budget(size_t bytes) override { tape_budget = bytes; }
 
   // One ETDRK4 step from tape slot i (s0) writing s1, s2, s3 and N1..N4 of that slot and the
   // new state to next (kept modes only: every later read masks), on stream st. Its scratch
@@ -1402,45 +1417,55 @@ struct Ctx final : CtxBase {
       } active;
       const long seg = std::min(every, nsteps);
       const long nseg = (nsteps + seg - 1) / seg;
-      const bool pipe = nseg > 1 && active.others == 0;   // recompute on rstream beside the sweep
-      if (tape_steps < seg) {
-        tape.alloc(sizeof(C) * fsz * (8 * seg + 1));
-        tape_steps = seg;
-      }
-      if (pipe && tape2_steps < seg) {
-        tape2.alloc(sizeof(C) * fsz * (8 * seg + 1));
-        tape2_steps = seg;
+      const size_t kb = sizeof(C) * (size_t)kstride();   // one tape field (a prefix of a state)
+      const size_t seg_bytes = kb * (size_t)(8 * seg + 1);
+      // pool slots: as many whole segments as the budget affords, at least two with the pipeline
+      // (which pays only when this is the one gradient in flight), never more than the run has
+      long nslot = (long)std::min<size_t>((size_t)nseg, tape_budget / seg_bytes);
+      const bool alone = active.others == 0;
+      nslot = std::max<long>(nslot, (nseg > 1 && alone) ? 2 : 1);
+      const int P = (int)nslot;
+      const bool pipe = nseg > P && alone;   // recompute on rstream beside the sweep
+      if (pool_seg < seg || pool_slots < P) {
+        pool.alloc(0);   // free first: the old and the new pool need not fit together
+        pool.alloc(sizeof(C) * (size_t)std::max<long>(P, pool_slots) * (8 * seg + 1) * kstride());
+        pool_seg = seg;
+        pool_slots = std::max(P, pool_slots);
       }
       if (pipe && !rstream) {
         HIPCHECK(dw::stream_create(&rstream));
         HIPCHECK(hipEventCreateWithFlags(&fwd_done, hipEventDisableTiming));
-        for (int k = 0; k < 2; ++k) {
-          HIPCHECK(hipEventCreateWithFlags(&adj_done[k], hipEventDisableTiming));
-          HIPCHECK(hipEventCreateWithFlags(&rec_done[k], hipEventDisableTiming));
-        }
+      }
+      while (pipe && (int)adj_done.size() < P) {
+        hipEvent_t a = nullptr, r = nullptr;
+        HIPCHECK(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&r, hipEventDisableTiming));
+        adj_done.push_back(a);
+        rec_done.push_back(r);
       }
       if (ckpt_slots < nseg) {
-        ckpts.alloc(sizeof(C) * fsz * nseg);
+        ckpts.alloc(kb * nseg);
         ckpt_slots = nseg;
       }
       const size_t fb = sizeof(C) * fsz;
       auto len = [&](long s) { return std::min(seg, (long)nsteps - s * seg); };
-      // forward: checkpoints at segment starts; segment s is taped into tape s mod 2 (into the
-      // one tape without the pipeline), and the last segment's tape stays for the sweep
-      auto tape_of = [&](long s) { return pipe ? (int)(s & 1) : 0; };
+      // forward: checkpoints at segment starts; segment s is taped into slot s mod P, so the last
+      // P segments' tapes stay for the sweep
+      auto tape_of = [&](long s) { return (int)(s % P); };
       for (long s = 0; s < nseg; ++s) {
         const int tp = tape_of(s);
-        HIPCHECK(hipMemcpyAsync(slot(0, 0, tp), s == 0 ? (const C*)sol.p : slot(len(s - 1), 0, tape_of(s - 1)), fb,
+        HIPCHECK(hipMemcpyAsync(slot(0, 0, tp), s == 0 ? (const C*)sol.p : slot(len(s - 1), 0, tape_of(s - 1)), kb,
                                 hipMemcpyDeviceToDevice, stream));
-        HIPCHECK(hipMemcpyAsync(ckpts.as<C>() + s * fsz, slot(0, 0, tp), fb, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(ckpts.as<C>() + s * kstride(), slot(0, 0, tp), kb, hipMemcpyDeviceToDevice, stream));
         const long n = len(s);
         for (long i = 0; i < n; ++i) tape_step(i, slot(i + 1, 0, tp), tp, stream);
       }
-      // loss and lambda_T = M (w/N) rfft2(zeta_T - target), zeta_T = irfft2(M sol_T)
+      // loss and lambda_T = M (w/N) rfft2(zeta_T - target), zeta_T = irfft2(M sol_T); the columns of
+      // cinp past the tape's prefix hold stale values, which the mask clears
       const long nlast = len(nseg - 1);
       const int tlast = tape_of(nseg - 1);
       const size_t np = (size_t)d.nx * d.ny;
-      HIPCHECK(hipMemcpyAsync(cinp.p, slot(nlast, 0, tlast), fb, hipMemcpyDeviceToDevice, stream));
+      HIPCHECK(hipMemcpyAsync(cinp.p, slot(nlast, 0, tlast), kb, hipMemcpyDeviceToDevice, stream));
       launch_mask<T>(g, cinp.as<C>(), 1, stream);
       inv_device(cinp.as<C>(), phys.as<T>(), 1);
       std::vector<T> r(np);
@@ -1463,27 +1488,31 @@ struct Ctx final : CtxBase {
         HIPCHECK(hipEventRecord(fwd_done, stream));
         HIPCHECK(hipStreamWaitEvent(rstream, fwd_done, 0));
       }
+      // segments s >= nseg - P are still taped from the forward pass
+      const long resident = nseg - P;
       for (long s = nseg - 1; s >= 0 && !pipe; --s) {
-        // one stream: recompute segment s into the tape, then its adjoint
+        // one stream: recompute segment s into its slot if it is not resident, then its adjoint
         const long n = len(s);
-        if (s != nseg - 1) {
-          HIPCHECK(hipMemcpyAsync(slot(0, 0), ckpts.as<C>() + s * fsz, fb, hipMemcpyDeviceToDevice, stream));
-          for (long i = 0; i < n; ++i) tape_step(i, slot(i + 1, 0), 0, stream);
+        const int tp = tape_of(s);
+        if (s < resident) {
+          HIPCHECK(hipMemcpyAsync(slot(0, 0, tp), ckpts.as<C>() + s * kstride(), kb, hipMemcpyDeviceToDevice, stream));
+          for (long i = 0; i < n; ++i) tape_step(i, slot(i + 1, 0, tp), tp, stream);
         }
-        for (long i = n - 1; i >= 0; --i) adjoint_step(i, dnu != nullptr, 0);
+        for (long i = n - 1; i >= 0; --i) adjoint_step(i, dnu != nullptr, tp);
       }
-      // (segment nseg - 2 is still in its tape from the forward pass and is not recomputed)
       for (long s = nseg - 1; s >= 0 && pipe; --s) {
-        const int tp = (int)(s & 1);
-        if (s >= 1 && s - 1 != nseg - 2) {
-          const int tq = tp ^ 1;
-          if (s + 1 <= nseg - 1) HIPCHECK(hipStreamWaitEvent(rstream, adj_done[tq], 0));
-          HIPCHECK(hipMemcpyAsync(slot(0, 0, tq), ckpts.as<C>() + (s - 1) * fsz, fb, hipMemcpyDeviceToDevice, rstream));
+        const int tp = tape_of(s);
+        if (s >= 1 && s - 1 < resident) {
+          // segment s - 1 goes into the slot of segment s - 1 + P, whose adjoint was queued before
+          const int tq = tape_of(s - 1);
+          HIPCHECK(hipStreamWaitEvent(rstream, adj_done[tq], 0));
+          HIPCHECK(hipMemcpyAsync(slot(0, 0, tq), ckpts.as<C>() + (s - 1) * kstride(), kb, hipMemcpyDeviceToDevice,
+                                  rstream));
           const long m = len(s - 1);
           for (long i = 0; i < m; ++i) tape_step(i, slot(i + 1, 0, tq), tq, rstream);
           HIPCHECK(hipEventRecord(rec_done[tq], rstream));
         }
-        if (s < nseg - 2) HIPCHECK(hipStreamWaitEvent(stream, rec_done[tp], 0));
+        if (s < resident) HIPCHECK(hipStreamWaitEvent(stream, rec_done[tp], 0));
         const long n = len(s);
         for (long i = n - 1; i >= 0; --i) adjoint_step(i, dnu != nullptr, tp);
         HIPCHECK(hipEventRecord(adj_done[tp], stream));
@@ -1509,16 +1538,22 @@ struct Ctx final : CtxBase {
 // members first).  Each member is a full Ctx on the shared stream, so the
 // single-instance entry points (state, physical fields, updatevars!, calcN!, and a
 // member-only step) work on it unchanged.
+template <typename T>
 struct BatchCtx final : CtxBase {
-  using T = double;
   using C = cplx<T>;
   hipStream_t stream = nullptr;
   std::vector<std::unique_ptr<Ctx<T>>> inst;
   int nst = 0;             // stages per step
-  DevBuf items[4][2];      // BatchItem per member for (stage, last step of the call)
-  DevBuf work[3][2];       // BatchWork table per kernel class and last-step flag
+  // BatchItem per member for (stage, variant) and BatchWork table per (kernel class, variant).
+  // variant: the last step of a call (1) or an interior step (0); for AB3, whose every step is
+  // computed in full, the forward-Euler start-up (1) or the Adams-Bashforth step (0)
+  DevBuf items[4][2];
+  DevBuf work[3][2];
   int nwork[3][2] = {};
   KernelTimer tm;
+  bool ring = false;       // some member has stochastic ring forcing
+  bool ab3m = false;       // the members step with AB3 / FilteredAB3
+  DevBuf ctr;              // ring forcing: the members' step number, read by K3 (fstep_dev)
 
   BatchCtx(const gf_problem_desc* ds, int n) {
     HIPCHECK(dw::stream_create(&stream));
@@ -1549,6 +1584,10 @@ struct BatchCtx final : CtxBase {
     const C* ins[4];
     int epis[4];
     nst = inst[0]->stages(ins, epis);
+    ab3m = inst[0]->ab3();
+    ring = false;
+    for (auto& m : inst) ring = ring || m->d.forcing == GF_FORCING_RING;
+    if (ring) ctr.alloc(sizeof(int64_t));
     std::vector<BatchItem<T>> h(n);
     k2_plain = true;
     for (int full = 0; full < 2; ++full) {
@@ -1558,10 +1597,14 @@ struct BatchCtx final : CtxBase {
           m.stages(ins, epis);
           h[i] = BatchItem<T>{};
           h[i].n = m.d.nx;
-          m.make_args(ins[k], epis[k], full != 0, h[i].k1, h[i].k2, h[i].k3);
+          if (ab3m) m.make_args(ins[k], epis[k], true, h[i].k1, h[i].k2, h[i].k3, full);
+          else m.make_args(ins[k], epis[k], full != 0, h[i].k1, h[i].k2, h[i].k3);
           h[i].k1.fsplit = 0;   // the batched launches are one-dimensional
           h[i].k2.hsplit = 0;
           h[i].k3.csplit = 0;
+          // ring forcing: every step's realisation is drawn from the step number in ctr, which the
+          // step sequence sets and advances (so the tables and a captured step stay valid)
+          if (ring) h[i].k3.fstep_dev = ctr.as<int64_t>();
           k2_plain = k2_plain && h[i].k2.mode == K2_ADVECT;
         }
         items[k][full].alloc(sizeof(BatchItem<T>) * n);
@@ -1571,7 +1614,7 @@ struct BatchCtx final : CtxBase {
         std::vector<BatchWork> w;
         for (int i : ord) {
           const Ctx<T>& m = *inst[i];
-          const int count = kc == 1 ? m.d.ny : (kc == 2 && full ? m.g.nkr : m.g.kmax);
+          const int count = kc == 1 ? m.d.ny : (kc == 2 && (full || ab3m) ? m.g.nkr : m.g.kmax);
           const int nb = batch_blocks(kc + 1, m.d.nx, count);
           for (int b = 0; b < nb; ++b) w.push_back(BatchWork{i, b});
         }
@@ -1595,6 +1638,7 @@ struct BatchCtx final : CtxBase {
       launch_batch<T, 3>(it, work[2][full].as<BatchWork>(), nwork[2][full], 0, stream);
       tm.rec(2, false, stream);
     }
+    if (ring) launch_counter(ctr.as<int64_t>(), 1, 1, stream);   // the next step's realisation
     HIPCHECK(hipGetLastError());
   }
   // the interior step (all stages, every member) captured once and replayed, as for one problem
@@ -1609,13 +1653,34 @@ struct BatchCtx final : CtxBase {
     for (auto& m : inst)
       if (m->is_harmonic())
         throw GfError(GF_EUNSUPPORTED, "time-harmonic forcing is not supported in ensembles (step the member alone)");
+    if (n <= 0) return;
+    if (ring || ab3m) {
+      // one step number for every member: the forcing counter and the AB3 start-up are shared
+      for (auto& m : inst)
+        if (m->stepno != inst[0]->stepno)
+          throw GfError(GF_EINVAL, "ensemble members with ring forcing or AB3 must be at the same step "
+                                   "(a member was stepped alone); set their clocks with gf_set_clock");
+      if (ring) launch_counter(ctr.as<int64_t>(), inst[0]->stepno, 0, stream);
+    }
     for (int s = 0; s < n; ++s) {
-      const int full = s == n - 1;
-      if (!full && graphs && !tm.on) {
+      const int last = s == n - 1;
+      // the tables' variant: the last step of the call, or for AB3 the forward-Euler start-up
+      const int v = ab3m ? (inst[0]->stepno < 3) : last;
+      if (last && ring) {
+        // the work diagnostics of stochastic forcing use the state at the start of the step
+        // (prevsol, src/twodnavierstokes.jl:311-319); kept for the last step of the call
+        for (auto& m : inst) {
+          if (m->d.forcing != GF_FORCING_RING) continue;
+          if (!m->prevsol.p) m->prevsol.alloc(sizeof(C) * m->fsz);
+          HIPCHECK(hipMemcpyAsync(m->prevsol.p, m->sol.p, sizeof(C) * m->fsz, hipMemcpyDeviceToDevice, stream));
+          m->prevsol_valid = true;
+        }
+      }
+      if (!last && v == 0 && graphs && !tm.on) {
         if (!gexec) build_graph();
         HIPCHECK(hipGraphLaunch(gexec, stream));
       } else {
-        step_launches(full);
+        step_launches(v);
       }
       for (auto& m : inst) { m->t += m->d.dt; m->stepno += 1; }
     }
@@ -1679,17 +1744,17 @@ int validate_batch(const gf_problem_desc* ds, int n, std::string& msg) {
   for (int i = 0; i < n; ++i) {
     const gf_problem_desc& d = ds[i];
     const std::string who = "ensemble member " + std::to_string(i) + ": ";
-    if (d.model == GF_MODEL_MLQG) { msg = who + "MultiLayerQG members are not supported in ensembles"; return GF_EUNSUPPORTED; }
+    if (d.model == GF_MODEL_MLQG && d.nlayers > 1) {
+      msg = who + "MultiLayerQG members must have one layer in ensembles"; return GF_EUNSUPPORTED;
+    }
     std::string m;
     const int rc = validate(&d, m);
     if (rc != GF_OK) { msg = who + m; return rc; }
-    if (d.dtype != GF_F64) { msg = who + "ensembles are FP64 only"; return GF_EUNSUPPORTED; }
+    if (d.dtype != ds[0].dtype) { msg = who + "all members must have the same precision"; return GF_EINVAL; }
     if (d.nx != d.ny || !batch_size_supported(d.nx)) {
       msg = who + "members must be square with n in {32, 64, 128, 256, 512}"; return GF_EUNSUPPORTED;
     }
-    if (d.forcing == GF_FORCING_RING) { msg = who + "stochastic ring forcing is not supported in ensembles"; return GF_EUNSUPPORTED; }
     if (d.stepper != ds[0].stepper) { msg = who + "all members must use the same stepper"; return GF_EINVAL; }
-    if (d.stepper == GF_AB3 || d.stepper == GF_FILTERED_AB3) { msg = who + "AB3 is not supported in ensembles"; return GF_EUNSUPPORTED; }
   }
   return GF_OK;
 }
@@ -1796,7 +1861,8 @@ int gf_ctx_create_batch(gf_ctx** out, const gf_problem_desc* descs, int n) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_err = "no HIP device"; return GF_ENODEVICE; }
   auto* c = new gf_ctx();
   try {
-    c->owned.reset(new BatchCtx(descs, n));
+    if (descs[0].dtype == GF_F64) c->owned.reset(new BatchCtx<double>(descs, n));
+    else c->owned.reset(new BatchCtx<float>(descs, n));
   } catch (const GfError& e) {
     g_create_err = e.what(); delete c; return e.code;
   } catch (const std::exception& e) {
@@ -1998,6 +2064,10 @@ int gf_diagnostic_async(gf_ctx* c, int kind, double xi, int nxi, double* out_dev
   int n = 0;
   const int rc = guarded(c, [&] { n = c->impl->diagnostic_async(kind, xi, nxi, out_dev, nout); });
   return rc == GF_OK ? n : rc;
+}
+
+int gf_adjoint_tape_budget(gf_ctx* c, size_t bytes) {
+  return guarded(c, [&] { c->impl->set_tape_budget(bytes); });
 }
 
 int gf_adjoint_grad(gf_ctx* c, const double* target, int nsteps, int checkpoint_every, double* dJ_dzeta0,

@@ -295,7 +295,7 @@ struct BatchWork {
 };
 
 // Launch kernel class K (1, 2, 3) of one stage for every instance: workgroup b runs
-// block work[b].blk of instance work[b].item (FP64 only).
+// block work[b].blk of instance work[b].item (FP64 or FP32).
 // plain (K = 2): every member's x pass is K2_ADVECT, which selects the kernel without the
 // background-gradient body
 template <typename T, int K> void launch_batch(const BatchItem<T>* items, const BatchWork* work, int nwork, int plain, hipStream_t s);

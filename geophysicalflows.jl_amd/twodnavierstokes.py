@@ -100,13 +100,17 @@ def enstrophy_work(prob):
     return prob._diagnostic(_lib.GF_DIAG_ENSTROPHY_WORK)[0]
 
 
-def adjoint_gradient(prob, target, nsteps, checkpoint=50, nu=True):
+def adjoint_gradient(prob, target, nsteps, checkpoint=50, nu=True, tape_budget=None):
     """Loss ``J = 1/2 sum (zeta(T) - target)^2`` after ``nsteps`` steps from the current state and
     its gradients with respect to the initial vorticity (physical, ``(ny, nx)``) and, if ``nu``,
     the viscosity: ``(J, dJ/dzeta0, dJ/dnu)``.  The discrete adjoint of the device ETDRK4 step
-    (unforced, FP64); checkpoints every ``checkpoint`` steps (no reference counterpart)."""
+    (unforced, FP64); checkpoints every ``checkpoint`` steps (no reference counterpart).
+    ``tape_budget`` (bytes): device memory the tape may take, so that the last segments stay
+    taped and only the earlier ones are recomputed (``gf_adjoint_tape_budget``)."""
     import ctypes
     from ._lib import check, lib
+    if tape_budget is not None:
+        check(lib.gf_adjoint_tape_budget(prob._ctx, int(tape_budget)), prob._ctx)
     g = prob.grid
     t = np.ascontiguousarray(np.asarray(target, dtype=np.float64).reshape(g.ny, g.nx))
     grad = np.empty_like(t)

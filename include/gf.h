@@ -214,10 +214,13 @@ int gf_kernel_bytes(const gf_ctx* ctx, int k, int full, double* bytes);
  *
  * No counterpart in the reference: there an ensemble is a Julia loop of
  * stepforward!(prob, n) over separate Problems (one calcN! per member per stage).
- * gf_ctx_create_batch builds n members from n descriptors (FP64; TwoDNS or
- * SingleLayerQG; square grids with n in 32..512, sizes may differ; one stepper
- * for all; no stochastic ring forcing).  gf_step on the ensemble advances every
- * member with one launch per kernel class and stage covering all of them.
+ * gf_ctx_create_batch builds n members from n descriptors (one precision, FP64 or
+ * FP32, for all; TwoDNS, SingleLayerQG or one-layer MultiLayerQG; square grids with
+ * n in 32..512, sizes may differ; one stepper for all, AB3 included; any forcing
+ * kind but the time-harmonic one).  gf_step on the ensemble advances every member
+ * with one launch per kernel class and stage covering all of them.  With ring
+ * forcing or AB3 every member must be at the same step (GF_EINVAL otherwise): the
+ * forcing's step number and the AB3 start-up are shared.
  * gf_ctx_instance returns a borrowed handle for member i, valid until the
  * ensemble is destroyed (gf_ctx_destroy on it does nothing); every
  * single-problem entry point works on it, and gf_step on it advances that member
@@ -297,6 +300,16 @@ int gf_device_copy(void* dst, const void* src, size_t bytes);
  *  - The context's state and clock are left unchanged. */
 int gf_adjoint_grad(gf_ctx* ctx, const double* target, int nsteps, int checkpoint_every,
                     double* dJ_dzeta0, double* dJ_dnu, double* J);
+
+/* Device memory the tape of gf_adjoint_grad may take (bytes). No counterpart in the reference.
+ * The tape is a pool of segment slots: the forward pass tapes segment s into slot s mod P, so the
+ * last P segments stay taped and the backward sweep recomputes only the earlier ones from their
+ * checkpoints. P is the number of whole segments the budget affords, at least two (one when
+ * other gradients are in flight in the process) and at most the number of segments. One
+ * segment of k steps takes (8 k + 1) fields of kmax_pair * ny complex values, kmax_pair being
+ * the surviving columns rounded up to a pair (172 at 512^2: 565 MB for 50 steps). 0, the
+ * default: the minimum. With 288 GB of HBM a budget near the whole run removes recomputation. */
+int gf_adjoint_tape_budget(gf_ctx* ctx, size_t bytes);
 
 /* Library version string. */
 const char* gf_version(void);

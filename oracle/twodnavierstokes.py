@@ -8,7 +8,7 @@
 Forcing kinds supported by the restatement (the reference accepts any Julia
 closure ``calcF!``): ``None``, a fixed spectral array ``Fh`` (what the
 reference's deterministic tests' closures do, ``test/test_twodnavierstokes.jl:239-243``),
-or a ``("ring", kf, dkf, eps, seed)`` tuple (the stochastic ring of
+or a ``("ring", kf, dkf, eps, seed[, instance])`` tuple (the stochastic ring of
 ``examples/twodnavierstokes_stochasticforcing.jl:53-82`` with the counter-based
 generator of ``oracle.forcing``).
 """
@@ -64,7 +64,8 @@ def addforcing(N, sol, t, clock, vars, params, grid):
     if isinstance(F, tuple) and F[0] == "ring":
         if t == clock.t:  # not a substep (src/twodnavierstokes.jl:311)
             vars.prevsol[...] = sol
-            vars.Fh[...] = _forcing.ring_forcing(grid, params.sqrt_spec, F[4], clock.step, clock.dt)
+            vars.Fh[...] = _forcing.ring_forcing(grid, params.sqrt_spec, F[4], clock.step, clock.dt,
+                                                 F[5] if len(F) > 5 else 0)
     elif callable(F):
         F(vars.Fh, sol, t, clock, vars, params, grid)
     else:
@@ -85,7 +86,7 @@ def Problem(nx=256, ny=None, Lx=2 * np.pi, Ly=None, nu=0.0, nnu=1, mu=0.0, nmu=0
     params = Params(float(nu), int(nnu), float(mu), int(nmu), calcF)
     ring = isinstance(calcF, tuple) and calcF[0] == "ring"
     if ring:
-        _, kf, dkf, eps, _seed = calcF
+        kf, dkf, eps = calcF[1:4]
         params.sqrt_spec = np.sqrt(_forcing.ring_spectrum(grid, kf, dkf, eps))
         stochastic = True
     vars = Vars(grid, forced=calcF is not None, stochastic=stochastic)

@@ -96,8 +96,8 @@ def _slqg(gf, n=64, **kw):
 
 
 @pytest.mark.parametrize("bad,code", [
-    (dict(dtype=1), -2), (dict(model=2, nlayers=2), -2), (dict(ny=64), -2), (dict(nx=1024, ny=1024), -2),
-    (dict(nx=16, ny=16), -2), (dict(forcing=2, dkf=1.0, kf=3.0), -2), (dict(stepper=2), -1), (dict(dt=0.0), -1),
+    (dict(dtype=1), -1), (dict(model=2, nlayers=2), -2), (dict(ny=64), -2), (dict(nx=1024, ny=1024), -2),
+    (dict(nx=16, ny=16), -2), (dict(stepper=6), -1), (dict(stepper=2), -1), (dict(dt=0.0), -1),
 ])
 def test_invalid_ensembles_are_rejected(gf, bad, code):
     lib = gf._lib.lib
