@@ -30,7 +30,7 @@ EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string"
            "gf_device_alloc", "gf_device_free", "gf_device_copy", "gf_use_graphs",
            "gf_set_schedule", "gf_get_schedule", "gf_autotune", "gf_schedules_save", "gf_schedules_load",
            "gf_schedules_clear", "gf_schedules_error", "gf_set_splits", "gf_get_splits",
-           "gf_set_forcing_harmonic", "gf_set_psi", "gf_adjoint_tape_budget")
+           "gf_set_forcing_harmonic", "gf_set_psi", "gf_diagnostic_async_noalloc", "gf_adjoint_tape_budget")
 
 
 class ProblemDesc(ctypes.Structure):
@@ -97,6 +97,7 @@ def _load():
     lib.gf_adjoint_tape_budget.argtypes = [vp, ctypes.c_size_t]
     lib.gf_diagnostic.argtypes = [vp, ip, ctypes.c_double, ip, dp, ip]
     lib.gf_diagnostic_async.argtypes = [vp, ip, ctypes.c_double, ip, vp, ip]
+    lib.gf_diagnostic_async_noalloc.argtypes = [vp, ip, ctypes.c_double, ip, vp, ip]
     lib.gf_snapshot.argtypes = [vp, vp]
     lib.gf_snapshot.restype = ctypes.c_int64
     lib.gf_snapshot_wait.argtypes = [vp, ctypes.c_int64]

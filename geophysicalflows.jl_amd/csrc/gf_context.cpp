@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <shared_mutex>
@@ -160,6 +161,34 @@ hipGraphExec_t capture(hipStream_t s, F&& rec) {
 }  // namespace dw
 // DEVWIDE-END
 
+// Scratch of gf_diagnostic_async_noalloc: the diagnostics' per-workgroup partial sums (at most
+// kScratchBlocks workgroups x DIAG_MAXOUT outputs, 120 KiB), in a static device array that every
+// GPU the library runs on holds until the process ends.  In host code `scratch` names a host-side
+// stand-in, not a device address: its address on a device comes from hipGetSymbolAddress with
+// that device current (scratch_on), and only that address is handed to kernels.
+constexpr int kScratchBlocks = 1024;
+constexpr size_t kScratchDoubles = (size_t)kScratchBlocks * DIAG_MAXOUT;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_addr;   // device -> that device's copy of scratch
+
+// Resolved once per device, with that device current (at context set-up).
+double* scratch_on(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_addr.find(dev);
+  if (it != g_scratch_addr.end()) return it->second;
+  int cur = -1;
+  HIPCHECK(hipGetDevice(&cur));
+  if (cur != dev) HIPCHECK(hipSetDevice(dev));
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (cur != dev) HIPCHECK(hipSetDevice(cur));
+  if (e != hipSuccess || ptr == nullptr)
+    throw GfError(GF_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_addr[dev] = (double*)ptr;
+  return (double*)ptr;
+}
+
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
@@ -254,6 +283,9 @@ struct CtxBase {
   virtual int diagnostic_async(int, double, int, double*, int) {
     throw GfError(GF_EINVAL, "diagnostics are computed per problem; address one member through gf_ctx_instance");
   }
+  virtual int diagnostic_noalloc(int, double, int, double*, int) {
+    throw GfError(GF_EINVAL, "diagnostics are computed per problem; address one member through gf_ctx_instance");
+  }
   virtual void set_tape_budget(size_t) { throw GfError(GF_EUNSUPPORTED, "the adjoint's tape is set per problem"); }
   virtual void adjoint_grad(const double*, int, int, double*, double*, double*) {
     throw GfError(GF_EUNSUPPORTED, "the adjoint is provided for TwoDNavierStokes FP64 ETDRK4 problems");
@@ -311,7 +343,11 @@ struct Ctx final : CtxBase {
   void clear_nonfinite() { *(volatile int*)nf_host = 0; }
 
   bool own_stream = true;
+  int device = 0;
+  double* scratch_dev = nullptr;   // this device's copy of the static scratch (scratch_on)
   explicit Ctx(const gf_problem_desc& desc, hipStream_t shared = nullptr) : d(desc) {
+    HIPCHECK(hipGetDevice(&device));
+    scratch_dev = scratch_on(device);
     if (shared) { stream = shared; own_stream = false; }
     else HIPCHECK(dw::stream_create(&stream));
     nl = d.model == GF_MODEL_MLQG ? d.nlayers : 1;
@@ -1251,6 +1287,29 @@ struct Ctx final : CtxBase {
     launch_flux<T>(f, part, nblocks, stream);
     HIPCHECK(hipGetLastError());
   }
+  // gf_diagnostic_async without any allocation: the same kernels and block count, the partials in
+  // the static scratch of this device instead of diag_part.  Calls that overlap on one GPU race
+  // on the scratch.
+  int diagnostic_noalloc(int kind, double xi, int nxi, double* out_dev, int nout) override {
+    DiagArgs<T> a{};
+    double scale = 1;
+    diag_setup(kind, xi, nxi, a, scale);
+    if (nout < a.nout) throw GfError(GF_EINVAL, "output array too small for this diagnostic");
+    const int nblocks = (int)std::min<long>(1024, (fsz + 255) / 256);
+    if (nblocks > kScratchBlocks || (size_t)nblocks * a.nout > kScratchDoubles)
+      throw GfError(GF_EINVAL, "gf_diagnostic_async_noalloc: " + std::to_string(nblocks) + " x " +
+                               std::to_string(a.nout) + " partial sums exceed the static scratch of " +
+                               std::to_string(kScratchDoubles) + " doubles");
+    if (kind == DIAG_FLUXES && d.model == GF_MODEL_MLQG) {
+      flux_launch(scratch_dev, nblocks);
+    } else {
+      launch_diag<T>(a, scratch_dev, nblocks, stream);
+      HIPCHECK(hipGetLastError());
+    }
+    launch_diag_finish(scratch_dev, nblocks, a.nout, scale, out_dev, stream);
+    HIPCHECK(hipGetLastError());
+    return a.nout;
+  }
   int diagnostic_async(int kind, double xi, int nxi, double* out_dev, int nout) override {
     DiagArgs<T> a{};
     double scale = 1;
@@ -2063,6 +2122,13 @@ int gf_diagnostic_async(gf_ctx* c, int kind, double xi, int nxi, double* out_dev
   if (!out_dev || nout < 1) return GF_EINVAL;
   int n = 0;
   const int rc = guarded(c, [&] { n = c->impl->diagnostic_async(kind, xi, nxi, out_dev, nout); });
+  return rc == GF_OK ? n : rc;
+}
+
+int gf_diagnostic_async_noalloc(gf_ctx* c, int kind, double xi, int nxi, double* out_dev, int nout) {
+  if (!out_dev || nout < 1) return GF_EINVAL;
+  int n = 0;
+  const int rc = guarded(c, [&] { n = c->impl->diagnostic_noalloc(kind, xi, nxi, out_dev, nout); });
   return rc == GF_OK ? n : rc;
 }
 

@@ -265,6 +265,17 @@ int gf_diagnostic(gf_ctx* ctx, int kind, double xi, int nxi, double* out, int no
  * equals gf_diagnostic's bit for bit. Returns the number of values written or a GF_E* code. */
 int gf_diagnostic_async(gf_ctx* ctx, int kind, double xi, int nxi, double* out_dev, int nout);
 
+/* gf_diagnostic_async without any allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory at the call (code under HIP stream capture, serving loops,
+ * harnesses that forbid hipMalloc). gf_diagnostic_async allocates its partial-sum buffer on its
+ * first call; this variant keeps the partial sums in a static device array of the library
+ * (1024 x 15 doubles, 120 KiB per GPU, held until the process ends), whose address on each device
+ * is resolved when the first context on that device is created. Same kernels, same result bit for
+ * bit. It calls no hipMalloc, hipFree or allocating routine. Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (on different streams or threads) race, so
+ * serialise them. Returns the number of values written or a GF_E* code. */
+int gf_diagnostic_async_noalloc(gf_ctx* ctx, int kind, double xi, int nxi, double* out_dev, int nout);
+
 /* Snapshot of the state for output ([FF] Output / saveoutput, which store prob.sol; e.g.
  * examples/twodnavierstokes_decaying.jl:111-115). Queues a copy of sol, in the Julia layout, to
  * the host buffer dst and returns a ticket >= 0 without waiting (or a negative GF_E* code).
