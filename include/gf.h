@@ -318,7 +318,7 @@ int gf_adjoint_grad(gf_ctx* ctx, const double* target, int nsteps, int checkpoin
  * checkpoints. P is the number of whole segments the budget affords, at least two (one when
  * other gradients are in flight in the process) and at most the number of segments. One
  * segment of k steps takes (8 k + 1) fields of kmax_pair * ny complex values, kmax_pair being
- * the surviving columns rounded up to a pair (172 at 512^2: 565 MB for 50 steps). 0, the
+ * the surviving columns rounded up to a pair (170 at 512^2: 559 MB for 50 steps). 0, the
  * default: the minimum. With 288 GB of HBM a budget near the whole run removes recomputation. */
 int gf_adjoint_tape_budget(gf_ctx* ctx, size_t bytes);
 

@@ -140,3 +140,28 @@ def test_workgroup_splits_give_the_same_gradient(gf, n):
     b = gf.TwoDNavierStokes.adjoint_gradient(prob, target, 9, 4)
     assert a[0] == b[0] and a[2] == b[2]
     assert np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("n", [64, 512])
+def test_tape_budget_keeps_segments_and_gives_the_same_gradient(gf, n):
+    # the recomputed tape holds the forward pass's bits, so the number of segments kept resident
+    # (pool slots from the budget) changes no bit of the gradient: 7 segments of 3 steps with 2
+    # (default), 4 and all 7 slots, alone (two streams) and beside another gradient (one stream)
+    from concurrent.futures import ThreadPoolExecutor
+    prob, z0, target = _setup(gf, n, 1e-3, 90)
+    base = gf.TwoDNavierStokes.adjoint_gradient(prob, target, 20, 3)
+    kmax2 = {64: 22, 512: 170}[n]   # surviving columns kr < kmax rounded up to a pair
+    seg_bytes = 16 * kmax2 * n * (8 * 3 + 1)
+    for slots in (4, 7):
+        got = gf.TwoDNavierStokes.adjoint_gradient(prob, target, 20, 3, tape_budget=slots * seg_bytes)
+        assert got[0] == base[0] and got[2] == base[2], slots
+        assert np.array_equal(got[1], base[1]), slots
+    other, _, tgt2 = _setup(gf, n, 1e-3, 91)
+    with ThreadPoolExecutor(max_workers=2) as pool:
+        fa = pool.submit(gf.TwoDNavierStokes.adjoint_gradient, prob, target, 20, 3, True, 4 * seg_bytes)
+        fb = pool.submit(gf.TwoDNavierStokes.adjoint_gradient, other, tgt2, 20, 3)
+        got, _ = fa.result(), fb.result()
+    assert got[0] == base[0] and got[2] == base[2] and np.array_equal(got[1], base[1])
+    # back to the minimum: the pool keeps its size, the gradient its bits
+    got = gf.TwoDNavierStokes.adjoint_gradient(prob, target, 20, 3, tape_budget=0)
+    assert np.array_equal(got[1], base[1])
