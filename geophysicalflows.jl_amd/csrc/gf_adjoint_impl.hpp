@@ -29,7 +29,7 @@ __device__ __forceinline__ void ka1_body(const AdjK1Args<T>& a, int blk, cplx<T>
   constexpr int TPT = NY / P;
   const YMap<TPT> m(threadIdx.x);
   const int t = m.t;
-  const int kr = 2 * (blk * G + m.slot) + m.c;
+  const int kr = 2 * (G == 1 ? pair_of_block(blk) : blk * G + m.slot) + m.c;
   const Geo<T>& g = a.g;
   const bool active = kr < a.ncols && kr < g.nkr;
   const bool krk = kr < g.kmax;
@@ -212,7 +212,7 @@ __device__ __forceinline__ void ka3_body(const AdjK3Args<T>& a, int blk, cplx<T>
   constexpr int TPT = NY / P;
   const YMap<TPT> m(threadIdx.x);
   const int t = m.t;
-  const int kr = 2 * (blk * G + m.slot) + m.c;
+  const int kr = 2 * (G == 1 ? pair_of_block(blk) : blk * G + m.slot) + m.c;
   const Geo<T>& g = a.g;
   const bool active = kr < a.ncols && kr < g.kmax;
   cplx<T>* L = col_buffer<T, NY>(lds, m.slot, m.c);
@@ -322,11 +322,11 @@ void adj_n(int k, const void* args, hipStream_t s) {
   constexpr int G = GY<N, P>::G;
   if (k == 1) {
     const AdjK1Args<T>& a = *(const AdjK1Args<T>*)args;
-    const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
+    const int nblk = pair_blocks(a.ncols, G);
     if (nblk > 0) hipLaunchKernelGGL((ka1_kernel<T, N, P, G>), dim3(nblk, a.fsplit ? 5 : 1), dim3(2 * G * (N / P)), 0, s, a);
   } else {
     const AdjK3Args<T>& a = *(const AdjK3Args<T>*)args;
-    const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
+    const int nblk = pair_blocks(a.ncols, G);
     if (nblk > 0) hipLaunchKernelGGL((ka3_kernel<T, N, P, G>), dim3(nblk), dim3(2 * G * (N / P)), 0, s, a);
   }
 }
@@ -343,7 +343,7 @@ int adj_blocks(int n, int ncols) {
     case 1024: G = GY<1024, PY<1024>::P>::G; break;
     default: G = GY<2048, PY<2048>::P>::G; break;
   }
-  return ((ncols + 1) / 2 + G - 1) / G;
+  return pair_blocks(ncols, G);
 }
 
 template <typename T>

@@ -53,7 +53,7 @@ void aliased_range(int nk, int nkr, double f, int& lo_full, int& hi_full, int& l
   lo_full = iL - 1; hi_full = iR; lo_r = iL - 1;
 }
 
-inline long boff_h(int kr, int j, int ny) { return ((long)(kr >> 1) * ny + j) * 2 + (kr & 1); }
+inline long boff_h(int kr, int j, int ny) { return ((long)(kr / BW) * ny + j) * BW + (kr % BW); }
 
 template <typename F>
 void parallel_for(long n, F f) {
@@ -354,7 +354,7 @@ struct Ctx final : CtxBase {
     HIPCHECK(dw::host_malloc((void**)&nf_host, 64));
     clear_nonfinite();
     const int nx = d.nx, ny = d.ny, nkr = nx / 2 + 1;
-    g.nx = nx; g.ny = ny; g.nkr = nkr; g.nkr2 = (nkr + 1) & ~1; g.nlayers = nl;
+    g.nx = nx; g.ny = ny; g.nkr = nkr; g.nkr2 = (nkr + BW - 1) / BW * BW; g.nlayers = nl;
     int lo_full, hi_full, lo_r, dum1, dum2;
     aliased_range(nx, nkr, d.aliased_fraction, dum1, dum2, lo_r);
     aliased_range(ny, ny / 2 + 1, d.aliased_fraction, lo_full, hi_full, dum1);
@@ -1353,9 +1353,9 @@ struct Ctx final : CtxBase {
   long ckpt_slots = 0;
   size_t tape_budget = 0;    // bytes the pool may take; 0: two slots
   int ka3_blocks = 0;
-  // A tape field holds only the columns kr < kmax (rounded up to a pair): every reader masks,
+  // A tape field holds only the columns kr < kmax (rounded up to a block): every reader masks,
   // and in the blocked layout those columns are a prefix of the field.
-  long kstride() const { return (long)((g.kmax + 1) & ~1) * d.ny; }
+  long kstride() const { return (long)((g.kmax + BW - 1) / BW * BW) * d.ny; }
 
   void adjoint_setup() {
     if (adj_mixed.p) return;

@@ -26,8 +26,8 @@ __global__ __launch_bounds__(DIAG_THREADS) void diag_kernel(const DiagArgs<T> a,
 #pragma unroll
   for (int k = 0; k < DIAG_MAXOUT; ++k) acc[k] = 0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < fsz; i += (long)gridDim.x * blockDim.x) {
-    const int kr = 2 * (int)(i / (2 * ny)) + (int)(i & 1);
-    const int l = (int)((i >> 1) % ny);
+    const int kr = BW * (int)(i / (BW * ny)) + (int)(i % BW);
+    const int l = (int)((i / BW) % ny);
     if (kr >= g.nkr) continue;
     const double w = kr == 0 ? 1.0 : 2.0;
     const double kv = (double)krwave(g, kr), lv = (double)lwave(g, l);
@@ -161,8 +161,8 @@ __global__ __launch_bounds__(DIAG_THREADS) void flux_fields_kernel(const FluxArg
   const int ny = g.ny, n = a.nl;
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.fstride) return;
-  const int kr = 2 * (int)(i / (2 * ny)) + (int)(i & 1);
-  const int l = (int)((i >> 1) % ny);
+  const int kr = BW * (int)(i / (BW * ny)) + (int)(i % BW);
+  const int l = (int)((i / BW) % ny);
   if (kr >= g.nkr) return;
   const T kv = krwave(g, kr), lv = lwave(g, l);
   const T K2 = kv * kv + lv * lv;
@@ -195,8 +195,8 @@ __global__ __launch_bounds__(DIAG_THREADS) void flux_kernel(const FluxArgs<T> a,
 #pragma unroll
   for (int k = 0; k < DIAG_MAXOUT; ++k) acc[k] = 0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.fstride; i += (long)gridDim.x * blockDim.x) {
-    const int kr = 2 * (int)(i / (2 * ny)) + (int)(i & 1);
-    const int y = (int)((i >> 1) % ny);
+    const int kr = BW * (int)(i / (BW * ny)) + (int)(i % BW);
+    const int y = (int)((i / BW) % ny);
     if (kr >= g.nkr) continue;
     const bool edge = kr == 0 || 2 * kr == g.nx;
     const double w = edge ? 1.0 : 2.0;
@@ -243,8 +243,8 @@ __global__ __launch_bounds__(DIAG_THREADS) void pv_from_psi_kernel(const PvArgs<
   const long fs = g.lstride;
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= fs) return;
-  const int kr = 2 * (int)(i / (2 * ny)) + (int)(i & 1);
-  const int l = (int)((i >> 1) % ny);
+  const int kr = BW * (int)(i / (BW * ny)) + (int)(i % BW);
+  const int l = (int)((i / BW) % ny);
   if (kr >= g.nkr) return;
   const T kv = krwave(g, kr), lv = lwave(g, l);
   const T K2 = kv * kv + lv * lv;

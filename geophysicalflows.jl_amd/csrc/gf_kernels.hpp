@@ -3,17 +3,24 @@
 //
 // Device data layout ("column-blocked"): a spectral or mixed field with
 // nkr columns (kr) and ny rows (l or y) is stored as
-//     off(kr, j) = ((kr >> 1) * ny + j) * 2 + (kr & 1)
-// i.e. pairs of kr columns are interleaved and each pair runs contiguously
-// along y.  The y-direction passes (K1, K3) read and write one column pair
-// as one contiguous 32*ny-byte run; the x-direction pass (K2) reads and writes
-// 64-byte pieces (2 rows x 2 columns).  Layers (MultiLayerQG) and extra fields
-// follow at fixed strides.
+//     off(kr, j) = ((kr / BW) * ny + j) * BW + (kr % BW)
+// i.e. blocks of BW kr columns are interleaved and each block runs contiguously
+// along y.  The y-direction passes (K1, K3) read and write column pairs of a
+// block; the x-direction pass (K2) reads and writes 16 BW-byte pieces (one row
+// of a block) in FP64.  Layers (MultiLayerQG) and extra fields follow at fixed
+// strides.
 #pragma once
 #include <stdint.h>
 #include "gf_fft.hpp"
 
 namespace gf {
+
+// Columns per block of the blocked layout: 2 (pairs) or 4 (quads).
+#ifndef GF_BW
+#define GF_BW 2
+#endif
+constexpr int BW = GF_BW;
+static_assert(BW == 2 || BW == 4, "the blocked layout interleaves 2 or 4 columns");
 
 enum Model { M_TWODNS = 0, M_SLQG = 1, M_MLQG = 2 };
 
@@ -40,7 +47,7 @@ constexpr int MAXL = 8;  // max layers
 
 template <typename T>
 struct Geo {
-  int nx, ny, nkr, nkr2;    // nkr2: nkr rounded up to even
+  int nx, ny, nkr, nkr2;    // nkr2: nkr rounded up to a multiple of the block width BW
   int kmax;                 // kr < kmax survive dealias
   int la0, la1;             // l-index range [la0, la1) is aliased
   int nlayers;
@@ -272,6 +279,15 @@ __host__ __device__ inline int ring_box_row(int l, int ny, int fky, int nbl) { r
 // K3 runs one column per workgroup (csplit) when a column's n/p threads fill at least one wave;
 // otherwise csplit is ignored and the column pairs share workgroups.
 constexpr bool k3_single_column(int n, int p) { return n / p >= 64; }
+// Workgroups of a launch with G column pairs per workgroup over ncols columns; with quads and
+// G = 1, a multiple of 16 (pair_of_block puts the two pairs of a block on one XCD).
+constexpr int pair_blocks(int ncols, int G) {
+  const int n = ((ncols + 1) / 2 + G - 1) / G;
+  return (BW == 4 && G == 1) ? 16 * ((n + 15) / 16) : n;
+}
+// Workgroups of a one-column-per-workgroup launch over ncols columns (single_column): the BW
+// columns of a block go to workgroups 8 apart, on one XCD.
+constexpr int column_blocks(int ncols) { return 8 * BW * (((ncols + BW - 1) / BW + 7) / 8); }
 bool size_supported(int n);
 
 }  // namespace gf

@@ -35,12 +35,17 @@ template <typename T> __device__ __forceinline__ T krwave(const Geo<T>& g, int k
   return (T)kr * g.kx0;
 }
 __device__ __forceinline__ long boff(int kr, int j, int ny) {
-  return ((long)(kr >> 1) * ny + j) * 2 + (kr & 1);
+  return ((long)(kr / BW) * ny + j) * BW + (kr % BW);
 }
 // 32-bit element offset in the column-blocked layout (fields stay below 4 GiB)
 __device__ __forceinline__ uint32_t boff32(int kr, int j, int ny) {
-  return ((uint32_t)(kr >> 1) * (uint32_t)ny + (uint32_t)j) * 2u + (uint32_t)(kr & 1);
+  return ((uint32_t)(kr / BW) * (uint32_t)ny + (uint32_t)j) * (uint32_t)BW + (uint32_t)(kr % BW);
 }
+// Column pair of pair-workgroup b (one pair per workgroup).  With quads the two pairs of a block
+// go to workgroups 16 (q / 8) + (q mod 8) and that + 8, which the dispatcher hands to one XCD
+// (workgroup b runs on XCD b mod 8): the pair that reads second finds the block's lines in that
+// XCD's L2.  With pairs, workgroup b takes pair b.
+__device__ __forceinline__ int pair_of_block(int b) { return BW == 4 ? 16 * (b / 16) + 2 * (b % 8) + (b / 8) % 2 : b; }
 // Element i of an array whose base pointer is uniform across the workgroup, addressed
 // through a 32-bit byte offset (scalar base + vector offset memory instructions).
 template <typename X> __device__ __forceinline__ const X& at(const X* base, uint32_t i) {
@@ -145,7 +150,7 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
   using FSET = Fields<MODEL, FS>;
   const YMap<TPT> m(threadIdx.x);
   const int t = m.t;
-  const int kr = 2 * (blk * G + m.slot) + m.c;
+  const int kr = 2 * (G == 1 ? pair_of_block(blk) : blk * G + m.slot) + m.c;
   const Geo<T>& g = a.g;
   const bool active = kr < a.ncols && kr < g.nkr;
   // the linearised MultiLayerQG calcN! does not dealias its input (src/multilayerqg.jl:675-683)
@@ -612,11 +617,11 @@ void k2_kernel(const K2Args<T> a) {
 // ---------------------------------------------------------------------------
 // K3: flux -> spectral N + stepper stage epilogue, one column / group
 // ---------------------------------------------------------------------------
-// Column of single-column workgroup b: the two columns of pair p go to workgroups
-// 16 (p / 8) + (p mod 8) and that + 8, which the dispatcher hands to the same XCD (workgroup b
-// runs on XCD b mod 8), so the second read of each 32-byte piece of the blocked layout (one row
-// of the pair) hits in that XCD's L2.
-__device__ __forceinline__ int single_column(int b) { return 2 * (8 * (b / 16) + b % 8) + (b / 8) % 2; }
+// Column of single-column workgroup b: the BW columns of block q go to workgroups
+// 8 BW (q / 8) + (q mod 8) + 8 c, c < BW, which the dispatcher hands to the same XCD (workgroup b
+// runs on XCD b mod 8), so the later reads of each 16 BW-byte piece of the blocked layout (one
+// row of the block) hit in that XCD's L2.
+__device__ __forceinline__ int single_column(int b) { return BW * (8 * (b / (8 * BW)) + b % 8) + (b / 8) % BW; }
 
 // Body of K3 for block blk of column pairs (CPW = 2, G pairs per workgroup) or, with CPW = 1,
 // for the single column single_column(blk) (G = 1).  EPI >= 0 fixes the stage epilogue at
@@ -633,7 +638,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
   if constexpr (CPW == 2) {
     const YMap<TPT> m(threadIdx.x);
     t = m.t;
-    kr = 2 * (blk * G + m.slot) + m.c;
+    kr = 2 * (G == 1 ? pair_of_block(blk) : blk * G + m.slot) + m.c;
     L = col_buffer<T, NY>(lds, m.slot, m.c);
   } else {
     t = threadIdx.x;
@@ -937,7 +942,7 @@ template <int N> struct PCand {
 template <typename T, int N, int NL, int P>
 void k1_np(const K1Args<T>& a, hipStream_t s) {
   constexpr int G = GY<N, P>::G;
-  const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
+  const int nblk = pair_blocks(a.ncols, G);
   if (nblk <= 0) return;
   const int gy = (NL > 1 && a.lsplit) ? NL : ((NL == 1 && a.fsplit && a.fieldset == FS_CALCN) ? 3 : 1);
   const dim3 grid(nblk, gy), blk(2 * G * (N / P));
@@ -1013,14 +1018,13 @@ void k3_np(const K3Args<T>& a, hipStream_t s) {
   if constexpr (NL > 1 && N >= 512 && k3_single_column(N, P)) {
     // multi-layer problems (fixed schedule): one column per workgroup from 512^2, where it
     // measured 3.7% faster over a 3-layer 512^2 ETDRK4 step (0.1915 -> 0.1845 ms)
-    const int npairs = (a.ncols + 1) / 2;
-    const int nblk = 16 * ((npairs + 7) / 8);
-    if (npairs > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, 1, NL, 1>), dim3(nblk, a.lsplit ? NL : 1), dim3(N / P), 0, s, a);
+    const int nblk = column_blocks(a.ncols);
+    if (a.ncols > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, 1, NL, 1>), dim3(nblk, a.lsplit ? NL : 1), dim3(N / P), 0, s, a);
     return;
   }
   if constexpr (NL == 1 && P == PY<N>::P) {
     if (a.model == M_MLQG) {   // one-layer MultiLayerQG: fixed schedule, column pairs
-      const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
+      const int nblk = pair_blocks(a.ncols, G);
       if (nblk > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, G, 1, 2, -1, true>), dim3(nblk), dim3(2 * G * (N / P)), 0, s, a);
       return;
     }
@@ -1030,9 +1034,8 @@ void k3_np(const K3Args<T>& a, hipStream_t s) {
     if (a.csplit) {
       // one column per workgroup, a pair's columns 8 workgroups apart (single_column): at 2048^2
       // 683 workgroups at 4 per CU instead of 342 column pairs at 2 per CU on 256 CUs
-      const int npairs = (a.ncols + 1) / 2;
-      const int nblk = 16 * ((npairs + 7) / 8);
-      if (npairs <= 0) return;
+      const int nblk = column_blocks(a.ncols);
+      if (a.ncols <= 0) return;
       const dim3 grid(nblk), blk(N / P);
       if constexpr (std::is_same<T, double>::value && N == 2048 && P == 8) {
         // FP64 at 2048, P = 8: one kernel per ETDRK4 stage.  Each fits the 4-wave budget of
@@ -1050,7 +1053,7 @@ void k3_np(const K3Args<T>& a, hipStream_t s) {
       return;
     }
   }
-  const int nblk = ((a.ncols + 1) / 2 + G - 1) / G;
+  const int nblk = pair_blocks(a.ncols, G);
   if (nblk > 0)
     hipLaunchKernelGGL((k3_kernel<T, N, P, G, NL>), dim3(nblk, (NL > 1 && a.lsplit) ? NL : 1), dim3(2 * G * (N / P)), 0, s, a);
 }
