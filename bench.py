@@ -7,8 +7,10 @@ injection rate 0.1), zeta0 = 0.  One step is one ETDRK4 step (four calcN!
 evaluations and the stage arithmetic).  All state is resident in HBM.
 
 Output: one JSON line with grid-point-steps/s (``value``), the roofline of
-the dominant kernel (per-launch durations from HIP events on the library's
-stream, algorithmic bytes from gf_kernel_bytes), the whole-step rate against
+the dominant kernel (its execution time from HIP events on the library's
+stream around back-to-back launches on the state the timed steps left, the
+per-launch events inside the steps beside it; algorithmic bytes from
+gf_kernel_bytes), the whole-step rate against
 the 64 S-per-step algorithmic traffic model, and the CPU baseline (the NumPy /
 SciPy restatement in oracle/, timed on a bounded sample on this host).
 """
@@ -34,8 +36,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "twodns_2048_fp64_etdrk4_hbm_traff
 def symbols(sched, splits):
     """Kernel symbols of the FP64 2048^2 headline path, as rocprofv3 names them, under the radix
     schedule sched = (P_K1, P_K2, P_K3) (one thread group per workgroup at this size) and the
-    workgroup splits (K3's sixth template argument is its columns per workgroup, 1 or 2), as a
-    list per kernel class."""
+    workgroup splits (K2 split 2 is the staged x pass; K3's sixth template argument is its
+    columns per workgroup, 1 or 2), as a list per kernel class."""
     p1, p2, p3 = sched
     cpw = 1 if splits[2] and 2048 // p3 >= 64 else 2
     if cpw == 1 and p3 == 8:
@@ -43,8 +45,9 @@ def symbols(sched, splits):
         k3 = [f"k3_kernel<double, 2048, 8, 1, 1, 1, {e}>" for e in (2, 3, 4, 5)]
     else:
         k3 = [f"k3_kernel<double, 2048, {p3}, 1, 1, {cpw}, -1>"]
+    k2 = f"k2_staged_kernel<double, 2048, {p2}>" if splits[1] == 2 else f"k2_kernel<double, 2048, {p2}, 1, 0, 1>"
     return {"y_inverse_K1": [f"k1_kernel<double, 2048, {p1}, 1, 1, 0, 0>"],
-            "x_pass_K2": [f"k2_kernel<double, 2048, {p2}, 1, 0, 1>"],
+            "x_pass_K2": [k2],
             "y_forward_stage_K3": k3}
 
 
@@ -63,6 +66,9 @@ def measured_traffic(sched, splits):
     except (OSError, IndexError, KeyError, ValueError):
         return {}
 
+
+# back-to-back launches per kernel class for the execution times behind the roofline
+KERNEL_REPS = 50
 
 # Steps before the untimed warm-up steps, so that the timed steps do not start on a GPU whose
 # clocks are still ramping up from idle (about 0.1 s of work at 2048^2)
@@ -104,7 +110,8 @@ def gpu_run(gf, steps, warmup, T, dt=CFG["dt"]):
     t1 = time.perf_counter()
     elapsed = t1 - t0
     finite = bool(np.isfinite(prob.sol).all())
-    # second pass with per-launch HIP events on the library's stream
+    # second pass with an event bound to each launch on the library's stream (gf_timing): a
+    # launch's interval also holds the dispatch gap that its profiled completion adds
     lib.gf_timing(prob._ctx, 1)
     _step(gf, prob, steps)
     _sync(gf, prob)
@@ -116,10 +123,16 @@ def gpu_run(gf, steps, warmup, T, dt=CFG["dt"]):
         b0, b1 = ctypes.c_double(), ctypes.c_double()
         lib.gf_kernel_bytes(prob._ctx, k, 0, ctypes.byref(b0))
         lib.gf_kernel_bytes(prob._ctx, k, 1, ctypes.byref(b1))
-        per_launch = ((steps - 1) * b0.value + b1.value) / steps
-        avg_s = ms.value * 1e-3 / max(n.value, 1)
-        kern[name] = dict(launches=n.value, avg_us=avg_s * 1e6, bytes_per_launch=per_launch,
-                          achieved_GBps=per_launch / avg_s / 1e9)
+        in_step_s = ms.value * 1e-3 / max(n.value, 1)
+        # the kernel's execution time: KERNEL_REPS launches back to back on the state the timed
+        # steps left, between two events on the same stream (gf_time_kernel), on the retained
+        # columns of an interior step (K3 with the calcN epilogue)
+        iso = ctypes.c_double()
+        gf._lib.check(lib.gf_time_kernel(prob._ctx, k, KERNEL_REPS, ctypes.byref(iso)), prob._ctx)
+        avg_s = iso.value * 1e-3
+        kern[name] = dict(launches=n.value, avg_us=avg_s * 1e6, bytes_per_launch=b0.value,
+                          achieved_GBps=b0.value / avg_s / 1e9, timed_launches=KERNEL_REPS,
+                          in_step_event_us=in_step_s * 1e6)
     lib.gf_timing(prob._ctx, 0)
     kern["schedule"] = list(gf.schedule(prob))
     kern["splits"] = list(gf.splits(prob))
@@ -488,7 +501,9 @@ def main():
                    "schedule_source": "geophysicalflows.jl_amd/schedules.json (autotuned on MI355X)"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["achieved_GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dk["achieved_GBps"] / HBM_PEAK_GBS,
-                     "bytes_per_launch": dk["bytes_per_launch"], "avg_us": dk["avg_us"], "traffic": traffic.get(dom),
+                     "bytes_per_launch": dk["bytes_per_launch"], "avg_us": dk["avg_us"],
+                     "timed": f"{KERNEL_REPS} back-to-back launches after the timed steps (gf_time_kernel)",
+                     "in_step_event_us": dk["in_step_event_us"], "traffic": traffic.get(dom),
                      "traffic_source": "profiles/twodns_2048_fp64_etdrk4_hbm_traffic.md" if traffic else None},
         "kernels": kern,
         "step_model": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,

@@ -24,7 +24,7 @@ EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string"
            "gf_set_state", "gf_get_state", "gf_set_state_dev", "gf_get_state_dev",
            "gf_set_physical", "gf_updatevars", "gf_num_vars", "gf_calcN", "gf_set_forcing",
            "gf_step", "gf_clock", "gf_set_clock", "gf_synchronize", "gf_timing",
-           "gf_timing_read", "gf_kernel_bytes", "gf_version", "gf_ctx_create_batch",
+           "gf_timing_read", "gf_time_kernel", "gf_kernel_bytes", "gf_version", "gf_ctx_create_batch",
            "gf_batch_size", "gf_ctx_instance", "gf_adjoint_grad", "gf_diagnostic",
            "gf_diagnostic_async", "gf_snapshot", "gf_snapshot_wait", "gf_host_alloc", "gf_host_free",
            "gf_device_alloc", "gf_device_free", "gf_device_copy", "gf_use_graphs",
@@ -88,6 +88,7 @@ def _load():
     lib.gf_synchronize.argtypes = [vp]
     lib.gf_timing.argtypes = [vp, ip]
     lib.gf_timing_read.argtypes = [vp, ip, i64p, dp]
+    lib.gf_time_kernel.argtypes = [vp, ip, ip, dp]
     lib.gf_kernel_bytes.argtypes = [vp, ip, ip, dp]
     lib.gf_version.restype = ctypes.c_char_p
     lib.gf_ctx_create_batch.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ProblemDesc), ip]

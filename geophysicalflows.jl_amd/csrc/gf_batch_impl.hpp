@@ -84,8 +84,8 @@ void batch_kernel(const BatchItem<T>* __restrict__ items, const BatchWork* __res
 template <typename T, int K>
 void launch_batch(const BatchItem<T>* items, const BatchWork* work, int nwork, int plain, hipStream_t s) {
   if (nwork <= 0) return;
-  if (K == 2 && plain) hipLaunchKernelGGL((batch_kernel<T, K, true>), dim3(nwork), dim3(BATCH_THREADS), 0, s, items, work);
-  else hipLaunchKernelGGL((batch_kernel<T, K, false>), dim3(nwork), dim3(BATCH_THREADS), 0, s, items, work);
+  if (K == 2 && plain) GF_LAUNCH((batch_kernel<T, K, true>), dim3(nwork), dim3(BATCH_THREADS), 0, s, items, work);
+  else GF_LAUNCH((batch_kernel<T, K, false>), dim3(nwork), dim3(BATCH_THREADS), 0, s, items, work);
 }
 template void launch_batch<double, GF_BATCH_K>(const BatchItem<double>*, const BatchWork*, int, int, hipStream_t);
 template void launch_batch<float, GF_BATCH_K>(const BatchItem<float>*, const BatchWork*, int, int, hipStream_t);

@@ -206,27 +206,53 @@ struct DevBuf {
   template <typename X> X* as() const { return (X*)p; }
 };
 
-// HIP events around each launch of kernel classes 0..2 (K1, K2, K3), summed on read.
+// Per-launch times of kernel classes 0..2 (K1, K2, K3), summed on read. Between rec(k, true) and
+// rec(k, false) the class's launch carries an event (launch_events) that the runtime stamps with
+// the kernel's end; a launch's interval runs from the end of the timed launch before it on the
+// stream (or from a marker, mark(), where other work came in between) to its own end: the
+// kernel's execution plus the short dispatch gap after its predecessor, with no marker packets
+// between the timed kernels. Events recorded on the stream around each launch would add the
+// queue's processing of those packets (about 4-5 us per kernel at 2048^2).
 struct KernelTimer {
-  struct Ev { int k; hipEvent_t a, b; };
+  struct Ivl { int k; size_t a, b; };   // class, indices of the events that open and close it
   bool on = false;
-  std::vector<Ev> evs;
+  std::vector<hipEvent_t> ev;
+  std::vector<Ivl> ivl;
+  long last = -1;                       // event closing the previous interval, -1: none yet
   int64_t launches[3] = {0, 0, 0};
   double ms[3] = {0, 0, 0};
   ~KernelTimer() { drop(); }
   void drop() {
-    for (auto& e : evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
-    evs.clear();
+    launch_events = LaunchEvents{};
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    ivl.clear();
+    last = -1;
+  }
+  hipEvent_t fresh() {
+    hipEvent_t e = nullptr;
+    // no system-scope release at the kernel's end: with it the end of a timed kernel waits for a
+    // cache write-back that an untimed kernel does not (about 4.5 us per kernel at 2048^2)
+    HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    ev.push_back(e);
+    return e;
+  }
+  // work not timed (a copy, another kernel, the host) may precede the next timed launch
+  void mark(hipStream_t s) {
+    if (!on) return;
+    HIPCHECK(hipEventRecord(fresh(), s));
+    last = (long)ev.size() - 1;
   }
   void rec(int k, bool start, hipStream_t s) {
     if (!on) return;
     if (start) {
-      Ev e; e.k = k;
-      HIPCHECK(hipEventCreate(&e.a)); HIPCHECK(hipEventCreate(&e.b));
-      HIPCHECK(hipEventRecord(e.a, s));
-      evs.push_back(e);
+      if (last < 0) mark(s);
+      hipEvent_t e = fresh();
+      ivl.push_back(Ivl{k, (size_t)last, ev.size() - 1});
+      launch_events = LaunchEvents{nullptr, e};
     } else {
-      HIPCHECK(hipEventRecord(evs.back().b, s));
+      launch_events = LaunchEvents{};
+      last = (long)ivl.back().b;
     }
   }
   void reset(int enable, hipStream_t s) {
@@ -238,10 +264,11 @@ struct KernelTimer {
   void read(int k, int64_t* n, double* t, hipStream_t s) {
     if (k < 0 || k > 2) throw GfError(GF_EINVAL, "timing class must be 0, 1 or 2");
     HIPCHECK(hipStreamSynchronize(s));
-    for (auto& e : evs) {
+    for (auto& i : ivl) {
       float x = 0;
-      HIPCHECK(hipEventElapsedTime(&x, e.a, e.b));
-      launches[e.k] += 1; ms[e.k] += x;
+      // a class call that launched nothing (no columns to transform) left its event unrecorded
+      if (hipEventElapsedTime(&x, ev[i.a], ev[i.b]) != hipSuccess) { (void)hipGetLastError(); continue; }
+      launches[i.k] += 1; ms[i.k] += x;
     }
     drop();
     *n = launches[k]; *t = ms[k];
@@ -263,6 +290,9 @@ struct CtxBase {
   virtual int nvars() const = 0;
   virtual void timing(int enable) = 0;
   virtual void timing_read(int k, int64_t* n, double* ms) = 0;
+  virtual void time_kernel(int, int, double*) {
+    throw GfError(GF_EUNSUPPORTED, "kernel classes are timed per problem; address one member through gf_ctx_instance");
+  }
   virtual double kernel_bytes(int k, int full) const = 0;
   virtual void sync() = 0;
   virtual int64_t snapshot(void*) { throw GfError(GF_EINVAL, "snapshots are taken per problem; address one member through gf_ctx_instance"); }
@@ -853,17 +883,26 @@ struct Ctx final : CtxBase {
   // sched[k]: FFT points per thread of K1, K2, K3 (0: the built-in choice); single-layer models.
   int sched[3] = {0, 0, 0};
   // split[0]: K1 transforms one field per workgroup row; split[1]: K2 forms u q and v q in two
-  // workgroup rows: more, shorter workgroups for grids too small to fill the GPU. split[2]: K3
+  // workgroup rows (1): more, shorter workgroups for grids too small to fill the GPU, or walks row
+  // pairs with each field's rows staged through LDS one field ahead (2, k2_staged_kernel; where
+  // k2_staged_offered does not hold the unsplit pass runs). split[2]: K3
   // takes one column per workgroup instead of a column pair, where a column fills a wave
   // (k3_single_column): 683 workgroups at 4 per CU balance 256 CUs better than 342 at 2.
   int split[3] = {0, 0, 0};
   // TwoDNavierStokes and SingleLayerQG take radix schedules and workgroup splits; MultiLayerQG
   // (any number of layers) has a fixed schedule
   bool tunable() const { return d.model != GF_MODEL_MLQG; }
+  // the staged x pass: advection without background gradients, whole waves per workgroup
+  bool k2_staged_offered(int p) const { return nl == 1 && BW == 2 && !advect_full && d.nx / p >= 64; }
   void set_splits(const int* v) override {
     if (!tunable() && (v[0] || v[1] || v[2])) throw GfError(GF_EINVAL, "workgroup splits are for TwoDNavierStokes and SingleLayerQG problems");
+    if (v[1] < 0 || v[1] > 2) throw GfError(GF_EINVAL, "K2 split must be 0, 1 (halves) or 2 (staged loads)");
     bool changed = false;
-    for (int k = 0; k < 3; ++k) { changed |= split[k] != (v[k] != 0); split[k] = v[k] != 0; }
+    for (int k = 0; k < 3; ++k) {
+      const int x = k == 1 ? v[k] : (v[k] != 0);
+      changed |= split[k] != x;
+      split[k] = x;
+    }
     if (changed)
       for (auto& ge : gexec) if (ge) { HIPCHECK(hipStreamSynchronize(stream)); (void)dw::exec_destroy(ge); ge = nullptr; }
   }
@@ -883,17 +922,17 @@ struct Ctx final : CtxBase {
   void get_schedule(int* p) const override {
     for (int k = 0; k < 3; ++k) p[k] = sched[k] ? sched[k] : (nl == 1 ? sched_default(sched_len(k)) : (d.ny >= 32 && k != 1 ? 4 : sched_default(sched_len(k))));
   }
-  // Times whole steps under every offered schedule -- P in {4, 8, 16} for each of K1, K2, K3
-  // and the three workgroup splits on or off, up to 216 combinations -- in two passes (the second
-  // in reverse order, best of the two), keeps the fastest and records it for this (model, size,
-  // dtype) so that later contexts start with it. The state, its stepper history and the clock
-  // are restored afterwards. us (nullable) receives 216 step times in microseconds, combination
-  // c = ((((i1 3 + i2) 3 + i3) 2 + split_K1) 2 + split_K2) 2 + split_K3 with i = 0, 1, 2 for
-  // P = 4, 8, 16, NaN where a combination is not offered (split_K3 = 1 is offered where
-  // k3_single_column holds for P_K3).
+  // Times whole steps under every offered schedule -- P in {4, 8, 16} for each of K1, K2, K3,
+  // the K1 and K3 splits on or off and the K2 split 0, 1 or 2, up to 324 combinations -- in two
+  // passes (the second in reverse order, best of the two), keeps the fastest and records it for
+  // this (model, size, dtype) so that later contexts start with it. The state, its stepper
+  // history and the clock are restored afterwards. us (nullable) receives 324 step times in
+  // microseconds, combination c = ((((i1 3 + i2) 3 + i3) 2 + split_K1) 3 + split_K2) 2 + split_K3
+  // with i = 0, 1, 2 for P = 4, 8, 16, NaN where a combination is not offered (split_K3 = 1 where
+  // k3_single_column holds for P_K3, split_K2 = 2 where k2_staged_offered holds for P_K2).
   void autotune(int steps, double* us) override {
     if (steps < 1) throw GfError(GF_EINVAL, "gf_autotune: steps must be >= 1");
-    constexpr int NC = 216;
+    constexpr int NC = 324;
     if (us) for (int j = 0; j < NC; ++j) us[j] = std::nan("");
     if (!tunable()) return;   // MultiLayerQG keeps its fixed schedule
     HIPCHECK(hipStreamSynchronize(stream));
@@ -914,8 +953,8 @@ struct Ctx final : CtxBase {
     const int old_split[3] = {split[0], split[1], split[2]};
     static constexpr int PV[3] = {4, 8, 16};
     auto decode = [](int c, int* q, int* v) {
-      v[2] = c & 1; v[1] = (c >> 1) & 1; v[0] = (c >> 2) & 1;
-      int r = c >> 3;
+      v[2] = c % 2; v[1] = (c / 2) % 3; v[0] = (c / 6) % 2;
+      int r = c / 12;
       q[2] = PV[r % 3]; r /= 3;
       q[1] = PV[r % 3]; r /= 3;
       q[0] = PV[r % 3];
@@ -933,6 +972,7 @@ struct Ctx final : CtxBase {
       bool ok = true;
       for (int k = 0; k < 3; ++k) ok &= sched_candidate(sched_len(k), q[k]);
       ok &= !v[2] || k3_single_column(sched_len(2), q[2]);
+      ok &= v[1] != 2 || k2_staged_offered(q[1]);
       if (ok) combos.push_back(c);
     }
     double best_t[NC];
@@ -1022,6 +1062,7 @@ struct Ctx final : CtxBase {
     int epis[4];
     const int ns = stages(ins, epis);
     bool counter_set = false;
+    tm.mark(stream);
     for (int s = 0; s < n;) {
       const bool full = s == n - 1;
       // the work diagnostics of stochastic forcing use the state at the start of the step
@@ -1030,6 +1071,7 @@ struct Ctx final : CtxBase {
         if (!prevsol.p) prevsol.alloc(sizeof(C) * fsz * nl);
         HIPCHECK(hipMemcpyAsync(prevsol.p, sol.p, sizeof(C) * fsz * nl, hipMemcpyDeviceToDevice, stream));
         prevsol_valid = true;
+        tm.mark(stream);
       }
       int done = 1;
       if (!full && graph_usable()) {
@@ -1127,6 +1169,7 @@ struct Ctx final : CtxBase {
     HIPCHECK(hipMemcpyAsync(jl.p, solh, bytes, hipMemcpyHostToDevice, stream));
     launch_from_julia<T>(g, jl.as<C>(), cinp.as<C>(), nl, stream);
     if (harmonic) cur_fscale = (T)std::cos(f_omega * tt + f_phase);   // calcF!(Fh, sol, t, ...) at t
+    tm.mark(stream);
     calcN_launch(cinp.as<C>(), E_CALCN, true);
     cur_fscale = 1;
     launch_to_julia<T>(g, nout.as<C>(), jl.as<C>(), nl, stream);
@@ -1154,6 +1197,41 @@ struct Ctx final : CtxBase {
   }
   void timing(int enable) override { tm.reset(enable, stream); }
   void timing_read(int k, int64_t* n, double* ms) override { tm.read(k, n, ms, stream); }
+  // Kernel class k (0 = K1, 1 = K2, 2 = K3 with the calcN epilogue) launched `reps` times back to
+  // back on the current state between two events. Consecutive launches of one kernel follow
+  // each other without a gap, so the mean is the kernel's execution time (what a kernel trace
+  // reports), which an event bound to each launch inside a step overstates by the dispatch gap
+  // that its completion signal adds. The state is not changed: K1 writes the mixed fields, K2
+  // the fluxes, K3 the calcN output buffer.
+  void time_kernel(int k, int reps, double* ms) override {
+    if (k < 0 || k > 2) throw GfError(GF_EINVAL, "kernel class must be 0, 1 or 2");
+    if (reps < 1) throw GfError(GF_EINVAL, "gf_time_kernel: reps must be >= 1");
+    K1Args<T> a; K2Args<T> b; K3Args<T> c;
+    make_args(sol.as<C>(), E_CALCN, false, a, b, c);
+    const bool was = tm.on;
+    tm.on = false;
+    launch_k1<T>(d.ny, a, stream);           // the inputs of K2 and K3 from this state
+    launch_k2<T>(d.nx, b, stream);
+    hipEvent_t e[2] = {nullptr, nullptr};
+    auto done = [&] { for (auto& x : e) if (x) { (void)hipEventDestroy(x); x = nullptr; } tm.on = was; };
+    try {
+      HIPCHECK(hipEventCreate(&e[0]));
+      HIPCHECK(hipEventCreate(&e[1]));
+      HIPCHECK(hipEventRecord(e[0], stream));
+      for (int r = 0; r < reps; ++r) {
+        if (k == 0) launch_k1<T>(d.ny, a, stream);
+        else if (k == 1) launch_k2<T>(d.nx, b, stream);
+        else launch_k3<T>(d.ny, c, stream);
+      }
+      HIPCHECK(hipEventRecord(e[1], stream));
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipEventSynchronize(e[1]));
+      float x = 0;
+      HIPCHECK(hipEventElapsedTime(&x, e[0], e[1]));
+      *ms = (double)x / reps;
+    } catch (...) { done(); throw; }
+    done();
+  }
   // compulsory HBM bytes per launch of kernel class k
   double kernel_bytes(int k, int full) const override {
     const double cs = sizeof(C), rs = sizeof(T);
@@ -1426,7 +1504,7 @@ struct Ctx final : CtxBase {
         AdjK2Args<T> b{};
         b.g = g; b.in = adj_mixed.as<C>(); b.fstride = fsz; b.kin = g.kmax;
         b.out = adj_flux.as<C>(); b.ostride = fsz; b.kout = g.kmax; b.tw = twx.as<C>();
-        b.hsplit = split[1];
+        b.hsplit = split[1] != 0;
         AdjK3Args<T> k{};
         k.g = g; k.in = adj_flux.as<C>(); k.fstride = fsz; k.ncols = g.kmax; k.stage = stage;
         k.lam = lamb.as<C>(); k.l0 = l0b.as<C>(); k.l1 = l1b.as<C>();
@@ -1685,6 +1763,7 @@ struct BatchCtx final : CtxBase {
   }
 
   void step_launches(int full) {
+    tm.mark(stream);
     for (int k = 0; k < nst; ++k) {
       const BatchItem<T>* it = items[k][full].as<BatchItem<T>>();
       tm.rec(0, true, stream);
@@ -2029,6 +2108,10 @@ int gf_timing(gf_ctx* c, int enable) { return guarded(c, [&] { c->impl->timing(e
 int gf_timing_read(gf_ctx* c, int k, int64_t* n, double* ms) {
   if (!n || !ms) return GF_EINVAL;
   return guarded(c, [&] { c->impl->timing_read(k, n, ms); });
+}
+int gf_time_kernel(gf_ctx* c, int k, int reps, double* ms) {
+  if (!ms) return GF_EINVAL;
+  return guarded(c, [&] { c->impl->time_kernel(k, reps, ms); });
 }
 int gf_diagnostic(gf_ctx* c, int kind, double xi, int nxi, double* out, int nout) {
   if (!out || nout < 1) return GF_EINVAL;

@@ -207,8 +207,20 @@ __device__ __forceinline__ cplx<T>* col_buffer(cplx<T>* lds, int slot, int c) {
   return lds + slot * PairLds<T, N>::value + c * (LdsN<N>::value + ColSkew<T>::value);
 }
 
-// One Stockham pass (index PASS, stride NS) on C transforms, then recurse.
-template <typename T, int N, int P, int C, int PASS, int NS>
+// Workgroup barrier for an exchange through LDS alone: this wave's LDS accesses complete, then the
+// barrier. Unlike __syncthreads() it carries no memory fence, so it does not wait for the wave's
+// outstanding global accesses, among them loads into LDS (global_load_lds) into another buffer
+// that are meant to stay in flight across the exchange. The "memory" clobber keeps the compiler
+// from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+template <bool LIGHT> __device__ __forceinline__ void xbarrier() {
+  if constexpr (LIGHT) lds_barrier();
+  else __syncthreads();
+}
+
+// One Stockham pass (index PASS, stride NS) on C transforms, then recurse.  LIGHT: exchange
+// barriers without a memory fence (lds_barrier).
+template <typename T, int N, int P, int C, int PASS, int NS, bool LIGHT = false>
 __device__ __forceinline__ void stockham(cplx<T> (&v)[C][P], cplx<T>* __restrict__ lds, int t,
                                          const cplx<T>* __restrict__ tw) {
   using PL = Plan<N, P>;
@@ -252,13 +264,13 @@ __device__ __forceinline__ void stockham(cplx<T> (&v)[C][P], cplx<T>* __restrict
 #pragma unroll
         for (int r = 0; r < R; ++r) wp[pad(r * NS)] = v[c][b + r * M];
       }
-      __syncthreads();
+      xbarrier<LIGHT>();
       const cplx<T>* rp = lds + pad(t);
 #pragma unroll
       for (int q = 0; q < P; ++q) v[c][q] = rp[pad(TPT * q)];
-      __syncthreads();
+      xbarrier<LIGHT>();
     }
-    stockham<T, N, P, C, PASS + 1, NS * R>(v, lds, t, tw);
+    stockham<T, N, P, C, PASS + 1, NS * R, LIGHT>(v, lds, t, tw);
   }
 }
 
@@ -274,14 +286,14 @@ __device__ __forceinline__ void stockham(cplx<T> (&v)[C][P], cplx<T>* __restrict
 // (147 instead of 122), one wave per SIMD fewer.  Recomputing them costs a few integer
 // operations per pass, 1-5% of a kernel's time where the registers it frees do not let more
 // workgroups run at once, so kernels ask for it only at sizes where they do.
-template <typename T, int N, int P, int C, bool FRESH = false>
+template <typename T, int N, int P, int C, bool FRESH = false, bool LIGHT = false>
 __device__ __forceinline__ void fft_forward(cplx<T> (&v)[C][P], cplx<T>* __restrict__ lds, int t,
                                             const cplx<T>* __restrict__ tw) {
   if constexpr (FRESH) {
     asm volatile("" : "+v"(t));
     __builtin_assume(t >= 0 && t < N / P);   // the asm hides t's range from the multiplies below
   }
-  stockham<T, N, P, C, 0, 1>(v, lds, t, tw);
+  stockham<T, N, P, C, 0, 1, LIGHT>(v, lds, t, tw);
 }
 
 }  // namespace gf

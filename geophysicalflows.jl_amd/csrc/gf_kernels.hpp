@@ -11,9 +11,26 @@
 // strides.
 #pragma once
 #include <stdint.h>
+#include <hip/hip_ext.h>
 #include "gf_fft.hpp"
 
 namespace gf {
+
+// Events that the next K1, K2, K3 or batched launch on this thread carries (kernel timing):
+// hipExtLaunchKernel stamps `stop` with the kernel's end (and `start`, if set, marks the stream
+// just before the launch). Timing by the ends of consecutive kernels leaves no event packets
+// between them, which would add the queue's processing to each interval (about 4-5 us per
+// kernel at 2048^2).
+struct LaunchEvents { hipEvent_t start = nullptr, stop = nullptr; };
+inline thread_local LaunchEvents launch_events;
+#define GF_LAUNCH(K, GRID, BLK, SHM, S, ...)                                                        \
+  do {                                                                                              \
+    if (::gf::launch_events.stop)                                                                   \
+      hipExtLaunchKernelGGL(K, GRID, BLK, SHM, S, ::gf::launch_events.start, ::gf::launch_events.stop, \
+                            0u, __VA_ARGS__);                                                       \
+    else                                                                                            \
+      hipLaunchKernelGGL(K, GRID, BLK, SHM, S, __VA_ARGS__);                                        \
+  } while (0)
 
 // Columns per block of the blocked layout: 2 (pairs) or 4 (quads).
 #ifndef GF_BW

@@ -451,7 +451,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
       }
     }
     // with hsplit, workgroup row 0 forms u q and row 1 v q (each transforms q itself)
-    const int half = a.hsplit ? (int)blockIdx.y : 2;
+    const int half = a.hsplit == 1 ? (int)blockIdx.y : 2;
     if (half != 1) {
       cplx<T> w[1][P];
       inverse_rows(fu, w);
@@ -612,6 +612,197 @@ void k2_kernel(const K2Args<T> a) {
   load_twiddles<T, NX, P>(tw, a.tw);
   __syncthreads();
   k2_body<T, NX, P, G, MODE, NL>(a, xcd_block(blockIdx.x, gridDim.x), lds, tw);
+}
+
+// ---------------------------------------------------------------------------
+// K2, staged: the x pass of advection without background gradients, loads one field ahead
+// ---------------------------------------------------------------------------
+// Each workgroup walks row pairs rp, rp + gridDim.x, ...  A field's two rows reach LDS through
+// direct loads into a staging buffer (global_load_lds, 16 bytes per lane), issued one field
+// ahead: the rows of u stream in while q is transformed, those of v while u q is formed and
+// stored, and q of the next row pair while v q is.  The exchanges inside the transforms use
+// lds_barrier, which does not wait for those loads; dma_wait waits for them.  The staging holds
+// the kept columns k < kin of the two rows as 64-byte pieces (both rows of a column pair), so
+// every four lanes load one contiguous piece, a pattern that moves K2's bytes about 1.25x
+// faster than one row per load (22.7 -> 18.3 us for 111.8 MB at 2048^2 FP64, memory alone).
+// The arithmetic is K2_ADVECT's, in the same order; the compiler may contract the products into
+// the first transform pass differently in the two kernels, so FP64 results agree with k2_kernel's
+// under the same P to rounding (about 1e-16 relative), not always bit for bit.
+template <typename T, int NX, int P> struct K2Stage {
+  static constexpr int EPL = 16 / (int)sizeof(cplx<T>);   // complex values per 16-byte lane load
+  static constexpr int SPAN = (NX / P) * EPL;               // slots one load instruction of every wave fills
+  // 4 slots per column pair (k <= NX/2 + 1), rounded up to whole spans
+  static constexpr int SLOTS = (NX + 4 + SPAN - 1) / SPAN * SPAN;
+};
+typedef __attribute__((address_space(1))) void gvoid_t;
+typedef __attribute__((address_space(3))) void lvoid_t;
+
+// Both rows y0, y0 + 1 of the kept columns k < kin of field f into S: slot 4 p + 2 row + c holds
+// column 2 p + c of row y0 + row, i.e. the 64-byte piece of column pair p at rows y0, y0 + 1.
+// Lane L of wave w writes slots j SPAN + 64 EPL w + EPL L .. + EPL - 1 in round j.  Every lane
+// issues every load of a round (lanes past the kept columns re-read the last piece into slots
+// nobody reads): with a load under a lane-dependent branch the compiler waits for it at the
+// branch's end.  The trip count is uniform.  Blocked pair layout only.
+template <typename T, int NX, int P>
+__device__ __forceinline__ void stage_rows(const cplx<T>* f, int y0, int ny, int kin, cplx<T>* S) {
+  static_assert(BW == 2, "the staged x pass reads the pair layout");
+  using KS = K2Stage<T, NX, P>;
+  const int nslots = 4 * ((kin + 1) / 2);
+  const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+  const int rounds = (nslots + KS::SPAN - 1) / KS::SPAN;
+  const char* fb = reinterpret_cast<const char*>(f);
+  for (int j = 0; j < rounds; ++j) {
+    const int base = j * KS::SPAN + wave * 64 * KS::EPL;
+    const int e0 = base + lane * KS::EPL;
+    const int e = e0 < nslots ? e0 : nslots - KS::EPL;
+    const uint32_t off = (((uint32_t)(e >> 2) * (uint32_t)ny + (uint32_t)y0) * 2u + (uint32_t)(e & 3)) * (uint32_t)sizeof(cplx<T>);
+    __builtin_amdgcn_global_load_lds((gvoid_t*)(fb + off), (lvoid_t*)(S + base), 16, 0, 0);
+  }
+}
+// Every wave's staged loads (and earlier global accesses) complete, then a barrier.
+__device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Hermitian-extended entry k of the staged half spectrum of row `row` (herm_load's rule).
+template <typename T, int NX>
+__device__ __forceinline__ cplx<T> staged(const cplx<T>* S, int k, int row, int kin) {
+  const int kk = k <= NX / 2 ? k : NX - k;
+  const cplx<T> v = S[4 * (kk >> 1) + 2 * row + (kk & 1)];
+  const T im = (kk == 0 || kk == NX / 2) ? (T)0 : (k > NX / 2 ? -v.im : v.im);
+  return kk < kin ? mk(v.re, im) : czero<T>();
+}
+
+// Staged rows -> packed inverse input conj(A + i B) for k <= NX/2, the mirrored half through L.
+// (Plain functions rather than lambdas: with the lambdas' captures the compiler lost track of
+// which LDS buffer the staged loads write and waited for them before every LDS read.)
+template <typename T, int NX, int P>
+__device__ __forceinline__ void staged_take(cplx<T> (&w)[1][P], const cplx<T>* __restrict__ S,
+                                            cplx<T>* __restrict__ L, int t, int kin) {
+  constexpr int TPT = NX / P;
+  constexpr int QH = P / 2 + 1;                     // the q with TPT q <= NX/2
+#pragma unroll
+  for (int q = 0; q < QH; ++q) {
+    const int k = t + TPT * q;
+    if (k > NX / 2) continue;
+    const cplx<T> A = staged<T, NX>(S, k, 0, kin), B = staged<T, NX>(S, k, 1, kin);
+    w[0][q] = mk(A.re - B.im, -(A.im + B.re));
+    if (k > 0 && k < NX / 2) L[pad(NX - k)] = mk(A.re + B.im, A.im - B.re);
+  }
+  lds_barrier();
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    if (TPT * q + TPT - 1 <= NX / 2) continue;
+    const int k = t + TPT * q;
+    if (k <= NX / 2) continue;
+    w[0][q] = L[pad(k)];
+  }
+  lds_barrier();   // S and L are free again
+}
+// Forward transform of the packed product rows y0, y0 + 1 and their split into kept columns of o.
+template <typename T, int NX, int P>
+__device__ __forceinline__ void staged_store(cplx<T>* o, cplx<T> (&w)[1][P], int y0, int ny, int kout,
+                                             cplx<T>* __restrict__ L, const cplx<T>* __restrict__ tw, int t) {
+  constexpr int TPT = NX / P;
+  constexpr int MAXI = (NX / 2 + 1 + TPT - 1) / TPT;
+  constexpr bool FR = NX >= FreshFrom<T>::k2;
+  fft_forward<T, NX, P, 1, FR, true>(w, L, t, tw);
+  const MirrorSlots<NX, TPT> ms(t);
+#pragma unroll
+  for (int q = 0; q < P; ++q) L[ms.k(q)] = w[0][q];
+  lds_barrier();
+#pragma unroll
+  for (int i = 0; i < MAXI; ++i) {
+    const int kr = t + i * TPT;
+    if (kr < kout) {
+      const cplx<T> Wk = L[ms.k(i)], Wm = L[ms.m(i)];
+      at(o, boff32(kr, y0, ny)) = split_row(Wk, Wm, 0);
+      at(o, boff32(kr, y0 + 1, ny)) = split_row(Wk, Wm, 1);
+    }
+  }
+  lds_barrier();
+}
+
+template <typename T, int NX, int P>
+__device__ __forceinline__ void k2_staged_body(const K2Args<T>& a, cplx<T>* __restrict__ L, cplx<T>* __restrict__ S,
+                                               const cplx<T>* __restrict__ tw) {
+  constexpr int TPT = NX / P;
+  constexpr bool FR = NX >= FreshFrom<T>::k2;
+  const int t = (int)threadIdx.x;
+  const Geo<T>& g = a.g;
+  const int ny = g.ny, npairs = ny / 2, kin = a.kin;
+  const cplx<T>* fu = a.in;
+  const cplx<T>* fv = a.in + a.fstride;
+  const cplx<T>* fq = a.in + 2 * a.fstride;
+  const int nwg = (int)gridDim.x;
+  const int first = xcd_block((int)blockIdx.x, nwg);
+  if (first < npairs) stage_rows<T, NX, P>(fq, 2 * first, ny, kin, S);
+  for (int rp = first; rp < npairs; rp += nwg) {
+    const int y0 = 2 * rp;
+    T q0[P], q1[P];
+    {
+      cplx<T> w[1][P];
+      dma_wait();
+      staged_take<T, NX, P>(w, S, L, t, kin);
+      stage_rows<T, NX, P>(fu, y0, ny, kin, S);
+      fft_forward<T, NX, P, 1, FR, true>(w, L, t, tw);
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int x = t + TPT * q;
+        q0[q] = w[0][q].re;
+        q1[q] = -w[0][q].im;
+        if (a.eta != nullptr) {
+          q0[q] = q0[q] + at(a.eta, (uint32_t)(y0 * NX + x));
+          q1[q] = q1[q] + at(a.eta, (uint32_t)((y0 + 1) * NX + x));
+        }
+      }
+    }
+    {
+      cplx<T> w[1][P];
+      dma_wait();
+      staged_take<T, NX, P>(w, S, L, t, kin);
+      stage_rows<T, NX, P>(fv, y0, ny, kin, S);
+      fft_forward<T, NX, P, 1, FR, true>(w, L, t, tw);
+#pragma unroll
+      for (int q = 0; q < P; ++q) w[0][q] = mk(w[0][q].re * q0[q], -w[0][q].im * q1[q]);   // u q, both rows
+      staged_store<T, NX, P>(a.out, w, y0, ny, a.kout, L, tw, t);
+    }
+    {
+      cplx<T> w[1][P];
+      dma_wait();
+      staged_take<T, NX, P>(w, S, L, t, kin);
+      if (rp + nwg < npairs) stage_rows<T, NX, P>(fq, 2 * (rp + nwg), ny, kin, S);
+      fft_forward<T, NX, P, 1, FR, true>(w, L, t, tw);
+#pragma unroll
+      for (int q = 0; q < P; ++q) w[0][q] = mk(w[0][q].re * q0[q], -w[0][q].im * q1[q]);   // v q, both rows
+      staged_store<T, NX, P>(a.out + a.ostride, w, y0, ny, a.kout, L, tw, t);
+    }
+  }
+}
+
+template <typename T, int NX, int P>
+__global__ __launch_bounds__(NX / P) __attribute__((amdgpu_waves_per_eu(2)))
+void k2_staged_kernel(const K2Args<T> a) {
+  __shared__ cplx<T> L[LdsN<NX>::value];
+  __shared__ cplx<T> S[K2Stage<T, NX, P>::SLOTS];
+  __shared__ cplx<T> tw[TwPlan<NX, P>::size()];
+  load_twiddles<T, NX, P>(tw, a.tw);
+  __syncthreads();
+  k2_staged_body<T, NX, P>(a, L, S, tw);
+}
+
+// Workgroups of the staged x pass: as many as the CUs hold at once (by LDS), at most one per
+// row pair.
+template <typename T, int NX, int P>
+int k2_staged_blocks(int npairs) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  }
+  constexpr int bytes = (int)sizeof(cplx<T>) * (LdsN<NX>::value + K2Stage<T, NX, P>::SLOTS + TwPlan<NX, P>::size());
+  constexpr int per_cu = 160 * 1024 / bytes > 0 ? 160 * 1024 / bytes : 1;
+  const int n = cus * per_cu;
+  return n < npairs ? n : npairs;
 }
 
 // ---------------------------------------------------------------------------
@@ -947,21 +1138,21 @@ void k1_np(const K1Args<T>& a, hipStream_t s) {
   const int gy = (NL > 1 && a.lsplit) ? NL : ((NL == 1 && a.fsplit && a.fieldset == FS_CALCN) ? 3 : 1);
   const dim3 grid(nblk, gy), blk(2 * G * (N / P));
   if constexpr (NL > 1) {
-    if (a.fieldset == FS_CALCN) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_CALCN>), grid, blk, 0, s, a);
-    else if (a.fieldset == FS_OUT) hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_OUT>), grid, blk, 0, s, a);
-    else hipLaunchKernelGGL((k1_kernel<T, N, P, G, NL, M_MLQG, FS_RAW>), grid, blk, 0, s, a);
+    if (a.fieldset == FS_CALCN) GF_LAUNCH((k1_kernel<T, N, P, G, NL, M_MLQG, FS_CALCN>), grid, blk, 0, s, a);
+    else if (a.fieldset == FS_OUT) GF_LAUNCH((k1_kernel<T, N, P, G, NL, M_MLQG, FS_OUT>), grid, blk, 0, s, a);
+    else GF_LAUNCH((k1_kernel<T, N, P, G, NL, M_MLQG, FS_RAW>), grid, blk, 0, s, a);
   } else if constexpr (P != PY<N>::P) {   // a tuned schedule: the calcN field set only
-    if (a.model == M_TWODNS) hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_CALCN>), grid, blk, 0, s, a);
-    else hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_SLQG, FS_CALCN>), grid, blk, 0, s, a);
+    if (a.model == M_TWODNS) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_CALCN>), grid, blk, 0, s, a);
+    else GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_CALCN>), grid, blk, 0, s, a);
   } else {
     if (a.model == M_TWODNS) {
-      if (a.fieldset == FS_CALCN) hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_CALCN>), grid, blk, 0, s, a);
-      else if (a.fieldset == FS_OUT) hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_OUT>), grid, blk, 0, s, a);
-      else hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_RAW>), grid, blk, 0, s, a);
+      if (a.fieldset == FS_CALCN) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_CALCN>), grid, blk, 0, s, a);
+      else if (a.fieldset == FS_OUT) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_OUT>), grid, blk, 0, s, a);
+      else GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_RAW>), grid, blk, 0, s, a);
     } else {
-      if (a.fieldset == FS_CALCN) hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_SLQG, FS_CALCN>), grid, blk, 0, s, a);
-      else if (a.fieldset == FS_OUT) hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_SLQG, FS_OUT>), grid, blk, 0, s, a);
-      else hipLaunchKernelGGL((k1_kernel<T, N, P, G, 1, M_SLQG, FS_RAW>), grid, blk, 0, s, a);
+      if (a.fieldset == FS_CALCN) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_CALCN>), grid, blk, 0, s, a);
+      else if (a.fieldset == FS_OUT) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_OUT>), grid, blk, 0, s, a);
+      else GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_RAW>), grid, blk, 0, s, a);
     }
   }
 }
@@ -980,17 +1171,23 @@ void k1_n(const K1Args<T>& a, hipStream_t s) {
 template <typename T, int N, int NL, int P>
 void k2_np(const K2Args<T>& a, hipStream_t s) {
   constexpr int G = GW<N, P>::G;
-  const int gy = (NL > 1 && a.lsplit) ? NL : ((NL == 1 && a.hsplit && a.mode == K2_ADVECT) ? 2 : 1);
+  const int gy = (NL > 1 && a.lsplit) ? NL : ((NL == 1 && a.hsplit == 1 && a.mode == K2_ADVECT) ? 2 : 1);
   const dim3 grid((a.g.ny / 2 + G - 1) / G, gy), blk(G * (N / P));
+  if constexpr (NL == 1 && BW == 2 && N / P >= 64) {   // whole waves: the staging is issued per wave
+    if (a.mode == K2_ADVECT && a.hsplit == 2) {   // the staged x pass (a schedule option)
+      GF_LAUNCH((k2_staged_kernel<T, N, P>), dim3(k2_staged_blocks<T, N, P>(a.g.ny / 2)), dim3(N / P), 0, s, a);
+      return;
+    }
+  }
   if constexpr (P != PY<N>::P) {          // a tuned schedule: the calcN modes only
-    if (a.mode == K2_ADVECT) { if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT, 1>), grid, blk, 0, s, a); }
-    else hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT_FULL, NL>), grid, blk, 0, s, a);
+    if (a.mode == K2_ADVECT) { if constexpr (NL == 1) GF_LAUNCH((k2_kernel<T, N, P, G, K2_ADVECT, 1>), grid, blk, 0, s, a); }
+    else GF_LAUNCH((k2_kernel<T, N, P, G, K2_ADVECT_FULL, NL>), grid, blk, 0, s, a);
   } else {
     switch (a.mode) {
-      case K2_ADVECT: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT, 1>), grid, blk, 0, s, a); break;
-      case K2_ADVECT_FULL: hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_ADVECT_FULL, NL>), grid, blk, 0, s, a); break;
-      case K2_INVERSE_OUT: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_INVERSE_OUT, 1>), grid, blk, 0, s, a); break;
-      case K2_FORWARD_IN: if constexpr (NL == 1) hipLaunchKernelGGL((k2_kernel<T, N, P, G, K2_FORWARD_IN, 1>), grid, blk, 0, s, a); break;
+      case K2_ADVECT: if constexpr (NL == 1) GF_LAUNCH((k2_kernel<T, N, P, G, K2_ADVECT, 1>), grid, blk, 0, s, a); break;
+      case K2_ADVECT_FULL: GF_LAUNCH((k2_kernel<T, N, P, G, K2_ADVECT_FULL, NL>), grid, blk, 0, s, a); break;
+      case K2_INVERSE_OUT: if constexpr (NL == 1) GF_LAUNCH((k2_kernel<T, N, P, G, K2_INVERSE_OUT, 1>), grid, blk, 0, s, a); break;
+      case K2_FORWARD_IN: if constexpr (NL == 1) GF_LAUNCH((k2_kernel<T, N, P, G, K2_FORWARD_IN, 1>), grid, blk, 0, s, a); break;
       default: break;
     }
   }
@@ -1019,13 +1216,13 @@ void k3_np(const K3Args<T>& a, hipStream_t s) {
     // multi-layer problems (fixed schedule): one column per workgroup from 512^2, where it
     // measured 3.7% faster over a 3-layer 512^2 ETDRK4 step (0.1915 -> 0.1845 ms)
     const int nblk = column_blocks(a.ncols);
-    if (a.ncols > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, 1, NL, 1>), dim3(nblk, a.lsplit ? NL : 1), dim3(N / P), 0, s, a);
+    if (a.ncols > 0) GF_LAUNCH((k3_kernel<T, N, P, 1, NL, 1>), dim3(nblk, a.lsplit ? NL : 1), dim3(N / P), 0, s, a);
     return;
   }
   if constexpr (NL == 1 && P == PY<N>::P) {
     if (a.model == M_MLQG) {   // one-layer MultiLayerQG: fixed schedule, column pairs
       const int nblk = pair_blocks(a.ncols, G);
-      if (nblk > 0) hipLaunchKernelGGL((k3_kernel<T, N, P, G, 1, 2, -1, true>), dim3(nblk), dim3(2 * G * (N / P)), 0, s, a);
+      if (nblk > 0) GF_LAUNCH((k3_kernel<T, N, P, G, 1, 2, -1, true>), dim3(nblk), dim3(2 * G * (N / P)), 0, s, a);
       return;
     }
   }
@@ -1042,20 +1239,20 @@ void k3_np(const K3Args<T>& a, hipStream_t s) {
         // 128 VGPRs without spilling (120-128); the kernel that switches on the stage needs
         // 137 and spills 8.
         switch (a.epi) {
-          case E_ETD1: hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1, E_ETD1>), grid, blk, 0, s, a); return;
-          case E_ETD2: hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1, E_ETD2>), grid, blk, 0, s, a); return;
-          case E_ETD3: hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1, E_ETD3>), grid, blk, 0, s, a); return;
-          case E_ETD4: hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1, E_ETD4>), grid, blk, 0, s, a); return;
+          case E_ETD1: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD1>), grid, blk, 0, s, a); return;
+          case E_ETD2: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD2>), grid, blk, 0, s, a); return;
+          case E_ETD3: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD3>), grid, blk, 0, s, a); return;
+          case E_ETD4: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD4>), grid, blk, 0, s, a); return;
           default: break;
         }
       }
-      hipLaunchKernelGGL((k3_kernel<T, N, P, 1, 1, 1>), grid, blk, 0, s, a);
+      GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1>), grid, blk, 0, s, a);
       return;
     }
   }
   const int nblk = pair_blocks(a.ncols, G);
   if (nblk > 0)
-    hipLaunchKernelGGL((k3_kernel<T, N, P, G, NL>), dim3(nblk, (NL > 1 && a.lsplit) ? NL : 1), dim3(2 * G * (N / P)), 0, s, a);
+    GF_LAUNCH((k3_kernel<T, N, P, G, NL>), dim3(nblk, (NL > 1 && a.lsplit) ? NL : 1), dim3(2 * G * (N / P)), 0, s, a);
 }
 template <typename T, int N, int NL>
 void k3_n(const K3Args<T>& a, hipStream_t s) {

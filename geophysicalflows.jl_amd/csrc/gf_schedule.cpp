@@ -175,7 +175,8 @@ int sched_load(const std::string& path) {
         throw std::runtime_error("schedule table: 'split' must hold 2 or 3 integers");
       for (int j = 0; j < (int)sp->arr.size(); ++j) {
         s.split[j] = as_int(&sp->arr[j], "split");
-        if (s.split[j] != 0 && s.split[j] != 1) throw std::runtime_error("schedule table: 'split' entries are 0 or 1");
+        if (s.split[j] < 0 || s.split[j] > (j == 1 ? 2 : 1))
+          throw std::runtime_error("schedule table: 'split' entries are 0 or 1 (0, 1 or 2 for K2)");
       }
       if ((s.split[0] || s.split[1] || s.split[2]) && k[1] != 1)
         throw std::runtime_error("schedule table: workgroup splits are for single-layer problems");

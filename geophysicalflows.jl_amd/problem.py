@@ -259,9 +259,11 @@ def schedule(prob):
 
 
 def set_splits(prob, k1_fields=0, k2_halves=0, k3_columns=0):
-    """Workgroup splits of a single-layer problem: K1 one field per workgroup row, K2 u q and v q
-    in two rows (more, shorter workgroups for small grids), K3 one column per workgroup instead
-    of a column pair (where a column fills a wave).  No counterpart in the reference."""
+    """Workgroup splits of a single-layer problem: K1 one field per workgroup row; K2 u q and v q
+    in two rows (1: more, shorter workgroups for small grids) or row pairs walked with each
+    field's rows loaded into LDS one field ahead (2: nx / P_K2 >= 64, no varying U); K3 one
+    column per workgroup instead of a column pair (where a column fills a wave).  No counterpart
+    in the reference."""
     check(lib.gf_set_splits(prob._ctx, int(k1_fields), int(k2_halves), int(k3_columns)), prob._ctx)
 
 
@@ -277,14 +279,14 @@ def autotune(prob, steps=20):
     clock restored), apply the fastest and record it for later problems of the same model, size
     and dtype.  Returns the chosen ``schedule + splits`` 6-tuple and the step time in
     microseconds of each combination tried, keyed the same way."""
-    us = (ctypes.c_double * 216)()
+    us = (ctypes.c_double * 324)()
     check(lib.gf_autotune(prob._ctx, int(steps), us), prob._ctx)
     times = {}
     pv = (4, 8, 16)
-    for c in range(216):
+    for c in range(324):
         if math.isfinite(us[c]):
-            r = c >> 3
-            key = (pv[r // 9], pv[(r // 3) % 3], pv[r % 3], (c >> 2) & 1, (c >> 1) & 1, c & 1)
+            r = c // 12
+            key = (pv[r // 9], pv[(r // 3) % 3], pv[r % 3], (c // 6) % 2, (c // 2) % 3, c % 2)
             times[key] = us[c]
     return schedule(prob) + splits(prob), times
 

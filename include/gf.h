@@ -161,21 +161,25 @@ int gf_get_schedule(gf_ctx* ctx, int* p_k1, int* p_k2, int* p_k3);
 
 /* Workgroup splits for single-layer problems (0 off, 1 on), part of the schedule:
  *   k1_fields:  K1 gives each of u, v, q its own row of workgroups;
- *   k2_halves:  K2 forms u q and v q in two rows of workgroups, each transforming q itself;
+ *   k2_halves:  K2 forms u q and v q in two rows of workgroups, each transforming q itself (1),
+ *               or walks row pairs persistently, each field's rows loaded straight into LDS one
+ *               field ahead of the transforms (2; TwoDNavierStokes and SingleLayerQG without a
+ *               varying U, nx / P_K2 >= 64; the unsplit pass runs elsewhere);
  *   k3_columns: K3 gives each spectral column its own workgroup instead of a column pair, where
  *               a column's ny / P_K3 threads fill a 64-lane wave (ignored elsewhere).
- * Workgroups are regrouped only: the results are the same bit for bit. */
+ * Workgroups are regrouped only: the results are the same bit for bit, except that k2_halves = 2
+ * may round FP64 products differently (agreement to about 1e-16 relative). */
 int gf_set_splits(gf_ctx* ctx, int k1_fields, int k2_halves, int k3_columns);
 int gf_get_splits(gf_ctx* ctx, int* k1_fields, int* k2_halves, int* k3_columns);
 
 /* The autotuner. It times `steps` whole steps under every offered schedule on this problem:
- * P in {4, 8, 16} for each of K1, K2, K3, with each workgroup split on or off, up to 216
- * combinations. It runs two passes, the second in reverse order, keeping each combination's
- * best, then applies the fastest. The state, its stepper history and the clock are restored
- * afterwards. The choice is recorded for (model, nlayers, nx, ny, dtype): contexts created later
- * in the process start with it. us (nullable) receives 216 step times in microseconds, indexed
- * ((((i1 * 3 + i2) * 3 + i3) * 2 + split_K1) * 2 + split_K2) * 2 + split_K3, where i = 0, 1, 2
- * for P = 4, 8, 16; NaN where a combination is not offered. */
+ * P in {4, 8, 16} for each of K1, K2, K3, with the K1 and K3 splits on or off and the K2 split
+ * 0, 1 or 2, up to 324 combinations. It runs two passes, the second in reverse order, keeping
+ * each combination's best, then applies the fastest. The state, its stepper history and the
+ * clock are restored afterwards. The choice is recorded for (model, nlayers, nx, ny, dtype):
+ * contexts created later in the process start with it. us (nullable) receives 324 step times in
+ * microseconds, indexed ((((i1 * 3 + i2) * 3 + i3) * 2 + split_K1) * 3 + split_K2) * 2 + split_K3,
+ * where i = 0, 1, 2 for P = 4, 8, 16; NaN where a combination is not offered. */
 int gf_autotune(gf_ctx* ctx, int steps, double* us);
 
 /* The process-wide table of tuned schedules as JSON ("gf-schedules-1": one entry per (model,
@@ -198,12 +202,25 @@ int gf_set_clock(gf_ctx* ctx, double t, int64_t step);
 /* Wait for all work queued on the context's stream; GF_ENOTFINITE as described at gf_step. */
 int gf_synchronize(gf_ctx* ctx);
 
-/* Per-kernel timing of gf_step with HIP events on the context's stream:
- * gf_timing(ctx, 1) starts recording, gf_timing_read() returns, for kernel
+/* Per-kernel timing of gf_step with HIP events that each launch carries (hipExtLaunchKernel
+ * stamps the event with the kernel's end): a launch's time runs from the end of the timed
+ * launch before it on the context's stream to its own end, i.e. its execution plus the short
+ * dispatch gap, with no event packets between the kernels. gf_timing(ctx, 1) starts recording
+ * (and replays no graphs meanwhile), gf_timing_read() returns, for kernel
  * class k (0 = y-inverse, 1 = x-pass, 2 = y-forward + stage), the number of
  * launches and the summed milliseconds since recording started. */
 int gf_timing(gf_ctx* ctx, int enable);
 int gf_timing_read(gf_ctx* ctx, int k, int64_t* launches, double* ms);
+
+/* Execution time of kernel class k (0 = y-inverse, 1 = x-pass, 2 = y-forward with the calcN
+ * epilogue) on the current state: `reps` launches back to back between two events on the
+ * context's stream, *ms_per_launch = their mean in milliseconds. Back-to-back launches of one
+ * kernel leave no gap between them, so this is the kernel's execution time as a kernel trace
+ * reports it; gf_timing's per-launch events add the dispatch gap that a profiled launch's
+ * completion brings (about 4.5 us per kernel at 2048^2). The state and clock are unchanged.
+ * Single problems only (GF_EUNSUPPORTED for an ensemble). No counterpart in the reference:
+ * a measurement hook, like gf_timing. */
+int gf_time_kernel(gf_ctx* ctx, int k, int reps, double* ms_per_launch);
 
 /* Bytes of HBM each kernel class must move per launch in the fused
  * algorithm (compulsory traffic of its inputs and outputs), for the given

@@ -16,17 +16,19 @@ def test_round_trip(gf, tmp_path):
     src = tmp_path / "in.json"
     src.write_text(json.dumps({"format": "gf-schedules-1",
                                "entries": [_entry(split=[1, 0]), _entry(nx=64, ny=32, p=[4, 8, 0], dtype=1),
-                                           _entry(nx=512, ny=512, split=[1, 1, 1])]}))
-    assert gf.load_schedules(src) == 3
+                                           _entry(nx=512, ny=512, split=[1, 1, 1]),
+                                           _entry(nx=1024, ny=1024, p=[8, 8, 8], split=[1, 2, 1])]}))
+    assert gf.load_schedules(src) == 4
     out = tmp_path / "out.json"
     gf.save_schedules(out)
     got = json.loads(out.read_text())
     assert got["format"] == "gf-schedules-1"
     ents = sorted(got["entries"], key=lambda e: e["nx"])
-    assert [e["p"] for e in ents] == [[4, 8, 0], [4, 8, 4], [4, 8, 4]]
+    assert [e["p"] for e in ents] == [[4, 8, 0], [4, 8, 4], [4, 8, 4], [8, 8, 8]]
     assert ents[1]["step_us"] == 81.25 and ents[0]["ny"] == 32 and ents[0]["dtype"] == 1
     # absent: no splits; two entries: K3's column split off
     assert ents[1]["split"] == [1, 0, 0] and ents[0]["split"] == [0, 0, 0] and ents[2]["split"] == [1, 1, 1]
+    assert ents[3]["split"] == [1, 2, 1]     # K2's staged pass
     gf._lib.lib.gf_schedules_clear()
     gf.save_schedules(out)
     assert json.loads(out.read_text())["entries"] == []
@@ -44,6 +46,8 @@ def test_round_trip(gf, tmp_path):
     ('{"format": "gf-schedules-1", "entries": []} x', "trailing"),
     (json.dumps({"format": "gf-schedules-1", "entries": [_entry(split=[2, 0])]}), "0 or 1"),
     (json.dumps({"format": "gf-schedules-1", "entries": [_entry(split=[0, 0, 3])]}), "0 or 1"),
+    (json.dumps({"format": "gf-schedules-1", "entries": [_entry(split=[0, 3, 0])]}), "0, 1 or 2 for K2"),
+    (json.dumps({"format": "gf-schedules-1", "entries": [_entry(split=[0, 0, 2])]}), "0 or 1"),
     (json.dumps({"format": "gf-schedules-1", "entries": [_entry(split=[0, 0, 0, 1])]}), "2 or 3 integers"),
     (json.dumps({"format": "gf-schedules-1", "entries": [_entry(nlayers=2, p=[0, 0, 0], split=[0, 0, 1])]}), "single-layer"),
     (json.dumps({"format": "gf-schedules-1", "entries": [_entry(nlayers=3, p=[0, 0, 0], split=[1, 0])]}), "single-layer"),
