@@ -273,7 +273,7 @@ def adjoint_run(gf, nsteps=2000, checkpoint=50):
     prob.synchronize()
     tf = time.perf_counter() - t0
     gf.TwoDNavierStokes.set_ζ(prob, z0)
-    # untimed: allocates the 2000-step tape (8 spectral fields per step, 33.8 GB) and the tables
+    # untimed: allocates the 2000-step tape (5 retained-column fields per step, 13.9 GB) and the tables
     gf.TwoDNavierStokes.adjoint_gradient(prob, target, nsteps, nsteps, nu=False)
     out = {"workload": "adjoint TwoDNavierStokes 512^2 FP64 ETDRK4 dt=0.005 nu=1e-3 nnu=1, 2000 steps, "
                        "gradient of 1/2|zeta(T)-target|^2 w.r.t. zeta0 and nu",

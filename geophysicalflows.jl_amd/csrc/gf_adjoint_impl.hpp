@@ -42,6 +42,24 @@ __device__ __forceinline__ void ka1_body(const AdjK1Args<T>& a, int blk, cplx<T>
     if (active && krk && kept_l(g, l)) v = at(f, boff32(kr, l, NY));
     return v;
   };
+  // the stage's calcN input at (kr, l), rebuilt from the taped s0 and Ns in the forward stages'
+  // operation order (the ETD stage epilogues of K3); every lane reads, a select masks
+  auto stage_state = [&](int l) -> cplx<T> {
+    const bool keep = active && krk && kept_l(g, l);
+    const uint32_t ix = active ? boff32(kr, l, NY) : 0u;
+    const cplx<T> s0 = at(a.s0, ix);
+    cplx<T> v = s0;
+    if (a.stage != 1) {
+      const T E2 = at(a.E2, ix), Z = at(a.Z, ix);
+      const cplx<T> nb = at(a.stage == 3 ? a.N2 : a.N1, ix);
+      v = rmul(E2, s0) + rmul(Z, nb);                        // s1 (stage 2, 4) or s2 (stage 3)
+      if (a.stage == 4) {
+        const cplx<T> n3 = at(a.N3, ix);
+        v = rmul(E2, v) + rmul(Z, mk((T)2 * n3.re - nb.re, (T)2 * n3.im - nb.im));   // s3
+      }
+    }
+    return keep ? v : czero<T>();
+  };
 #pragma unroll
   for (int f = 0; f < 5; ++f) {
     if (a.fsplit && f != (int)blockIdx.y) continue;   // one field per workgroup row (uniform)
@@ -51,7 +69,7 @@ __device__ __forceinline__ void ka1_body(const AdjK1Args<T>& a, int blk, cplx<T>
     cplx<T> vs[P];
     T cs[P];
 #pragma unroll
-    for (int q = 0; q < P; ++q) vs[q] = masked(f < 3 ? a.s : a.x, t + TPT * q);
+    for (int q = 0; q < P; ++q) vs[q] = f < 3 ? stage_state(t + TPT * q) : masked(a.x, t + TPT * q);
     if (f >= 3 && a.lamc != nullptr) {
 #pragma unroll
       for (int q = 0; q < P; ++q) cs[q] = at(a.lamc, active ? boff32(kr, t + TPT * q, NY) : 0u);
@@ -256,8 +274,9 @@ __device__ __forceinline__ void ka3_body(const AdjK3Args<T>& a, int blk, cplx<T>
         at(a.lN1, ix) = rmul(Al, lam) - rmul(Z, tv);
         (void)E;
         if (a.nu_part != nullptr) {
-          const cplx<T> s0 = at(a.s0, ix), s1 = at(a.s1, ix);
+          const cplx<T> s0 = at(a.s0, ix);
           const cplx<T> N1 = at(a.N1, ix), N2 = at(a.N2, ix), N3 = at(a.N3, ix), N4 = at(a.N4, ix);
+          const cplx<T> s1 = rmul(E2, s0) + rmul(Z, N1);   // stage 2's input, as the forward formed it
           const cplx<T> d1 = rmul(at(a.dE, ix), s0) + rmul(at(a.dA, ix), N1) +
                              rmul((T)2 * at(a.dB, ix), N2 + N3) + rmul(at(a.dG, ix), N4);
           const cplx<T> d3 = rmul(at(a.dE2, ix), s1) + rmul(at(a.dZ, ix), mk((T)2 * N3.re - N1.re, (T)2 * N3.im - N1.im));

@@ -1462,32 +1462,35 @@ struct Ctx final : CtxBase {
     for (int k = 0; k < 6; ++k) upload_blocked(dtab[k], t6[k]);
   }
 
-  // step i, field f of pool slot tp (fields: s0, s1, s2, s3, N1, N2, N3, N4; step i + 1's s0 follows)
+  // Step i, field f of pool slot tp: fields s0 (the step's input), N1, N2, N3, N4; step i + 1's
+  // s0 follows. The stage inputs s1, s2, s3 are not taped: the adjoint rebuilds them from s0 and
+  // the Ns (E2 s0 + Z N1, E2 s0 + Z N2, E2 s1 + Z (2 N3 - N1)), 5 fields per step instead of 8.
+  static constexpr int TAPE_FIELDS = 5;
   C* slot(long i, int f, int tp = 0) const {
-    return pool.as<C>() + ((size_t)tp * (8 * pool_seg + 1) + (size_t)i * 8 + f) * kstride();
+    return pool.as<C>() + ((size_t)tp * (TAPE_FIELDS * pool_seg + 1) + (size_t)i * TAPE_FIELDS + f) * kstride();
   }
   void set_tape_budget(size_t bytes) override { tape_budget = bytes; }
 
-  // One ETDRK4 step from tape slot i (s0) writing s1, s2, s3 and N1..N4 of that slot and the
-  // new state to next (kept modes only: every later read masks), on stream st. Its scratch
-  // (mixed, flux, n1, n23) is the forward path's; the adjoint kernels use none of it.
+  // One ETDRK4 step from tape slot i (s0) writing N1..N4 of that slot and the new state to next
+  // (kept modes only: every later read masks), on stream st. The stage inputs s1, s2 and s3 (in
+  // s2's buffer, as in a plain step) go to the forward path's buffers, like the rest of its
+  // scratch (mixed, flux, n1, n23); the adjoint kernels use none of them.
   void tape_step(long i, C* next, int tp, hipStream_t st) {
     C* s0 = slot(i, 0, tp);
-    C* s1 = slot(i, 1, tp);
-    C* s2 = slot(i, 2, tp);
-    C* s3 = slot(i, 3, tp);
+    C* b1 = s1.as<C>();
+    C* b2 = s2.as<C>();
     K1Args<T> a; K2Args<T> b; K3Args<T> k;
     auto run = [&]() {
       launch_k1<T>(d.ny, a, st); launch_k2<T>(d.nx, b, st); launch_k3<T>(d.ny, k, st);
     };
     make_args(s0, E_ETD1, false, a, b, k);
-    k.sol = s0; k.s1 = s1; k.nsave = slot(i, 4, tp); run();
-    make_args(s1, E_ETD2, false, a, b, k);
-    k.sol = s0; k.s2 = s2; k.nsave = slot(i, 5, tp); run();
-    make_args(s2, E_ETD3, false, a, b, k);
-    k.s1 = s1; k.s2 = s3; k.nsave = slot(i, 6, tp); run();
-    make_args(s3, E_ETD4, false, a, b, k);
-    k.sol = s0; k.sol_out = next; k.nsave = slot(i, 7, tp); k.nonfinite = nullptr; run();
+    k.sol = s0; k.s1 = b1; k.nsave = slot(i, 1, tp); run();
+    make_args(b1, E_ETD2, false, a, b, k);
+    k.sol = s0; k.s2 = b2; k.nsave = slot(i, 2, tp); run();
+    make_args(b2, E_ETD3, false, a, b, k);
+    k.s1 = b1; k.s2 = b2; k.nsave = slot(i, 3, tp); run();
+    make_args(b2, E_ETD4, false, a, b, k);
+    k.sol = s0; k.sol_out = next; k.nsave = slot(i, 4, tp); k.nonfinite = nullptr; run();
     HIPCHECK(hipGetLastError());
   }
 
@@ -1496,7 +1499,9 @@ struct Ctx final : CtxBase {
     if constexpr (std::is_same<T, double>::value) {
       for (int stage = 4; stage >= 1; --stage) {
         AdjK1Args<T> a{};
-        a.g = g; a.s = slot(i, stage - 1, tp);
+        a.g = g; a.stage = stage;
+        a.s0 = slot(i, 0, tp); a.N1 = slot(i, 1, tp); a.N2 = slot(i, 2, tp); a.N3 = slot(i, 3, tp);
+        a.E2 = (const T*)coef[0].p; a.Z = (const T*)coef[1].p;
         a.x = stage == 4 ? lamb.as<C>() : stage == 3 ? lN3b.as<C>() : stage == 2 ? lN2b.as<C>() : lN1b.as<C>();
         a.lamc = stage == 4 ? (const T*)coef[5].p : nullptr;
         a.out = adj_mixed.as<C>(); a.fstride = fsz; a.ncols = g.kmax; a.tw = twy.as<C>();
@@ -1513,8 +1518,8 @@ struct Ctx final : CtxBase {
         k.Al = (const T*)coef[3].p; k.Be = (const T*)coef[4].p;
         k.dE = dtab[0].as<T>(); k.dE2 = dtab[1].as<T>(); k.dZ = dtab[2].as<T>();
         k.dA = dtab[3].as<T>(); k.dB = dtab[4].as<T>(); k.dG = dtab[5].as<T>();
-        k.s0 = slot(i, 0, tp); k.s1 = slot(i, 1, tp);
-        k.N1 = slot(i, 4, tp); k.N2 = slot(i, 5, tp); k.N3 = slot(i, 6, tp); k.N4 = slot(i, 7, tp);
+        k.s0 = slot(i, 0, tp);
+        k.N1 = slot(i, 1, tp); k.N2 = slot(i, 2, tp); k.N3 = slot(i, 3, tp); k.N4 = slot(i, 4, tp);
         k.nu_part = want_nu ? nu_part.as<T>() : nullptr;
         k.tw = twy.as<C>();
         rec(0, true); launch_adj<T>(1, d.ny, &a, stream); rec(0, false);
@@ -1555,7 +1560,7 @@ struct Ctx final : CtxBase {
       const long seg = std::min(every, nsteps);
       const long nseg = (nsteps + seg - 1) / seg;
       const size_t kb = sizeof(C) * (size_t)kstride();   // one tape field (a prefix of a state)
-      const size_t seg_bytes = kb * (size_t)(8 * seg + 1);
+      const size_t seg_bytes = kb * (size_t)(TAPE_FIELDS * seg + 1);
       // pool slots: as many whole segments as the budget affords, at least two with the pipeline
       // (which pays only when this is the one gradient in flight), never more than the run has
       long nslot = (long)std::min<size_t>((size_t)nseg, tape_budget / seg_bytes);
@@ -1565,7 +1570,7 @@ struct Ctx final : CtxBase {
       const bool pipe = nseg > P && alone;   // recompute on rstream beside the sweep
       if (pool_seg < seg || pool_slots < P) {
         pool.alloc(0);   // free first: the old and the new pool need not fit together
-        pool.alloc(sizeof(C) * (size_t)std::max<long>(P, pool_slots) * (8 * seg + 1) * kstride());
+        pool.alloc(sizeof(C) * (size_t)std::max<long>(P, pool_slots) * (TAPE_FIELDS * seg + 1) * kstride());
         pool_seg = seg;
         pool_slots = std::max(P, pool_slots);
       }

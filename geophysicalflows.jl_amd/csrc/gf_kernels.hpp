@@ -172,7 +172,12 @@ struct K3Args {
 template <typename T>
 struct AdjK1Args {
   Geo<T> g;
-  const cplx<T>* s;         // stage state (masked on read)
+  // the stage's calcN input, rebuilt from the step's input s0 and the taped N1, N2, N3 as the
+  // forward stages formed it (stage 1: s0; 2: E2 s0 + Z N1; 3: E2 s0 + Z N2;
+  // 4: E2 s1 + Z (2 N3 - N1)); masked on read
+  const cplx<T>* s0; const cplx<T>* N1; const cplx<T>* N2; const cplx<T>* N3;
+  const T* E2; const T* Z;  // ETDRK4 coefficients (real, blocked)
+  int stage;                // 4, 3, 2, 1
   const cplx<T>* x;         // co-state source; lambda_N = lamc * x (masked on read)
   const T* lamc;            // real coefficient table (blocked), nullable = 1
   cplx<T>* out;             // 5 mixed fields: u, v, zeta, pA, pB
@@ -205,9 +210,10 @@ struct AdjK3Args {
   // co-state buffers (blocked): lambda of sol' (in; out at stage 1), partial sums
   cplx<T>* lam; cplx<T>* l0; cplx<T>* l1; cplx<T>* lN1; cplx<T>* lN2; cplx<T>* lN3;
   const T* E; const T* E2; const T* Z; const T* Al; const T* Be;   // ETDRK4 coefficients (real)
-  // dJ/dnu: coefficient derivatives times dL/dnu, the step's tape, one slot per workgroup
+  // dJ/dnu: coefficient derivatives times dL/dnu, the step's tape (s1 = E2 s0 + Z N1 is rebuilt),
+  // one slot per workgroup
   const T* dE; const T* dE2; const T* dZ; const T* dA; const T* dB; const T* dG;
-  const cplx<T>* s0; const cplx<T>* s1;
+  const cplx<T>* s0;
   const cplx<T>* N1; const cplx<T>* N2; const cplx<T>* N3; const cplx<T>* N4;
   T* nu_part;               // nullable: no nu gradient
   const cplx<T>* tw;

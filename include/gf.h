@@ -319,7 +319,8 @@ int gf_device_copy(void* dst, const void* src, size_t bytes);
  * library's own ETDRK4 step, including calcN!'s in-place dealias!.
  *  - Scope: unforced TwoDNavierStokes, FP64, square grids.
  *  - The forward pass keeps a checkpoint every checkpoint_every steps. The backward sweep
- *    recomputes each segment into a device tape of 8 spectral fields per step.
+ *    recomputes each segment into a device tape of 5 spectral fields per step (the step's input
+ *    and its four calcN outputs; the adjoint rebuilds the stage inputs from them).
  *    checkpoint_every >= nsteps tapes the whole run once.
  *  - Outputs:
  *      dJ_dzeta0: nx*ny doubles, physical layout, x fastest; nullable;
@@ -334,8 +335,8 @@ int gf_adjoint_grad(gf_ctx* ctx, const double* target, int nsteps, int checkpoin
  * last P segments stay taped and the backward sweep recomputes only the earlier ones from their
  * checkpoints. P is the number of whole segments the budget affords, at least two (one when
  * other gradients are in flight in the process) and at most the number of segments. One
- * segment of k steps takes (8 k + 1) fields of kmax_pair * ny complex values, kmax_pair being
- * the surviving columns rounded up to a pair (170 at 512^2: 559 MB for 50 steps). 0, the
+ * segment of k steps takes (5 k + 1) fields of kmax_pair * ny complex values, kmax_pair being
+ * the surviving columns rounded up to a pair (170 at 512^2: 349 MB for 50 steps). 0, the
  * default: the minimum. With 288 GB of HBM a budget near the whole run removes recomputation. */
 int gf_adjoint_tape_budget(gf_ctx* ctx, size_t bytes);
 
