@@ -296,4 +296,37 @@ __device__ __forceinline__ void fft_forward(cplx<T> (&v)[C][P], cplx<T>* __restr
   stockham<T, N, P, C, 0, 1, LIGHT>(v, lds, t, tw);
 }
 
+// Direct DFT of a length n < N that is not a power of two, in the data layout of fft_forward:
+// thread t holds x[t + TPT q] (zero where t + TPT q >= n) and receives y[k] = sum_j x[j] w^(jk),
+// w = exp(-2 pi i/n), at k = t + TPT q < n (the other slots come back zero).  roots[j] = w^j,
+// j < n, in LDS.  Each thread forms its P outputs from all n inputs (P n complex products), the
+// index j k mod n advanced by k per term; the inputs are read from lds (broadcast across lanes).
+template <typename T, int N, int P>
+__device__ __forceinline__ void dft_direct(cplx<T> (&v)[1][P], cplx<T>* __restrict__ lds, int t, int n,
+                                           const cplx<T>* __restrict__ roots) {
+  constexpr int TPT = N / P;
+  __syncthreads();   // lds is free: every earlier reader of it is done
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    const int j = t + TPT * q;
+    if (j < n) lds[j] = v[0][q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    const int k = t + TPT * q;
+    cplx<T> acc = mk((T)0, (T)0);
+    if (k < n) {
+      int idx = 0;
+      for (int j = 0; j < n; ++j) {
+        acc = acc + cmul(lds[j], roots[idx]);
+        idx += k;
+        if (idx >= n) idx -= n;
+      }
+    }
+    v[0][q] = acc;
+  }
+  __syncthreads();   // lds is free again for the caller
+}
+
 }  // namespace gf

@@ -1,8 +1,9 @@
 // Size dispatch shared by the instantiation units: FN<T, n, NL>(args, stream) for the grid sizes
-// this build compiles (powers of two, 16..2048).
+// this build compiles (powers of two, 16..2048). A column length that is not a power of two runs
+// in the y-pass kernels of the next power of two (the kernels read the true length from Geo).
 #pragma once
 #define GF_SIZES(FN, T, NL, n, a, s)                     \
-  switch (n) {                                           \
+  switch (pow2_ceil(n)) {                                \
     case 16: FN<T, 16, NL>(a, s); break;                 \
     case 32: FN<T, 32, NL>(a, s); break;                 \
     case 64: FN<T, 64, NL>(a, s); break;                 \

@@ -141,6 +141,38 @@ template <int TPT> struct YMap {
   __device__ __forceinline__ YMap(int x) : c(x & 1), t((x >> 1) % TPT), slot((x >> 1) / TPT) {}
 };
 
+// Non-power-of-two column lengths ny (even, up to GEN_MAX) run in the y-pass kernels of
+// NY = pow2_ceil(ny): the rows ny..NY-1 of each thread's points stay unused, and a direct DFT
+// (dft_direct) replaces the Stockham transform. The LDS twiddle array then holds the ny roots of
+// unity (the global table, which has ny entries) instead of the per-pass tables.
+template <int NY, int P> struct YTw {
+  static constexpr int size = (NY <= GEN_MAX && NY > TwPlan<NY, P>::size()) ? NY : TwPlan<NY, P>::size();
+};
+template <typename T, int NY, int P>
+__device__ __forceinline__ void load_y_twiddles(cplx<T>* __restrict__ tw, const cplx<T>* __restrict__ twg, int ny) {
+  if constexpr (NY <= GEN_MAX) {
+    if (ny != NY) {
+      for (int i = threadIdx.x; i < ny; i += blockDim.x) tw[i] = twg[i];
+      return;
+    }
+  }
+  load_twiddles<T, NY, P>(tw, twg);
+}
+// the y-direction transform of a column of ny points held as fft_forward holds them
+template <typename T, int NY, int P, bool FR>
+__device__ __forceinline__ void y_transform(cplx<T> (&w)[1][P], cplx<T>* __restrict__ L, int t,
+                                            const cplx<T>* __restrict__ tw, int ny) {
+  if constexpr (NY <= GEN_MAX) {
+    if (ny != NY) { dft_direct<T, NY, P>(w, L, t, ny, tw); return; }
+  }
+  fft_forward<T, NY, P, 1, FR>(w, L, t, tw);
+}
+// the column length the y-pass kernels of size NY index with: ny itself where NY may hold a
+// shorter column, the constant NY otherwise
+template <int NY, typename T> __device__ __forceinline__ int ycol(const Geo<T>& g) {
+  if constexpr (NY <= GEN_MAX) return g.ny; else return NY;
+}
+
 // Body of K1 for block blk of column pairs; lds holds 2G padded N-point buffers and
 // tw the per-pass twiddle tables (already loaded).
 template <typename T, int NY, int P, int G, int NL, int MODEL, int FS>
@@ -159,12 +191,13 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
   const T kv = krwave(g, kr);
   cplx<T>* L = col_buffer<T, NY>(lds, m.slot, m.c);
   const T nrm = g.inv_norm;
+  const int ny = ycol<NY>(g);
   // masked state value of layer j at y-index l (re-read per field: an L2 hit
   // after the first field, and it keeps the register footprint small)
   auto sval = [&](int j, int l) -> cplx<T> {
     const bool lk = kept_l(g, l) || !masked;
     cplx<T> v = czero<T>();
-    if (active && lk && krk) v = at(a.sol, (uint32_t)(j * g.lstride) + boff32(kr, l, NY));
+    if (active && lk && krk && l < ny) v = at(a.sol, (uint32_t)(j * g.lstride) + boff32(kr, l, ny));
     return v;
   };
   // multi-layer problems may give each layer its own workgroups (blockIdx.y): the inversion
@@ -194,14 +227,14 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
           iK[q] = (kr == 0 && l == 0) ? (T)0 : (T)1 / K2;
         } else if constexpr (NL > 1) {
           cplx<T> acc = czero<T>();
-          if (active) {
-            const T* mm = a.sinv + boff(kr, l, NY) * (NL * NL) + jl * NL;
+          if (active && l < ny) {
+            const T* mm = a.sinv + boff(kr, l, ny) * (NL * NL) + jl * NL;
 #pragma unroll
             for (int k = 0; k < NL; ++k) acc = acc + rmul(mm[k], sval(k, l));
           }
           ps[q] = acc;
           // the bottom layer's psih, for K3's bottom drag of the same stage
-          if (FS == FS_CALCN && jl == NL - 1 && a.psi_out != nullptr && active) at(a.psi_out, boff32(kr, l, NY)) = acc;
+          if (FS == FS_CALCN && jl == NL - 1 && a.psi_out != nullptr && active && l < ny) at(a.psi_out, boff32(kr, l, ny)) = acc;
         } else {
           const cplx<T> sv = svs[q];
           if (a.eb) ps[q] = mk(-sv.re / (K2 + a.ell2inv), -sv.im / (K2 + a.ell2inv));
@@ -245,11 +278,12 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
         // normalised inverse = conj(F(conj(x)))/(nx ny); the scale is a power of two
         w[0][q] = mk(r.re * nrm, -(r.im * nrm));
       }
-      fft_forward<T, NY, P, 1, (NY >= FreshFrom<T>::k1)>(w, L, t, tw);
+      y_transform<T, NY, P, (NY >= FreshFrom<T>::k1)>(w, L, t, tw, ny);
       if (active) {
         cplx<T>* o = a.out + (long)(jl * FSET::N + f) * a.fstride;
 #pragma unroll
-        for (int q = 0; q < P; ++q) at(o, boff32(kr, t + TPT * q, NY)) = conj(w[0][q]);
+        for (int q = 0; q < P; ++q)
+          if (t + TPT * q < ny) at(o, boff32(kr, t + TPT * q, ny)) = conj(w[0][q]);
       }
     }
   }
@@ -259,8 +293,8 @@ template <typename T, int NY, int P, int G, int NL, int MODEL, int FS>
 __global__ __launch_bounds__(2 * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K1_WAVES)))
 void k1_kernel(const K1Args<T> a) {
   __shared__ cplx<T> lds[G * PairLds<T, NY>::value];
-  __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
-  load_twiddles<T, NY, P>(tw, a.tw);
+  __shared__ cplx<T> tw[YTw<NY, P>::size];
+  load_y_twiddles<T, NY, P>(tw, a.tw, a.g.ny);
   __syncthreads();
   k1_body<T, NY, P, G, NL, MODEL, FS>(a, blockIdx.x, lds, tw);
 }
@@ -840,6 +874,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
   const bool active = kr < a.ncols && kr < g.nkr;
   const T kv = krwave(g, kr);
   const int epi = EPI >= 0 ? EPI : a.epi;
+  const int ny = ycol<NY>(g);
 
   // Ring forcing (examples/twodnavierstokes_stochasticforcing.jl:78-86: N += sqrt(spectrum) xi
   // / sqrt(dt), xi complex standard normal per mode and step): this column's terms are drawn
@@ -853,11 +888,11 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
     const int64_t fstep = a.fstep_dev != nullptr ? a.fstep_dev[0] : a.fstep;
     for (int q = 0; q < P; ++q) {
       const int l = t + TPT * q;
-      if (l <= a.fky || l >= NY - a.fky) {
-        const T sp = at(a.sqrt_spec, boff32(kr, l, NY));
+      if (l < ny && (l <= a.fky || l >= ny - a.fky)) {
+        const T sp = at(a.sqrt_spec, boff32(kr, l, ny));
         if (sp != (T)0) {
           const cplx<T> xi = complex_normal<T>(a.seed, fstep, (uint32_t)(l * g.nkr + kr), (uint32_t)a.instance);
-          noise[kr * a.nbl + ring_box_row(l, NY, a.fky, a.nbl)] = mk((xi.re * sp) / a.sqrt_dt, (xi.im * sp) / a.sqrt_dt);
+          noise[kr * a.nbl + ring_box_row(l, ny, a.fky, a.nbl)] = mk((xi.re * sp) / a.sqrt_dt, (xi.im * sp) / a.sqrt_dt);
         }
       }
     }
@@ -869,15 +904,16 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       cplx<T> w[1][P];
       const cplx<T>* f = a.in + (long)jl * a.fstride;
 #pragma unroll
-      for (int q = 0; q < P; ++q) w[0][q] = active ? f[boff(kr, t + TPT * q, NY)] : czero<T>();
-      fft_forward<T, NY, P, 1, (NY >= FreshFrom<T>::k3)>(w, L, t, tw);
+      for (int q = 0; q < P; ++q) w[0][q] = (active && t + TPT * q < ny) ? f[boff(kr, t + TPT * q, ny)] : czero<T>();
+      y_transform<T, NY, P, (NY >= FreshFrom<T>::k3)>(w, L, t, tw, ny);
       if (active) {
 #pragma unroll
         for (int q = 0; q < P; ++q) {
           const int l = t + TPT * q;
+          if (l >= ny) continue;
           cplx<T> v = w[0][q];
           if (a.zero_mean && kr == 0 && l == 0) v = czero<T>();
-          a.sol[jl * g.lstride + boff(kr, l, NY)] = v;
+          a.sol[jl * g.lstride + boff(kr, l, ny)] = v;
         }
       }
     }
@@ -898,8 +934,8 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       for (int q = 0; q < P; ++q) {
         const int l = t + TPT * q;
         cplx<T> ps = czero<T>();
-        if (active && a.drag_src != nullptr && ((kept_l(g, l) && kr < g.kmax) || a.nomask)) {
-          const long idx = boff(kr, l, NY);
+        if (active && l < ny && a.drag_src != nullptr && ((kept_l(g, l) && kr < g.kmax) || a.nomask)) {
+          const long idx = boff(kr, l, ny);
           if (a.psi_in != nullptr) {
             ps = a.psi_in[idx];
           } else {
@@ -919,21 +955,22 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int y = t + TPT * q;
-      A[0][q] = active ? at(fa, boff32(kr, y, NY)) : czero<T>();
-      B[0][q] = active ? at(fb, boff32(kr, y, NY)) : czero<T>();
+      A[0][q] = (active && y < ny) ? at(fa, boff32(kr, y, ny)) : czero<T>();
+      B[0][q] = (active && y < ny) ? at(fb, boff32(kr, y, ny)) : czero<T>();
     }
-    fft_forward<T, NY, P, 1, (NY >= FreshFrom<T>::k3)>(A, L, t, tw);
-    fft_forward<T, NY, P, 1, (NY >= FreshFrom<T>::k3)>(B, L, t, tw);
+    y_transform<T, NY, P, (NY >= FreshFrom<T>::k3)>(A, L, t, tw, ny);
+    y_transform<T, NY, P, (NY >= FreshFrom<T>::k3)>(B, L, t, tw, ny);
     if (!active) continue;
     const long lo = (long)jl * g.lstride;
     bool bad = false;   // a non-finite value in the new state (final stages)
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int l = t + TPT * q;
+      if (l >= ny) continue;
       const bool kept = (kept_l(g, l) && kr < g.kmax) || ((NL > 1 || ML1) && a.nomask);
       if (!kept && !a.full) continue;
       const T lv = lwave(g, l);
-      const uint32_t idx = boff32(kr, l, NY);
+      const uint32_t idx = boff32(kr, l, ny);
       const uint32_t ix = (uint32_t)lo + idx;
       cplx<T> N;
       const cplx<T> Av = A[0][q], Bv = B[0][q];
@@ -968,14 +1005,14 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       }
       if (a.Fh != nullptr) N = N + rmul(a.fscale, at(a.Fh, ix));
       if (a.sqrt_spec != nullptr) {
-        if (kr <= a.fkx && (l <= a.fky || l >= NY - a.fky)) {
-          if (at(a.sqrt_spec, idx) != (T)0) N = N + noise[kr * a.nbl + ring_box_row(l, NY, a.fky, a.nbl)];
+        if (kr <= a.fkx && (l <= a.fky || l >= ny - a.fky)) {
+          if (at(a.sqrt_spec, idx) != (T)0) N = N + noise[kr * a.nbl + ring_box_row(l, ny, a.fky, a.nbl)];
         }
       }
       // The stepper's coefficient tables and the filter depend on kr and Krsq only, so rows l and
       // NY - l hold equal values: reading the lower row lets the mirrored point's read hit in
       // cache and halves the tables' HBM traffic.
-      const uint32_t cf = boff32(kr, l <= NY / 2 ? l : NY - l, NY);
+      const uint32_t cf = boff32(kr, l <= ny / 2 ? l : ny - l, ny);
       const uint32_t ci = a.cl_layered ? ix : cf;
       const cplx<T> svr = at(a.sol, ix);     // every lane of an active column has this point
       const cplx<T> sv = kept ? svr : czero<T>();
@@ -1094,8 +1131,8 @@ template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1, b
 __global__ __launch_bounds__(CPW * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(K3Waves<NL, CPW>::value)))
 void k3_kernel(const K3Args<T> a) {
   __shared__ cplx<T> lds[CPW == 2 ? G * PairLds<T, NY>::value : LdsN<NY>::value];
-  __shared__ cplx<T> tw[TwPlan<NY, P>::size()];
-  load_twiddles<T, NY, P>(tw, a.tw);
+  __shared__ cplx<T> tw[YTw<NY, P>::size];
+  load_y_twiddles<T, NY, P>(tw, a.tw, a.g.ny);
   __syncthreads();
   k3_body<T, NY, P, G, NL, CPW, EPI, ML1>(a, blockIdx.x, lds, tw);
 }
