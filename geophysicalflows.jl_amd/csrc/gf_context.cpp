@@ -923,7 +923,7 @@ struct Ctx final : CtxBase {
       for (auto& ge : gexec) if (ge) { HIPCHECK(hipStreamSynchronize(stream)); (void)dw::exec_destroy(ge); ge = nullptr; }
   }
   void get_schedule(int* p) const override {
-    for (int k = 0; k < 3; ++k) p[k] = sched[k] ? sched[k] : (nl == 1 ? sched_default(sched_len(k)) : (d.ny >= 32 && k != 1 ? 4 : sched_default(sched_len(k))));
+    for (int k = 0; k < 3; ++k) p[k] = sched[k] ? sched[k] : (nl == 1 ? sched_default(sched_len(k)) : (d.ny >= 32 && k != 1 ? (d.ny >= 4096 ? 8 : 4) : sched_default(sched_len(k))));
   }
   // Times whole steps under every offered schedule -- P in {4, 8, 16} for each of K1, K2, K3,
   // the K1 and K3 splits on or off and the K2 split 0, 1 or 2, up to 324 combinations -- in two
@@ -1553,6 +1553,7 @@ struct Ctx final : CtxBase {
       if (d.model != GF_MODEL_TWODNS || d.stepper != GF_ETDRK4 || d.forcing != GF_FORCING_NONE)
         throw GfError(GF_EUNSUPPORTED, "the adjoint is provided for unforced TwoDNavierStokes ETDRK4 problems");
       if (d.nx != d.ny) throw GfError(GF_EUNSUPPORTED, "the adjoint needs a square grid");
+      if (d.nx > 2048) throw GfError(GF_EUNSUPPORTED, "the adjoint kernels are built for grids up to 2048^2");
       if (nsteps < 1 || every < 1) throw GfError(GF_EINVAL, "nsteps and checkpoint_every must be >= 1");
       adjoint_setup();
       struct Active {
@@ -1864,7 +1865,7 @@ int validate(const gf_problem_desc* d, std::string& msg) {
   if (d->model < 0 || d->model > 2) { msg = "model must be GF_MODEL_TWODNS, GF_MODEL_SLQG or GF_MODEL_MLQG"; return GF_EINVAL; }
   if (d->dtype != GF_F64 && d->dtype != GF_F32) { msg = "dtype must be GF_F64 or GF_F32"; return GF_EINVAL; }
   if (!size_supported(d->nx) || !(size_supported(d->ny) || ny_generic_supported(d->ny))) {
-    msg = "nx must be a power of two between 16 and 2048, ny a power of two between 16 and 2048 or even between 18 and " +
+    msg = "nx must be a power of two between 16 and 4096, ny a power of two between 16 and 4096 or even between 18 and " +
           std::to_string(GEN_MAX) + " (got " + std::to_string(d->nx) + "x" + std::to_string(d->ny) + ")";
     return GF_EUNSUPPORTED;
   }

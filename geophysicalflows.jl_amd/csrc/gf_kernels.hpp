@@ -299,9 +299,11 @@ __host__ __device__ constexpr int pow2_ceil(int n) { int m = 1; while (m < n) m 
 __host__ __device__ constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 inline bool ny_generic_supported(int n) { return n % 2 == 0 && n >= 18 && n <= GEN_MAX && !is_pow2(n); }
 // Radix schedules offered to the autotuner for an FFT of length n (the host side of PCand).
-inline bool sched_candidate(int n, int p) { return p == 4 ? n >= 16 : (p == 8 ? n >= 32 : (p == 16 ? n >= 1024 : false)); }
-// The built-in choice (PY in the kernels): P = 4 below 32 points, else 8.
-inline int sched_default(int n) { return n < 32 ? 4 : 8; }
+inline bool sched_candidate(int n, int p) {
+  return p == 4 ? (n >= 16 && n <= 2048) : (p == 8 ? n >= 32 : (p == 16 ? n >= 1024 : false));
+}
+// The built-in choice (PY in the kernels): P = 4 below 32 points, 16 from 4096, else 8.
+inline int sched_default(int n) { return n < 32 ? 4 : (n >= 4096 ? 16 : 8); }
 // Row of spectral row l in a ring-forcing box of nbl = min(ny, 2 fky + 1) rows holding
 // l <= fky and l >= ny - fky.
 __host__ __device__ inline int ring_box_row(int l, int ny, int fky, int nbl) { return l <= fky ? l : l - ny + nbl; }

@@ -141,41 +141,42 @@ template <int TPT> struct YMap {
   __device__ __forceinline__ YMap(int x) : c(x & 1), t((x >> 1) % TPT), slot((x >> 1) / TPT) {}
 };
 
-// Non-power-of-two column lengths ny (even, up to GEN_MAX) run in the y-pass kernels of
-// NY = pow2_ceil(ny): the rows ny..NY-1 of each thread's points stay unused, and a direct DFT
-// (dft_direct) replaces the Stockham transform. The LDS twiddle array then holds the ny roots of
-// unity (the global table, which has ny entries) instead of the per-pass tables.
-template <int NY, int P> struct YTw {
-  static constexpr int size = (NY <= GEN_MAX && NY > TwPlan<NY, P>::size()) ? NY : TwPlan<NY, P>::size();
+// Non-power-of-two column lengths ny (even, up to GEN_MAX) run in separate instantiations
+// (GEN = true) of the y-pass kernels of NY = pow2_ceil(ny): they index with the runtime ny, the
+// rows ny..NY-1 of each thread's points stay unused, and a direct DFT (dft_direct) replaces the
+// Stockham transform, with the ny roots of unity (the global table, which has ny entries) in the
+// LDS twiddle array. The power-of-two kernels keep the constant NY (a runtime ny in them cost
+// 6-7% per step at 64^2-256^2).
+template <int NY, int P, bool GEN> struct YTw {
+  static constexpr int size = (GEN && NY > TwPlan<NY, P>::size()) ? NY : TwPlan<NY, P>::size();
 };
-template <typename T, int NY, int P>
+template <typename T, int NY, int P, bool GEN>
 __device__ __forceinline__ void load_y_twiddles(cplx<T>* __restrict__ tw, const cplx<T>* __restrict__ twg, int ny) {
-  if constexpr (NY <= GEN_MAX) {
-    if (ny != NY) {
-      for (int i = threadIdx.x; i < ny; i += blockDim.x) tw[i] = twg[i];
-      return;
-    }
+  if constexpr (GEN) {
+    for (int i = threadIdx.x; i < ny; i += blockDim.x) tw[i] = twg[i];
+  } else {
+    load_twiddles<T, NY, P>(tw, twg);
   }
-  load_twiddles<T, NY, P>(tw, twg);
 }
 // the y-direction transform of a column of ny points held as fft_forward holds them
-template <typename T, int NY, int P, bool FR>
+template <typename T, int NY, int P, bool FR, bool GEN>
 __device__ __forceinline__ void y_transform(cplx<T> (&w)[1][P], cplx<T>* __restrict__ L, int t,
                                             const cplx<T>* __restrict__ tw, int ny) {
-  if constexpr (NY <= GEN_MAX) {
-    if (ny != NY) { dft_direct<T, NY, P>(w, L, t, ny, tw); return; }
-  }
-  fft_forward<T, NY, P, 1, FR>(w, L, t, tw);
+  if constexpr (GEN) dft_direct<T, NY, P>(w, L, t, ny, tw);
+  else fft_forward<T, NY, P, 1, FR>(w, L, t, tw);
 }
-// the column length the y-pass kernels of size NY index with: ny itself where NY may hold a
-// shorter column, the constant NY otherwise
-template <int NY, typename T> __device__ __forceinline__ int ycol(const Geo<T>& g) {
-  if constexpr (NY <= GEN_MAX) return g.ny; else return NY;
+// the column length the y-pass kernels index with
+template <int NY, bool GEN, typename T> __device__ __forceinline__ int ycol(const Geo<T>& g) {
+  if constexpr (GEN) return g.ny; else return NY;
+}
+// row l lies in the column (always, where the column fills the kernel's NY points)
+template <bool GEN> __device__ __forceinline__ bool in_col(int l, int ny) {
+  if constexpr (GEN) return l < ny; else return true;
 }
 
 // Body of K1 for block blk of column pairs; lds holds 2G padded N-point buffers and
 // tw the per-pass twiddle tables (already loaded).
-template <typename T, int NY, int P, int G, int NL, int MODEL, int FS>
+template <typename T, int NY, int P, int G, int NL, int MODEL, int FS, bool GEN = false>
 __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __restrict__ lds,
                                         const cplx<T>* __restrict__ tw) {
   constexpr int TPT = NY / P;
@@ -191,13 +192,13 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
   const T kv = krwave(g, kr);
   cplx<T>* L = col_buffer<T, NY>(lds, m.slot, m.c);
   const T nrm = g.inv_norm;
-  const int ny = ycol<NY>(g);
+  const int ny = ycol<NY, GEN>(g);
   // masked state value of layer j at y-index l (re-read per field: an L2 hit
   // after the first field, and it keeps the register footprint small)
   auto sval = [&](int j, int l) -> cplx<T> {
     const bool lk = kept_l(g, l) || !masked;
     cplx<T> v = czero<T>();
-    if (active && lk && krk && l < ny) v = at(a.sol, (uint32_t)(j * g.lstride) + boff32(kr, l, ny));
+    if (active && lk && krk && in_col<GEN>(l, ny)) v = at(a.sol, (uint32_t)(j * g.lstride) + boff32(kr, l, ny));
     return v;
   };
   // multi-layer problems may give each layer its own workgroups (blockIdx.y): the inversion
@@ -227,14 +228,14 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
           iK[q] = (kr == 0 && l == 0) ? (T)0 : (T)1 / K2;
         } else if constexpr (NL > 1) {
           cplx<T> acc = czero<T>();
-          if (active && l < ny) {
+          if (active && in_col<GEN>(l, ny)) {
             const T* mm = a.sinv + boff(kr, l, ny) * (NL * NL) + jl * NL;
 #pragma unroll
             for (int k = 0; k < NL; ++k) acc = acc + rmul(mm[k], sval(k, l));
           }
           ps[q] = acc;
           // the bottom layer's psih, for K3's bottom drag of the same stage
-          if (FS == FS_CALCN && jl == NL - 1 && a.psi_out != nullptr && active && l < ny) at(a.psi_out, boff32(kr, l, ny)) = acc;
+          if (FS == FS_CALCN && jl == NL - 1 && a.psi_out != nullptr && active && in_col<GEN>(l, ny)) at(a.psi_out, boff32(kr, l, ny)) = acc;
         } else {
           const cplx<T> sv = svs[q];
           if (a.eb) ps[q] = mk(-sv.re / (K2 + a.ell2inv), -sv.im / (K2 + a.ell2inv));
@@ -278,25 +279,25 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
         // normalised inverse = conj(F(conj(x)))/(nx ny); the scale is a power of two
         w[0][q] = mk(r.re * nrm, -(r.im * nrm));
       }
-      y_transform<T, NY, P, (NY >= FreshFrom<T>::k1)>(w, L, t, tw, ny);
+      y_transform<T, NY, P, (NY >= FreshFrom<T>::k1), GEN>(w, L, t, tw, ny);
       if (active) {
         cplx<T>* o = a.out + (long)(jl * FSET::N + f) * a.fstride;
 #pragma unroll
         for (int q = 0; q < P; ++q)
-          if (t + TPT * q < ny) at(o, boff32(kr, t + TPT * q, ny)) = conj(w[0][q]);
+          if (in_col<GEN>(t + TPT * q, ny)) at(o, boff32(kr, t + TPT * q, ny)) = conj(w[0][q]);
       }
     }
   }
 }
 
-template <typename T, int NY, int P, int G, int NL, int MODEL, int FS>
+template <typename T, int NY, int P, int G, int NL, int MODEL, int FS, bool GEN = false>
 __global__ __launch_bounds__(2 * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(GF_K1_WAVES)))
 void k1_kernel(const K1Args<T> a) {
   __shared__ cplx<T> lds[G * PairLds<T, NY>::value];
-  __shared__ cplx<T> tw[YTw<NY, P>::size];
-  load_y_twiddles<T, NY, P>(tw, a.tw, a.g.ny);
+  __shared__ cplx<T> tw[YTw<NY, P, GEN>::size];
+  load_y_twiddles<T, NY, P, GEN>(tw, a.tw, a.g.ny);
   __syncthreads();
-  k1_body<T, NY, P, G, NL, MODEL, FS>(a, blockIdx.x, lds, tw);
+  k1_body<T, NY, P, G, NL, MODEL, FS, GEN>(a, blockIdx.x, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -853,7 +854,7 @@ __device__ __forceinline__ int single_column(int b) { return BW * (8 * (b / (8 *
 // compile time (it must equal a.epi); EPI = -1 switches on a.epi.  ML1: a one-layer MultiLayerQG
 // problem (SingleLayerParams), whose bottom drag mu Krsq psih uses psih = -invKrsq qh of the calcN
 // input (src/multilayerqg.jl:658, 547-551) and whose linearised equations step every mode.
-template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1, bool ML1 = false>
+template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1, bool ML1 = false, bool GEN = false>
 __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __restrict__ lds,
                                         const cplx<T>* __restrict__ tw) {
   constexpr int TPT = NY / P;
@@ -874,7 +875,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
   const bool active = kr < a.ncols && kr < g.nkr;
   const T kv = krwave(g, kr);
   const int epi = EPI >= 0 ? EPI : a.epi;
-  const int ny = ycol<NY>(g);
+  const int ny = ycol<NY, GEN>(g);
 
   // Ring forcing (examples/twodnavierstokes_stochasticforcing.jl:78-86: N += sqrt(spectrum) xi
   // / sqrt(dt), xi complex standard normal per mode and step): this column's terms are drawn
@@ -888,7 +889,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
     const int64_t fstep = a.fstep_dev != nullptr ? a.fstep_dev[0] : a.fstep;
     for (int q = 0; q < P; ++q) {
       const int l = t + TPT * q;
-      if (l < ny && (l <= a.fky || l >= ny - a.fky)) {
+      if (in_col<GEN>(l, ny) && (l <= a.fky || l >= ny - a.fky)) {
         const T sp = at(a.sqrt_spec, boff32(kr, l, ny));
         if (sp != (T)0) {
           const cplx<T> xi = complex_normal<T>(a.seed, fstep, (uint32_t)(l * g.nkr + kr), (uint32_t)a.instance);
@@ -904,13 +905,13 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       cplx<T> w[1][P];
       const cplx<T>* f = a.in + (long)jl * a.fstride;
 #pragma unroll
-      for (int q = 0; q < P; ++q) w[0][q] = (active && t + TPT * q < ny) ? f[boff(kr, t + TPT * q, ny)] : czero<T>();
-      y_transform<T, NY, P, (NY >= FreshFrom<T>::k3)>(w, L, t, tw, ny);
+      for (int q = 0; q < P; ++q) w[0][q] = (active && in_col<GEN>(t + TPT * q, ny)) ? f[boff(kr, t + TPT * q, ny)] : czero<T>();
+      y_transform<T, NY, P, (NY >= FreshFrom<T>::k3), GEN>(w, L, t, tw, ny);
       if (active) {
 #pragma unroll
         for (int q = 0; q < P; ++q) {
           const int l = t + TPT * q;
-          if (l >= ny) continue;
+          if (!in_col<GEN>(l, ny)) continue;
           cplx<T> v = w[0][q];
           if (a.zero_mean && kr == 0 && l == 0) v = czero<T>();
           a.sol[jl * g.lstride + boff(kr, l, ny)] = v;
@@ -934,7 +935,7 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
       for (int q = 0; q < P; ++q) {
         const int l = t + TPT * q;
         cplx<T> ps = czero<T>();
-        if (active && l < ny && a.drag_src != nullptr && ((kept_l(g, l) && kr < g.kmax) || a.nomask)) {
+        if (active && in_col<GEN>(l, ny) && a.drag_src != nullptr && ((kept_l(g, l) && kr < g.kmax) || a.nomask)) {
           const long idx = boff(kr, l, ny);
           if (a.psi_in != nullptr) {
             ps = a.psi_in[idx];
@@ -955,18 +956,18 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int y = t + TPT * q;
-      A[0][q] = (active && y < ny) ? at(fa, boff32(kr, y, ny)) : czero<T>();
-      B[0][q] = (active && y < ny) ? at(fb, boff32(kr, y, ny)) : czero<T>();
+      A[0][q] = (active && in_col<GEN>(y, ny)) ? at(fa, boff32(kr, y, ny)) : czero<T>();
+      B[0][q] = (active && in_col<GEN>(y, ny)) ? at(fb, boff32(kr, y, ny)) : czero<T>();
     }
-    y_transform<T, NY, P, (NY >= FreshFrom<T>::k3)>(A, L, t, tw, ny);
-    y_transform<T, NY, P, (NY >= FreshFrom<T>::k3)>(B, L, t, tw, ny);
+    y_transform<T, NY, P, (NY >= FreshFrom<T>::k3), GEN>(A, L, t, tw, ny);
+    y_transform<T, NY, P, (NY >= FreshFrom<T>::k3), GEN>(B, L, t, tw, ny);
     if (!active) continue;
     const long lo = (long)jl * g.lstride;
     bool bad = false;   // a non-finite value in the new state (final stages)
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int l = t + TPT * q;
-      if (l >= ny) continue;
+      if (!in_col<GEN>(l, ny)) continue;
       const bool kept = (kept_l(g, l) && kr < g.kmax) || ((NL > 1 || ML1) && a.nomask);
       if (!kept && !a.full) continue;
       const T lv = lwave(g, l);
@@ -1125,16 +1126,17 @@ __device__ __forceinline__ void k3_body(const K3Args<T>& a, int blk, cplx<T>* __
 // Occupancy target of K3.  The multi-layer kernels need about 170 VGPRs (the bottom drag's psi
 // per point, layered coefficient tables): held to 128 for 4 waves they spilled 30-50.  At 2
 // waves they do not spill, and a CU still holds more of their workgroups than the layered grids
-// supply (513 single-column workgroups at 512^2 x 3).
-template <int NL, int CPW> struct K3Waves { static constexpr int value = NL > 1 ? 2 : GF_K3_WAVES; };
-template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1, bool ML1 = false>
-__global__ __launch_bounds__(CPW * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(K3Waves<NL, CPW>::value)))
+// supply (513 single-column workgroups at 512^2 x 3).  At 4096 points (16 per thread) the
+// epilogue's registers double too, and one 147 KB column-pair workgroup fills a CU's LDS anyway.
+template <int N, int NL, int CPW> struct K3Waves { static constexpr int value = (NL > 1 || N >= 4096) ? 2 : GF_K3_WAVES; };
+template <typename T, int NY, int P, int G, int NL, int CPW = 2, int EPI = -1, bool ML1 = false, bool GEN = false>
+__global__ __launch_bounds__(CPW * G * (NY / P)) __attribute__((amdgpu_waves_per_eu(K3Waves<NY, NL, CPW>::value)))
 void k3_kernel(const K3Args<T> a) {
   __shared__ cplx<T> lds[CPW == 2 ? G * PairLds<T, NY>::value : LdsN<NY>::value];
-  __shared__ cplx<T> tw[YTw<NY, P>::size];
-  load_y_twiddles<T, NY, P>(tw, a.tw, a.g.ny);
+  __shared__ cplx<T> tw[YTw<NY, P, GEN>::size];
+  load_y_twiddles<T, NY, P, GEN>(tw, a.tw, a.g.ny);
   __syncthreads();
-  k3_body<T, NY, P, G, NL, CPW, EPI, ML1>(a, blockIdx.x, lds, tw);
+  k3_body<T, NY, P, G, NL, CPW, EPI, ML1, GEN>(a, blockIdx.x, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -1164,10 +1166,16 @@ template <int N, int P> struct GY { static constexpr int G = 2 * (N / P) >= 64 ?
 // (2048 = 16 16 8) at twice the registers, which Float32 can afford.  Every other path uses the
 // built-in choice.  Keep in step with sched_candidate() on the host.
 template <int N> struct PCand {
-  static constexpr bool has(int p) { return p == 4 ? N >= 16 : (p == 8 ? N >= 32 : (p == 16 ? N >= 1024 : false)); }
+  // (4 points per thread stops at 2048: a K1/K3 column pair of 4096 points would need 2048 threads)
+  static constexpr bool has(int p) {
+    return p == 4 ? (N >= 16 && N <= 2048) : (p == 8 ? N >= 32 : (p == 16 ? N >= 1024 : false));
+  }
 };
+// Points per thread of the multi-layer y passes: 4 (more, shorter transforms) up to 2048, 8 at
+// 4096 (a column pair of 4096 points at 4 per thread would need 2048 threads)
+template <int N> struct PML { static constexpr int P = N >= 4096 ? 8 : (N >= 32 ? 4 : PY<N>::P); };
 
-template <typename T, int N, int NL, int P>
+template <typename T, int N, int NL, int P, bool GEN = false>
 void k1_np(const K1Args<T>& a, hipStream_t s) {
   constexpr int G = GY<N, P>::G;
   const int nblk = pair_blocks(a.ncols, G);
@@ -1175,29 +1183,35 @@ void k1_np(const K1Args<T>& a, hipStream_t s) {
   const int gy = (NL > 1 && a.lsplit) ? NL : ((NL == 1 && a.fsplit && a.fieldset == FS_CALCN) ? 3 : 1);
   const dim3 grid(nblk, gy), blk(2 * G * (N / P));
   if constexpr (NL > 1) {
-    if (a.fieldset == FS_CALCN) GF_LAUNCH((k1_kernel<T, N, P, G, NL, M_MLQG, FS_CALCN>), grid, blk, 0, s, a);
-    else if (a.fieldset == FS_OUT) GF_LAUNCH((k1_kernel<T, N, P, G, NL, M_MLQG, FS_OUT>), grid, blk, 0, s, a);
-    else GF_LAUNCH((k1_kernel<T, N, P, G, NL, M_MLQG, FS_RAW>), grid, blk, 0, s, a);
+    if (a.fieldset == FS_CALCN) GF_LAUNCH((k1_kernel<T, N, P, G, NL, M_MLQG, FS_CALCN, GEN>), grid, blk, 0, s, a);
+    else if (a.fieldset == FS_OUT) GF_LAUNCH((k1_kernel<T, N, P, G, NL, M_MLQG, FS_OUT, GEN>), grid, blk, 0, s, a);
+    else GF_LAUNCH((k1_kernel<T, N, P, G, NL, M_MLQG, FS_RAW, GEN>), grid, blk, 0, s, a);
   } else if constexpr (P != PY<N>::P) {   // a tuned schedule: the calcN field set only
-    if (a.model == M_TWODNS) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_CALCN>), grid, blk, 0, s, a);
-    else GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_CALCN>), grid, blk, 0, s, a);
+    if (a.model == M_TWODNS) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_CALCN, GEN>), grid, blk, 0, s, a);
+    else GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_CALCN, GEN>), grid, blk, 0, s, a);
   } else {
     if (a.model == M_TWODNS) {
-      if (a.fieldset == FS_CALCN) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_CALCN>), grid, blk, 0, s, a);
-      else if (a.fieldset == FS_OUT) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_OUT>), grid, blk, 0, s, a);
-      else GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_RAW>), grid, blk, 0, s, a);
+      if (a.fieldset == FS_CALCN) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_CALCN, GEN>), grid, blk, 0, s, a);
+      else if (a.fieldset == FS_OUT) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_OUT, GEN>), grid, blk, 0, s, a);
+      else GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_TWODNS, FS_RAW, GEN>), grid, blk, 0, s, a);
     } else {
-      if (a.fieldset == FS_CALCN) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_CALCN>), grid, blk, 0, s, a);
-      else if (a.fieldset == FS_OUT) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_OUT>), grid, blk, 0, s, a);
-      else GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_RAW>), grid, blk, 0, s, a);
+      if (a.fieldset == FS_CALCN) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_CALCN, GEN>), grid, blk, 0, s, a);
+      else if (a.fieldset == FS_OUT) GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_OUT, GEN>), grid, blk, 0, s, a);
+      else GF_LAUNCH((k1_kernel<T, N, P, G, 1, M_SLQG, FS_RAW, GEN>), grid, blk, 0, s, a);
     }
   }
 }
 template <typename T, int N, int NL>
 void k1_n(const K1Args<T>& a, hipStream_t s) {
   if constexpr (NL > 1) {
-    k1_np<T, N, NL, (N >= 32 ? 4 : PY<N>::P)>(a, s);
+    if constexpr (N <= GEN_MAX) {
+      if (a.g.ny != N) { k1_np<T, N, NL, PML<N>::P, true>(a, s); return; }   // a column length below N
+    }
+    k1_np<T, N, NL, PML<N>::P>(a, s);
   } else {
+    if constexpr (N <= GEN_MAX) {
+      if (a.g.ny != N) { k1_np<T, N, 1, PY<N>::P, true>(a, s); return; }
+    }
     const bool tuned = a.fieldset == FS_CALCN && a.p != 0 && a.p != PY<N>::P && PCand<N>::has(a.p);
     if (tuned && a.p == 4) { if constexpr (PCand<N>::has(4) && PY<N>::P != 4) { k1_np<T, N, 1, 4>(a, s); return; } }
     if (tuned && a.p == 8) { if constexpr (PCand<N>::has(8) && PY<N>::P != 8) { k1_np<T, N, 1, 8>(a, s); return; } }
@@ -1246,20 +1260,20 @@ void k2_n(const K2Args<T>& a, hipStream_t s) {
   }
   k2_np<T, N, NL, PY<N>::P>(a, s);
 }
-template <typename T, int N, int NL, int P>
+template <typename T, int N, int NL, int P, bool GEN = false>
 void k3_np(const K3Args<T>& a, hipStream_t s) {
   constexpr int G = GY<N, P>::G;
   if constexpr (NL > 1 && N >= 512 && k3_single_column(N, P)) {
     // multi-layer problems (fixed schedule): one column per workgroup from 512^2, where it
     // measured 3.7% faster over a 3-layer 512^2 ETDRK4 step (0.1915 -> 0.1845 ms)
     const int nblk = column_blocks(a.ncols);
-    if (a.ncols > 0) GF_LAUNCH((k3_kernel<T, N, P, 1, NL, 1>), dim3(nblk, a.lsplit ? NL : 1), dim3(N / P), 0, s, a);
+    if (a.ncols > 0) GF_LAUNCH((k3_kernel<T, N, P, 1, NL, 1, -1, false, GEN>), dim3(nblk, a.lsplit ? NL : 1), dim3(N / P), 0, s, a);
     return;
   }
   if constexpr (NL == 1 && P == PY<N>::P) {
     if (a.model == M_MLQG) {   // one-layer MultiLayerQG: fixed schedule, column pairs
       const int nblk = pair_blocks(a.ncols, G);
-      if (nblk > 0) GF_LAUNCH((k3_kernel<T, N, P, G, 1, 2, -1, true>), dim3(nblk), dim3(2 * G * (N / P)), 0, s, a);
+      if (nblk > 0) GF_LAUNCH((k3_kernel<T, N, P, G, 1, 2, -1, true, GEN>), dim3(nblk), dim3(2 * G * (N / P)), 0, s, a);
       return;
     }
   }
@@ -1276,26 +1290,32 @@ void k3_np(const K3Args<T>& a, hipStream_t s) {
         // 128 VGPRs without spilling (120-128); the kernel that switches on the stage needs
         // 137 and spills 8.
         switch (a.epi) {
-          case E_ETD1: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD1>), grid, blk, 0, s, a); return;
-          case E_ETD2: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD2>), grid, blk, 0, s, a); return;
-          case E_ETD3: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD3>), grid, blk, 0, s, a); return;
-          case E_ETD4: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD4>), grid, blk, 0, s, a); return;
+          case E_ETD1: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD1, false, GEN>), grid, blk, 0, s, a); return;
+          case E_ETD2: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD2, false, GEN>), grid, blk, 0, s, a); return;
+          case E_ETD3: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD3, false, GEN>), grid, blk, 0, s, a); return;
+          case E_ETD4: GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, E_ETD4, false, GEN>), grid, blk, 0, s, a); return;
           default: break;
         }
       }
-      GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1>), grid, blk, 0, s, a);
+      GF_LAUNCH((k3_kernel<T, N, P, 1, 1, 1, -1, false, GEN>), grid, blk, 0, s, a);
       return;
     }
   }
   const int nblk = pair_blocks(a.ncols, G);
   if (nblk > 0)
-    GF_LAUNCH((k3_kernel<T, N, P, G, NL>), dim3(nblk, (NL > 1 && a.lsplit) ? NL : 1), dim3(2 * G * (N / P)), 0, s, a);
+    GF_LAUNCH((k3_kernel<T, N, P, G, NL, 2, -1, false, GEN>), dim3(nblk, (NL > 1 && a.lsplit) ? NL : 1), dim3(2 * G * (N / P)), 0, s, a);
 }
 template <typename T, int N, int NL>
 void k3_n(const K3Args<T>& a, hipStream_t s) {
   if constexpr (NL > 1) {
-    k3_np<T, N, NL, (N >= 32 ? 4 : PY<N>::P)>(a, s);
+    if constexpr (N <= GEN_MAX) {
+      if (a.g.ny != N) { k3_np<T, N, NL, PML<N>::P, true>(a, s); return; }   // a column length below N
+    }
+    k3_np<T, N, NL, PML<N>::P>(a, s);
   } else {
+    if constexpr (N <= GEN_MAX) {
+      if (a.g.ny != N) { k3_np<T, N, 1, PY<N>::P, true>(a, s); return; }
+    }
     const bool tuned = a.p != 0 && a.p != PY<N>::P && PCand<N>::has(a.p);
     if (tuned && a.p == 4) { if constexpr (PCand<N>::has(4) && PY<N>::P != 4) { k3_np<T, N, 1, 4>(a, s); return; } }
     if (tuned && a.p == 8) { if constexpr (PCand<N>::has(8) && PY<N>::P != 8) { k3_np<T, N, 1, 8>(a, s); return; } }

@@ -58,7 +58,7 @@ void launch_counter(int64_t* c, int64_t v, int add, hipStream_t s) {
 }
 
 bool size_supported(int n) {
-  return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048;
+  return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096;
 }
 
 #define INST(T)                                                                                  \

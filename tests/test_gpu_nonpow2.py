@@ -197,6 +197,8 @@ def test_unsupported_column_lengths_and_schedules(gf):
         gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, ny=258, stepper="RK4")
     with pytest.raises(gf._lib.GFError, match="nx must be a power of two"):
         gf.TwoDNavierStokes.Problem(gf.GPU(), nx=66, ny=64, stepper="RK4")
+    with pytest.raises(gf._lib.GFError, match="between 16 and 4096"):
+        gf.TwoDNavierStokes.Problem(gf.GPU(), nx=8192, stepper="RK4")
     p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, ny=66, stepper="RK4")
     assert gf.schedule(p) == (8, 8, 8)        # the 128-point y kernels' built-in P
     with pytest.raises(gf._lib.GFError, match="not offered"):

@@ -406,6 +406,35 @@ def test_full_size_2048_fp32_shipped_schedule_matches_fp64_oracle(gf):
     assert rel(p.sol, r.sol) < 2e-5
 
 
+def test_4096_calcN_and_step_match_oracle(gf):
+    """Beyond the benchmark size: 4096^2, under the shipped tuned schedules (FP64 (8, 8, 8), FP32
+    (8, 16, 8) with the K3 column split) and the built-in one (16 points per thread in every pass;
+    the y passes' column-pair workgroups then take 147 KB of LDS in FP64).  FP64 calcN at 1e-13
+    and one RK4 step at 1e-12 against the oracle, FP32 calcN at 2e-6."""
+    n = 4096
+    kw = dict(nx=n, Lx=TWO_PI, dt=0.001, stepper="RK4")
+    rng = np.random.default_rng(40)
+    s = 0.01 * (rng.standard_normal((n, n // 2 + 1)) + 1j * rng.standard_normal((n, n // 2 + 1)))
+    s[0, 0] = 0
+    r = o2d.Problem(nu=1e-25, nnu=4, mu=0.1, nmu=0, **kw)
+    Nr, sr = np.zeros_like(s), s.copy()
+    o2d.calcN(Nr, sr, 0.0, r.clock, r.vars, r.params, r.grid)
+    for dtype, tol, shipped in ((np.float32, 2e-6, (8, 16, 8)), (np.float64, 1e-13, (8, 8, 8))):
+        p = gf.TwoDNavierStokes.Problem(gf.GPU(), ν=1e-25, nν=4, μ=0.1, nμ=0, T=dtype, **kw)
+        assert gf.schedule(p) == shipped
+        N, _ = p.calcN(s)
+        assert rel(N, Nr) < tol, dtype
+        gf.set_schedule(p, 16, 16, 16)
+        gf.set_splits(p, 0, 0, 0)
+        N, _ = p.calcN(s)
+        assert rel(N, Nr) < tol, dtype
+    p.sol = s
+    r.sol[...] = s
+    gf.stepforward(p, 1)
+    off.stepforward(r, 1)
+    assert rel(p.sol, r.sol) < 1e-12
+
+
 @pytest.mark.parametrize("n", [16, 64])
 @pytest.mark.parametrize("f", [1 / 3, 1 / 2])
 def test_device_calcN_retained_modes_are_alias_free(gf, n, f):
