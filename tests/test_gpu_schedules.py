@@ -310,3 +310,14 @@ def test_time_kernel_leaves_the_state_and_rejects_bad_classes(gf):
     assert np.array_equal(p.sol, q.sol)
     assert lib.gf_time_kernel(p._ctx, 3, 5, ctypes.byref(ms)) == gf._lib.GF_EINVAL
     assert lib.gf_time_kernel(p._ctx, 1, 0, ctypes.byref(ms)) == gf._lib.GF_EINVAL
+    # a three-layer MultiLayerQG problem (layered kernels, bottom drag) and an ensemble
+    m = gf.MultiLayerQG.Problem(3, gf.GPU(), nx=128, f0=1.0, H=[0.2, 0.3, 0.5], b=[-1.0, -1.2, -1.3], μ=0.01,
+                                U=[0.3, 0.1, 0.0], dt=0.005, stepper="ETDRK4")
+    m.sol = 1e-3 * (np.random.default_rng(5).standard_normal(m._sshape) + 0j)
+    gf.stepforward(m, 2)
+    m0 = m.sol
+    for k in range(3):
+        assert lib.gf_time_kernel(m._ctx, k, 3, ctypes.byref(ms)) == 0 and ms.value > 0
+    assert np.array_equal(m.sol, m0)
+    e = gf.Ensemble([(gf.TwoDNavierStokes.Problem, dict(nx=64, dt=0.01, stepper="RK4")) for _ in range(2)])
+    assert lib.gf_time_kernel(e._ctx, 1, 3, ctypes.byref(ms)) == gf._lib.GF_EUNSUPPORTED
