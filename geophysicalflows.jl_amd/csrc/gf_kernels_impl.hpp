@@ -193,8 +193,7 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
   cplx<T>* L = col_buffer<T, NY>(lds, m.slot, m.c);
   const T nrm = g.inv_norm;
   const int ny = ycol<NY, GEN>(g);
-  // masked state value of layer j at y-index l (re-read per field: an L2 hit
-  // after the first field, and it keeps the register footprint small)
+  // masked state value of layer j at y-index l
   auto sval = [&](int j, int l) -> cplx<T> {
     const bool lk = kept_l(g, l) || !masked;
     cplx<T> v = czero<T>();
@@ -252,7 +251,14 @@ __device__ __forceinline__ void k1_body(const K1Args<T>& a, int blk, cplx<T>* __
       }
       const int code = FSET::code(f);
       cplx<T> w[1][P];
-      if (code == F_RAW || code == F_Q || MODEL == M_TWODNS) {
+      // TwoDNS reads the state again for each field (an L2 hit, mostly, and it keeps the registers
+      // few), except at 16 points per thread: there LDS already limits the kernel to 2 waves per
+      // SIMD, so the state stays in registers across the three fields for free. At 2048^2 FP64
+      // that took K1 from 22.9 to 21.2 us and the step from 0.3434 to 0.3308 ms; at 8 points
+      // per thread (4096^2) holding it cost a wave per SIMD (K1 89.8 -> 118 us).
+      constexpr bool HOLD = MODEL == M_TWODNS && P >= 16;
+      const bool fsp = NL == 1 && FS == FS_CALCN && a.fsplit;
+      if (HOLD ? (f == 0 || fsp) : (code == F_RAW || code == F_Q || MODEL == M_TWODNS)) {
 #pragma unroll
         for (int q = 0; q < P; ++q) svs[q] = sval(jl, t + TPT * q);
       }
