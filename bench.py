@@ -42,11 +42,11 @@ def symbols(sched, splits):
     cpw = 1 if splits[2] and 2048 // p3 >= 64 else 2
     if cpw == 1 and p3 == 8:
         # one kernel per ETDRK4 stage (the seventh argument is the stage's epilogue code)
-        k3 = [f"k3_kernel<double, 2048, 8, 1, 1, 1, {e}>" for e in (2, 3, 4, 5)]
+        k3 = [f"k3_kernel<double, 2048, 8, 1, 1, 1, {e}, false, false>" for e in (2, 3, 4, 5)]
     else:
-        k3 = [f"k3_kernel<double, 2048, {p3}, 1, 1, {cpw}, -1>"]
+        k3 = [f"k3_kernel<double, 2048, {p3}, 1, 1, {cpw}, -1, false, false>"]
     k2 = f"k2_staged_kernel<double, 2048, {p2}>" if splits[1] == 2 else f"k2_kernel<double, 2048, {p2}, 1, 0, 1>"
-    return {"y_inverse_K1": [f"k1_kernel<double, 2048, {p1}, 1, 1, 0, 0>"],
+    return {"y_inverse_K1": [f"k1_kernel<double, 2048, {p1}, 1, 1, 0, 0, false>"],
             "x_pass_K2": [k2],
             "y_forward_stage_K3": k3}
 
