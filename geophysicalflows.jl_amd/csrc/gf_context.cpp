@@ -891,13 +891,13 @@ struct Ctx final : CtxBase {
   int split[3] = {0, 0, 0};
   // TwoDNavierStokes and SingleLayerQG take radix schedules and workgroup splits; MultiLayerQG
   // (any number of layers) has a fixed schedule
-  // (a column length that is not a power of two keeps the built-in schedule too)
-  bool tunable() const { return d.model != GF_MODEL_MLQG && is_pow2(d.ny); }
+  // (a grid length that is not a power of two keeps the built-in schedule too)
+  bool tunable() const { return d.model != GF_MODEL_MLQG && is_pow2(d.nx) && is_pow2(d.ny); }
   // the staged x pass: advection without background gradients, whole waves per workgroup
   bool k2_staged_offered(int p) const { return nl == 1 && BW == 2 && !advect_full && d.nx / p >= 64; }
   void set_splits(const int* v) override {
     if (!tunable() && (v[0] || v[1] || v[2]))
-      throw GfError(GF_EINVAL, "workgroup splits are for TwoDNavierStokes and SingleLayerQG problems with ny a power of two");
+      throw GfError(GF_EINVAL, "workgroup splits are for TwoDNavierStokes and SingleLayerQG problems with nx and ny powers of two");
     if (v[1] < 0 || v[1] > 2) throw GfError(GF_EINVAL, "K2 split must be 0, 1 (halves) or 2 (staged loads)");
     bool changed = false;
     for (int k = 0; k < 3; ++k) {
@@ -910,8 +910,9 @@ struct Ctx final : CtxBase {
   }
   void get_splits(int* v) const override { for (int k = 0; k < 3; ++k) v[k] = split[k]; }
   SchedKey sched_key() const { return SchedKey{d.model, nl, d.nx, d.ny, d.dtype}; }
-  // K2 transforms rows, K1/K3 columns (in the kernels of the next power of two where ny is not one)
-  int sched_len(int k) const { return k == 1 ? d.nx : pow2_ceil(d.ny); }
+  // K2 transforms rows, K1/K3 columns (in the kernels of the next power of two where ny is not one;
+  // a row length that is not one runs in the x pass of direct sums, which has no schedule)
+  int sched_len(int k) const { return k == 1 ? pow2_ceil(d.nx) : pow2_ceil(d.ny); }
   void set_schedule(const int* p) override {
     for (int k = 0; k < 3; ++k)
       if (p[k] != 0 && !(tunable() && sched_candidate(sched_len(k), p[k])))
@@ -1553,7 +1554,8 @@ struct Ctx final : CtxBase {
       if (d.model != GF_MODEL_TWODNS || d.stepper != GF_ETDRK4 || d.forcing != GF_FORCING_NONE)
         throw GfError(GF_EUNSUPPORTED, "the adjoint is provided for unforced TwoDNavierStokes ETDRK4 problems");
       if (d.nx != d.ny) throw GfError(GF_EUNSUPPORTED, "the adjoint needs a square grid");
-      if (d.nx > 2048) throw GfError(GF_EUNSUPPORTED, "the adjoint kernels are built for grids up to 2048^2");
+      if (d.nx > 2048 || !is_pow2(d.nx))
+        throw GfError(GF_EUNSUPPORTED, "the adjoint kernels are built for power-of-two grids up to 2048^2");
       if (nsteps < 1 || every < 1) throw GfError(GF_EINVAL, "nsteps and checkpoint_every must be >= 1");
       adjoint_setup();
       struct Active {
@@ -1864,8 +1866,8 @@ int validate(const gf_problem_desc* d, std::string& msg) {
   if (!d) { msg = "null descriptor"; return GF_EINVAL; }
   if (d->model < 0 || d->model > 2) { msg = "model must be GF_MODEL_TWODNS, GF_MODEL_SLQG or GF_MODEL_MLQG"; return GF_EINVAL; }
   if (d->dtype != GF_F64 && d->dtype != GF_F32) { msg = "dtype must be GF_F64 or GF_F32"; return GF_EINVAL; }
-  if (!size_supported(d->nx) || !(size_supported(d->ny) || ny_generic_supported(d->ny))) {
-    msg = "nx must be a power of two between 16 and 4096, ny a power of two between 16 and 4096 or even between 18 and " +
+  if (!(size_supported(d->nx) || generic_len_supported(d->nx)) || !(size_supported(d->ny) || generic_len_supported(d->ny))) {
+    msg = "nx and ny must each be a power of two between 16 and 4096 or even between 18 and " +
           std::to_string(GEN_MAX) + " (got " + std::to_string(d->nx) + "x" + std::to_string(d->ny) + ")";
     return GF_EUNSUPPORTED;
   }

@@ -292,12 +292,13 @@ template <typename T> void launch_to_julia(const Geo<T>& g, const cplx<T>* src, 
 template <typename T> void launch_from_julia(const Geo<T>& g, const cplx<T>* src, cplx<T>* dst, int nl, hipStream_t s);
 template <typename T> void launch_mask(const Geo<T>& g, cplx<T>* sol, int nl, hipStream_t s);
 void launch_counter(int64_t* c, int64_t v, int add, hipStream_t s);
-// Column lengths ny that are not powers of two (even, 18..GEN_MAX) run in the y-pass kernels of
-// the next power of two with a direct DFT (dft_direct); the x pass needs nx a power of two.
+// Grid lengths that are not powers of two (even, 18..GEN_MAX): a column length ny runs in the
+// y-pass kernels of the next power of two with a direct DFT (dft_direct), a row length nx in the
+// x-pass kernel of direct sums (k2_gen_kernel).
 constexpr int GEN_MAX = 256;
 __host__ __device__ constexpr int pow2_ceil(int n) { int m = 1; while (m < n) m *= 2; return m; }
 __host__ __device__ constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-inline bool ny_generic_supported(int n) { return n % 2 == 0 && n >= 18 && n <= GEN_MAX && !is_pow2(n); }
+inline bool generic_len_supported(int n) { return n % 2 == 0 && n >= 18 && n <= GEN_MAX && !is_pow2(n); }
 // Radix schedules offered to the autotuner for an FFT of length n (the host side of PCand).
 inline bool sched_candidate(int n, int p) {
   return p == 4 ? (n >= 16 && n <= 2048) : (p == 8 ? n >= 32 : (p == 16 ? n >= 1024 : false));

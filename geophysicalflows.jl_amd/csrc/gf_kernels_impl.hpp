@@ -641,6 +641,135 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
   }
 }
 
+// ---------------------------------------------------------------------------
+// K2 for rows whose length nx is not a power of two (even, 18..GEN_MAX): one workgroup per row,
+// direct sums over the row's modes and points (O(nx^2) per row) in place of the packed two-row
+// transforms. The same fields, products and outputs as k2_body's modes; the row's half spectra,
+// physical products and the nx roots of unity w^j = exp(-2 pi i j/nx) (the global table) sit in
+// LDS.
+// ---------------------------------------------------------------------------
+constexpr int K2G_THREADS = 64;
+template <typename T> struct K2GLds {
+  cplx<T> hs[3][GEN_MAX / 2 + 1];
+  T ph[3][GEN_MAX];
+  cplx<T> roots[GEN_MAX];
+};
+// c2r of the half spectrum h (columns kr < kin present) at point x: Re h_0 + 2 sum Re(h_k w^-kx)
+// + Re h_(nx/2) (-1)^x; the imaginary parts of the DC and Nyquist columns are ignored, as c2r does
+template <typename T>
+__device__ __forceinline__ T c2r_point(const cplx<T>* h, int kin, int nx, int x, const cplx<T>* roots) {
+  const int kc = kin < nx / 2 + 1 ? kin : nx / 2 + 1;
+  T acc = kc > 0 ? h[0].re : (T)0;
+  int idx = 0;
+  for (int k = 1; k < kc; ++k) {
+    idx += x;
+    if (idx >= nx) idx -= nx;                          // k x mod nx
+    const cplx<T> w = roots[idx];
+    if (2 * k == nx) acc += h[k].re * w.re;
+    else acc += (T)2 * (h[k].re * w.re + h[k].im * w.im);   // 2 Re(h_k conj(w))
+  }
+  return acc;
+}
+// r2c of the physical row p at column kr: sum_x p_x w^(kr x)
+template <typename T>
+__device__ __forceinline__ cplx<T> r2c_point(const T* p, int nx, int kr, const cplx<T>* roots) {
+  cplx<T> acc = czero<T>();
+  int idx = 0;
+  for (int x = 0; x < nx; ++x) {
+    acc = acc + rmul(p[x], roots[idx]);
+    idx += kr;
+    if (idx >= nx) idx -= nx;
+  }
+  return acc;
+}
+template <typename T, int MODE>
+__global__ __launch_bounds__(K2G_THREADS) void k2_gen_kernel(const K2Args<T> a) {
+  __shared__ K2GLds<T> S;
+  const Geo<T>& g = a.g;
+  const int nx = g.nx, ny = g.ny, nkr = nx / 2 + 1, t = (int)threadIdx.x, y = (int)blockIdx.x;
+  for (int i = t; i < nx; i += K2G_THREADS) S.roots[i] = a.tw[i];
+  // half spectrum of row y of field f (kr < kin, zero beyond) into S.hs[s]
+  auto load = [&](int s, const cplx<T>* f) {
+    for (int k = t; k < nkr; k += K2G_THREADS) S.hs[s][k] = k < a.kin ? at(f, boff32(k, y, ny)) : czero<T>();
+  };
+  if constexpr (MODE == K2_INVERSE_OUT) {
+    for (int f = 0; f < a.nfields; ++f) {
+      __syncthreads();
+      load(0, a.in + (long)f * a.fstride);
+      __syncthreads();
+      for (int x = t; x < nx; x += K2G_THREADS) a.phys[f * a.pstride + (long)y * nx + x] = c2r_point(S.hs[0], a.kin, nx, x, S.roots);
+    }
+  } else if constexpr (MODE == K2_FORWARD_IN) {
+    for (int f = 0; f < a.nfields; ++f) {
+      __syncthreads();
+      for (int x = t; x < nx; x += K2G_THREADS) S.ph[0][x] = a.phys_in[f * a.pstride + (long)y * nx + x];
+      __syncthreads();
+      cplx<T>* o = a.out + (long)f * a.ostride;
+      for (int kr = t; kr < a.kout; kr += K2G_THREADS) at(o, boff32(kr, y, ny)) = r2c_point(S.ph[0], nx, kr, S.roots);
+    }
+  } else {
+    // K2_ADVECT: u (q [+ eta]), v (q [+ eta]); K2_ADVECT_FULL, per layer: u += U,
+    // (-i kr (u q)^ - (u Qx + v Qy)^, (v q)^), or with `linear` U q in place of u q and no v q
+    constexpr bool FULL = MODE == K2_ADVECT_FULL;
+    const int nl = FULL ? g.nlayers : 1;
+    for (int jl = 0; jl < nl; ++jl) {
+      __syncthreads();
+      for (int s = 0; s < 3; ++s) load(s, a.in + (long)(3 * jl + s) * a.fstride);
+      __syncthreads();
+      for (int x = t; x < nx; x += K2G_THREADS) {
+        T u = c2r_point(S.hs[0], a.kin, nx, x, S.roots);
+        const T v = c2r_point(S.hs[1], a.kin, nx, x, S.roots);
+        T q = c2r_point(S.hs[2], a.kin, nx, x, S.roots);
+        if constexpr (!FULL) {
+          if (a.eta != nullptr) q = q + a.eta[(long)y * nx + x];
+          S.ph[0][x] = u * q;
+          S.ph[1][x] = v * q;
+        } else {
+          T c = 0, U = 0;
+          if (a.has_bg) {
+            U = a.Uy[jl * ny + y];
+            u = u + U;
+            T qy = a.Qy1[jl * ny + y], qx = 0;
+            if (jl == nl - 1) {
+              if (a.etax) qx = a.etax[(long)y * nx + x];
+              if (a.etay) qy = qy + a.etay[(long)y * nx + x];
+            }
+            c = u * qx + v * qy;
+          }
+          S.ph[0][x] = a.linear ? U * q : u * q;
+          S.ph[1][x] = a.linear ? (T)0 : v * q;
+          S.ph[2][x] = c;
+        }
+      }
+      __syncthreads();
+      cplx<T>* oA = a.out + (long)(FULL ? 2 * jl : 0) * a.ostride;
+      cplx<T>* oB = oA + a.ostride;
+      for (int kr = t; kr < a.kout; kr += K2G_THREADS) {
+        cplx<T> A = r2c_point(S.ph[0], nx, kr, S.roots);
+        const cplx<T> B = r2c_point(S.ph[1], nx, kr, S.roots);
+        if constexpr (FULL) {
+          const cplx<T> C = r2c_point(S.ph[2], nx, kr, S.roots);
+          const T kv = krwave(g, kr);
+          A = mk(kv * A.im - C.re, -kv * A.re - C.im);   // -i kr (u q)^ - (u Qx + v Qy)^
+        }
+        at(oA, boff32(kr, y, ny)) = A;
+        at(oB, boff32(kr, y, ny)) = B;
+      }
+    }
+  }
+}
+template <typename T>
+void k2_gen(const K2Args<T>& a, hipStream_t s) {
+  const dim3 grid(a.g.ny), blk(K2G_THREADS);
+  switch (a.mode) {
+    case K2_ADVECT: GF_LAUNCH((k2_gen_kernel<T, K2_ADVECT>), grid, blk, 0, s, a); break;
+    case K2_ADVECT_FULL: GF_LAUNCH((k2_gen_kernel<T, K2_ADVECT_FULL>), grid, blk, 0, s, a); break;
+    case K2_INVERSE_OUT: GF_LAUNCH((k2_gen_kernel<T, K2_INVERSE_OUT>), grid, blk, 0, s, a); break;
+    case K2_FORWARD_IN: GF_LAUNCH((k2_gen_kernel<T, K2_FORWARD_IN>), grid, blk, 0, s, a); break;
+    default: break;
+  }
+}
+
 #ifndef GF_K2_ADVECT_WAVES
 #define GF_K2_ADVECT_WAVES 3   // a floor: at 2048, P = 8 the kernel needs 122 VGPRs and runs at 4
 #endif
@@ -1251,6 +1380,9 @@ void k2_np(const K2Args<T>& a, hipStream_t s) {
 }
 template <typename T, int N, int NL>
 void k2_n(const K2Args<T>& a, hipStream_t s) {
+  if constexpr (N <= GEN_MAX) {
+    if (a.g.nx != N) { k2_gen<T>(a, s); return; }   // a row length below N: direct sums
+  }
   const bool tuned = NL == 1 && (a.mode == K2_ADVECT || a.mode == K2_ADVECT_FULL) && a.p != 0 &&
                      a.p != PY<N>::P && PCand<N>::has(a.p);
   if constexpr (NL == 1) {

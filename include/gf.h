@@ -56,7 +56,7 @@ enum { GF_FORCING_NONE = 0, GF_FORCING_FIXED = 1, GF_FORCING_RING = 2 };
 typedef struct gf_problem_desc {
   int model;                 /* GF_MODEL_*                                         */
   int dtype;                 /* GF_F64 | GF_F32  (keyword T)                        */
-  int nx, ny, nlayers;       /* nx a power of two 16..4096; ny the same or even 18..256; nlayers: MLQG 1..8, else 1 */
+  int nx, ny, nlayers;       /* nx, ny each a power of two 16..4096 or even 18..256; nlayers: MLQG 1..8, else 1 */
   double Lx, Ly;
   double aliased_fraction;   /* default 1/3                                         */
   int stepper;               /* GF_ETDRK4 ...                                       */

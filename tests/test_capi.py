@@ -63,7 +63,8 @@ def _desc(gf, **kw):
 
 
 @pytest.mark.parametrize("bad,code", [
-    (dict(nx=48), -2), (dict(ny=8192), -2), (dict(ny=65), -2), (dict(ny=258), -2), (dict(nx=66), -2),
+    (dict(nx=8192), -2), (dict(ny=8192), -2), (dict(ny=65), -2), (dict(ny=258), -2), (dict(nx=67), -2),
+    (dict(nx=260), -2), (dict(nx=16, ny=14), -2),
     (dict(dt=0.0), -1), (dict(model=7), -1),
     (dict(stepper=9), -1), (dict(dtype=3), -1), (dict(model=2, nlayers=9), -2),
     (dict(aliased_fraction=1.5), -1), (dict(Lx=-1.0), -1), (dict(nlayers=2), -1),

@@ -1,6 +1,7 @@
-"""Column lengths ny that are not powers of two (even, 18..256): the reference's own grids
-64 x 66, 128 x 126 and 32 x 34 (test/test_multilayerqg.jl:232-701) run in the y-pass kernels of
-the next power of two, with a direct DFT in place of the Stockham transform.
+"""Grid lengths that are not powers of two (even, 18..256). A column length ny (the reference's
+own grids 64 x 66, 128 x 126 and 32 x 34, test/test_multilayerqg.jl:232-701) runs in the y-pass
+kernels of the next power of two, with a direct DFT in place of the Stockham transform; a row
+length nx runs in the x pass of direct sums (one workgroup per row).
 
 Tolerances: FP64 1e-13 relative for one calcN and for updatevars/set_psi!, 1e-12 relative L2
 after 3-4 steps; FP32 2e-6 for calcN and 2e-5 after 3 steps, as the other FP32 checks; the
@@ -26,7 +27,9 @@ def rand_spec(rng, shape):
 
 
 @pytest.mark.parametrize("nx,ny,dtype", [(64, 66, np.float64), (128, 126, np.float64), (32, 34, np.float64),
-                                         (256, 250, np.float64), (64, 18, np.float64), (64, 66, np.float32)])
+                                         (256, 250, np.float64), (64, 18, np.float64), (64, 66, np.float32),
+                                         (66, 64, np.float64), (126, 126, np.float64), (254, 128, np.float64),
+                                         (18, 32, np.float64), (68, 66, np.float64), (66, 64, np.float32)])
 def test_twodns_ring_forced_calcN_steps_updatevars(gf, nx, ny, dtype):
     Ly = TWO_PI * ny / nx
     kw = dict(nx=nx, ny=ny, Lx=TWO_PI, Ly=Ly, dt=0.002, stepper="ETDRK4")
@@ -51,10 +54,10 @@ def test_twodns_ring_forced_calcN_steps_updatevars(gf, nx, ny, dtype):
     assert rel(p.vars.ζ, r.vars.zeta) < t1 and rel(p.vars.u, r.vars.u) < t1
 
 
-def test_twodns_set_zeta_round_trip_and_energy(gf):
-    # physical -> spectral (K2 forward, K3 store) and back on 64 x 66, and the device energy of
-    # a state with a closed form: E = <zeta^2>/(2 |k|^2) per mode, (1/4)/13/2 + (1/8)/5/2
-    nx, ny = 64, 66
+@pytest.mark.parametrize("nx,ny", [(64, 66), (66, 64), (90, 90)])
+def test_twodns_set_zeta_round_trip_and_energy(gf, nx, ny):
+    # physical -> spectral (K2 forward, K3 store) and back, and the device energy of a state
+    # with a closed form: E = <zeta^2>/(2 |k|^2) per mode, (1/4)/13/2 + (1/8)/5/2
     p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=nx, ny=ny, Lx=TWO_PI, Ly=TWO_PI, dt=0.01, stepper="RK4")
     r = o2d.Problem(nx=nx, ny=ny, Lx=TWO_PI, Ly=TWO_PI, dt=0.01, stepper="RK4")
     x, y = r.grid.gridpoints()
@@ -68,12 +71,15 @@ def test_twodns_set_zeta_round_trip_and_energy(gf):
     assert abs(e - (0.25 / 13 / 2 + 0.125 / 5 / 2)) < 1e-13
 
 
-@pytest.mark.parametrize("ell,U,topo", [(None, 0.0, False), (1.5, 0.3, True)])
-def test_slqg_calcN_and_steps(gf, ell, U, topo):
-    nx, ny = 64, 66
+@pytest.mark.parametrize("nx,ny,ell,U,topo", [(64, 66, None, 0.0, False), (64, 66, 1.5, 0.3, True),
+                                              (66, 64, None, 0.0, False), (66, 64, 1.5, 0.3, True),
+                                              (66, 64, None, "y", True)])
+def test_slqg_calcN_and_steps(gf, nx, ny, ell, U, topo):
     g = off.TwoDGrid(nx, TWO_PI, ny, TWO_PI)
     X, Y = g.gridpoints()
     eta = 0.4 * np.cos(3 * X) * np.cos(2 * Y) if topo else None
+    if U == "y":   # U(y): the x pass with background gradients
+        U = 0.3 / np.cosh(2 * g.y) ** 2
     kw = dict(nx=nx, ny=ny, Lx=TWO_PI, Ly=TWO_PI, U=U, eta=eta, dt=0.005, stepper="FilteredRK4")
     p = gf.SingleLayerQG.Problem(gf.GPU(), β=2.0, deformation_radius=ell, ν=1e-3, μ=0.01, **kw)
     r = osl.Problem(beta=2.0, deformation_radius=ell, nu=1e-3, mu=0.01, **kw)
@@ -89,11 +95,11 @@ def test_slqg_calcN_and_steps(gf, ell, U, topo):
     assert rel(p.sol, r.sol) < 1e-12
 
 
-@pytest.mark.parametrize("nlayers", [1, 2, 3])
-def test_mlqg_calcN_steps_updatevars_64x66(gf, nlayers):
-    # the reference's 64 x 66 grid (test/test_multilayerqg.jl:232-300), with U(y), topography,
-    # beta and bottom drag
-    nx, ny = 64, 66
+@pytest.mark.parametrize("nlayers,nx,ny", [(1, 64, 66), (2, 64, 66), (3, 64, 66), (1, 66, 66), (2, 66, 66),
+                                           (3, 66, 66), (2, 126, 64)])
+def test_mlqg_calcN_steps_updatevars(gf, nlayers, nx, ny):
+    # the reference's 64 x 66 grid (test/test_multilayerqg.jl:232-300), and non-power-of-two rows,
+    # with U(y), topography, beta and bottom drag
     H = [0.2, 0.3, 0.5][:nlayers] if nlayers > 1 else None
     b = [-1.0, -1.2, -1.3][:nlayers] if nlayers > 1 else None
     g = off.TwoDGrid(nx, TWO_PI, ny, TWO_PI)
@@ -120,6 +126,50 @@ def test_mlqg_calcN_steps_updatevars_64x66(gf, nlayers):
     oml.updatevars(r)
     for a, bb in ((p.vars.q, r.vars.q), (p.vars.ψ, r.vars.psi), (p.vars.u, r.vars.u), (p.vars.v, r.vars.v)):
         assert rel(np.reshape(a, bb.shape), bb) < 1e-13
+
+
+@pytest.mark.parametrize("nlayers", [1, 2])
+def test_mlqg_linear_on_nonpow2_rows(gf, nlayers):
+    # the linearised equations (U q in place of u q) on 66 x 64
+    nx, ny = 66, 64
+    g = off.TwoDGrid(nx, TWO_PI, ny, TWO_PI)
+    Uy = np.stack([0.3 * np.sin(2 * g.y) * (nlayers - j) / nlayers for j in range(nlayers)], axis=1)
+    kw = dict(nx=nx, ny=ny, Lx=TWO_PI, Ly=TWO_PI, U=Uy if nlayers > 1 else Uy[:, 0], dt=0.004,
+              stepper="RK4", linear=True)
+    if nlayers > 1:
+        kw.update(H=[0.4, 0.6], b=[-1.0, -1.2])
+    p = gf.MultiLayerQG.Problem(nlayers, gf.GPU(), β=3.0, μ=0.05, ν=1e-3, **kw)
+    r = oml.Problem(nlayers, beta=3.0, mu=0.05, nu=1e-3, **kw)
+    s = rand_spec(np.random.default_rng(70 + nlayers), r.sol.shape) * 0.05
+    p.sol = s.reshape(p._sshape)
+    r.sol[...] = s
+    gf.stepforward(p, 3)
+    off.stepforward(r, 3)
+    assert rel(p.sol.reshape(r.sol.shape), r.sol) < 1e-12
+
+
+@pytest.mark.parametrize("topo,vary", [(True, False), (False, True)])
+def test_slqg_reference_nonlinear_advection_126x128(gf, topo, vary):
+    """test/test_singlelayerqg.jl:276-399 (ForwardEuler, dt = 5e-4, 2000 steps, tolerance 1e-13)
+    on a 126 x 128 grid: the steady state's closed form holds whatever the row length."""
+    from tests.test_gpu_parity import _slqg_forcing
+    nx, ny = 126, 128
+    g = off.TwoDGrid(nx, TWO_PI, ny, TWO_PI)
+    x, y = g.gridpoints()
+    eta0 = 0.4 if topo else 0.0
+    eta = eta0 * np.cos(10 * x) * np.cos(10 * y)
+    qf = -8 * np.sin(2 * x) * np.cos(2 * y) - 20 * np.sin(x) * np.cos(3 * y)
+    U, Uyy = 0.2, 0.0
+    if vary:
+        U = 0.2 / np.cosh(2 * y) ** 2
+        Uyy = g.irfft(-g.Lw ** 2 * g.rfft(U))
+    Fh = _slqg_forcing(g, eta0, U, Uyy)
+    p = gf.SingleLayerQG.Problem(gf.GPU(), nx=nx, ny=ny, Lx=TWO_PI, Ly=TWO_PI, U=U[:, 0] if vary else U,
+                                 eta=eta, ν=1e-2, nν=1, dt=5e-4, stepper="ForwardEuler", calcF=Fh)
+    gf.SingleLayerQG.set_q(p, qf)
+    gf.stepforward(p, 2000)
+    gf.SingleLayerQG.updatevars(p)
+    assert rel(p.vars.q, qf) < 1e-13
 
 
 def test_mlqg_energies_on_the_reference_grid(gf):
@@ -190,18 +240,17 @@ def test_mlqg_set_psi_set_q_round_trip_32x34(gf):
     assert rel(np.reshape(p.vars.q, f.shape), f) < 1e-13
 
 
-def test_unsupported_column_lengths_and_schedules(gf):
-    with pytest.raises(gf._lib.GFError, match="even between 18 and 256"):
-        gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, ny=65, stepper="RK4")
-    with pytest.raises(gf._lib.GFError, match="even between 18 and 256"):
-        gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, ny=258, stepper="RK4")
-    with pytest.raises(gf._lib.GFError, match="nx must be a power of two"):
-        gf.TwoDNavierStokes.Problem(gf.GPU(), nx=66, ny=64, stepper="RK4")
-    with pytest.raises(gf._lib.GFError, match="between 16 and 4096"):
-        gf.TwoDNavierStokes.Problem(gf.GPU(), nx=8192, stepper="RK4")
-    p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, ny=66, stepper="RK4")
-    assert gf.schedule(p) == (8, 8, 8)        # the 128-point y kernels' built-in P
-    with pytest.raises(gf._lib.GFError, match="not offered"):
-        gf.set_schedule(p, 4, 8, 4)
-    with pytest.raises(gf._lib.GFError, match="ny a power of two"):
-        gf.set_splits(p, 1, 0, 0)
+def test_unsupported_grid_lengths_and_schedules(gf):
+    for nx, ny in ((64, 65), (64, 258), (65, 64), (258, 64), (8192, 64)):
+        with pytest.raises(gf._lib.GFError, match="even between 18 and 256"):
+            gf.TwoDNavierStokes.Problem(gf.GPU(), nx=nx, ny=ny, stepper="RK4")
+    for nx, ny in ((64, 66), (66, 64)):
+        p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=nx, ny=ny, stepper="RK4")
+        assert gf.schedule(p) == (8, 8, 8)        # the 128-point kernels' built-in P
+        with pytest.raises(gf._lib.GFError, match="not offered"):
+            gf.set_schedule(p, 4, 8, 4)
+        with pytest.raises(gf._lib.GFError, match="nx and ny powers of two"):
+            gf.set_splits(p, 1, 0, 0)
+    p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=66, ny=66, dt=0.01, stepper="ETDRK4")
+    with pytest.raises(gf._lib.GFError, match="power-of-two grids"):
+        gf.TwoDNavierStokes.adjoint_gradient(p, np.zeros((66, 66)), 2)
