@@ -168,29 +168,48 @@ __device__ __forceinline__ void powers(cplx<T> (&w)[R]) {
 // address register per pass and no per-access shift and mask.
 __host__ __device__ __forceinline__ constexpr int pad(int i) { return i + (i >> 3); }
 template <int N> struct LdsN { static constexpr int value = N + N / 8; };
+// One pad slot after every G points (G a power of two, G >= 8: within LdsN), additive as pad is
+// for sums whose parts mod G add up to less than G.
+template <int G> __host__ __device__ __forceinline__ constexpr int padg(int i) { return i + i / G; }
+// Padding of the Stockham exchange after a pass of radix R at stride NS, for points of S bytes,
+// from LDS timings on gfx950 (64 banks of 4 bytes). The exchange writes point base + r NS and
+// reads consecutive points on consecutive lanes.
+// - FP64 (16-byte slots, ds_read_b128 / ds_write_b128): the reads run at full rate with one pad
+//   per 32 points and at half rate with one per 8; writes at stride 16 run as fast with one per
+//   32 as with one per 8, writes at stride 8 or 4 (a first pass of radix 8 or 4) only with one
+//   per 8.
+// - FP32 (8-byte slots, 64-bit accesses): one pad per 16 points keeps the reads and the writes
+//   at stride 16 or 8 at full rate (one per 8 halves the reads); only the radix-4 passes at
+//   stride 1 or 4 keep one per 8.
+template <int S, int NS, int R>
+constexpr int xpad_g() {
+  if constexpr (S == 16) return (NS == 1 && R <= 8) ? 8 : 32;
+  else return (R <= 4 && NS <= 4) ? 8 : 16;
+}
 
 // Slots of kr = t + i TPT and of its mirror (N - kr) mod N in an N-point buffer, as used by
 // the split of a packed spectrum W = F(a + i b) of two real sequences (W[k] with W[N - k]).
 // For t > 0 the mirror is tm + (N - (i + 1) TPT) with tm = TPT - t; for t = 0 it is
 // (N - i TPT) mod N.  Both are a per-thread base plus a compile-time offset when TPT is a
-// multiple of 8 (pad is then additive over multiples of TPT); smaller TPT computes each slot.
-template <int N, int TPT>
+// multiple of the padding period G (padg<G> is then additive over multiples of TPT); smaller TPT
+// computes each slot.
+template <int N, int TPT, int G = 8>
 struct MirrorSlots {
   int lo, m0, mb;
   __host__ __device__ explicit MirrorSlots(int t) {
-    lo = pad(t);
-    if constexpr (TPT % 8 == 0) {
-      m0 = t != 0 ? pad(TPT - t) + pad(N - TPT) : 0;
-      mb = t != 0 ? pad(TPT - t) - pad(TPT) : 0;
+    lo = padg<G>(t);
+    if constexpr (TPT % G == 0) {
+      m0 = t != 0 ? padg<G>(TPT - t) + padg<G>(N - TPT) : 0;
+      mb = t != 0 ? padg<G>(TPT - t) - padg<G>(TPT) : 0;
     } else {
       m0 = t;   // the thread index, for the computed form
       mb = 0;
     }
   }
-  __host__ __device__ int k(int i) const { return lo + pad(i * TPT); }
+  __host__ __device__ int k(int i) const { return lo + padg<G>(i * TPT); }
   __host__ __device__ int m(int i) const {
-    if constexpr (TPT % 8 == 0) return i == 0 ? m0 : mb + pad(N - i * TPT);
-    else return pad((N - m0 - i * TPT) & (N - 1));
+    if constexpr (TPT % G == 0) return i == 0 ? m0 : mb + padg<G>(N - i * TPT);
+    else return padg<G>((N - m0 - i * TPT) & (N - 1));
   }
 };
 
@@ -254,20 +273,21 @@ __device__ __forceinline__ void stockham(cplx<T> (&v)[C][P], cplx<T>* __restrict
     }
   }
   if constexpr (PASS + 1 < PL::NPASS) {
+    constexpr int G = xpad_g<(int)sizeof(cplx<T>), NS, R>();
 #pragma unroll
     for (int c = 0; c < C; ++c) {
 #pragma unroll
       for (int b = 0; b < M; ++b) {
         const int j = t + b * TPT;
         const int base = (j / NS) * NS * R + (j % NS);
-        cplx<T>* wp = lds + pad(base);   // slot base + r NS is pad(base) + pad(r NS)
+        cplx<T>* wp = lds + padg<G>(base);   // slot base + r NS is padg(base) + padg(r NS)
 #pragma unroll
-        for (int r = 0; r < R; ++r) wp[pad(r * NS)] = v[c][b + r * M];
+        for (int r = 0; r < R; ++r) wp[padg<G>(r * NS)] = v[c][b + r * M];
       }
       xbarrier<LIGHT>();
-      const cplx<T>* rp = lds + pad(t);
+      const cplx<T>* rp = lds + padg<G>(t);
 #pragma unroll
-      for (int q = 0; q < P; ++q) v[c][q] = rp[pad(TPT * q)];
+      for (int q = 0; q < P; ++q) v[c][q] = rp[padg<G>(TPT * q)];
       xbarrier<LIGHT>();
     }
     stockham<T, N, P, C, PASS + 1, NS * R, LIGHT>(v, lds, t, tw);

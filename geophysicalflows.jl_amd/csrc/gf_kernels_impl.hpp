@@ -309,6 +309,10 @@ void k1_kernel(const K1Args<T> a) {
 // ---------------------------------------------------------------------------
 // K2: mixed -> flux (x-direction c2r, products, r2c) / forward / inverse-out
 // ---------------------------------------------------------------------------
+// Padding period of the x pass's own exchanges through LDS (mirrored points, spectrum splits),
+// where consecutive lanes touch consecutive points: one pad per 32 FP64 or 16 FP32 points keeps
+// gfx950's LDS reads at full rate (one per 8 halves them).
+template <typename T> struct XPad { static constexpr int G = sizeof(T) == 8 ? 32 : 16; };
 // Hermitian-extended entry k of a half spectrum (kr = k or NX-k).
 template <typename T, int NX>
 __device__ __forceinline__ cplx<T> herm_load(const cplx<T>* f, int k, int y, int ny, int kin, bool active) {
@@ -372,13 +376,13 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
       }
       fft_forward<T, NX, P, 1, (NX >= FreshFrom<T>::k2)>(w, L, t, tw);
 #pragma unroll
-      for (int q = 0; q < P; ++q) L[pad(t) + pad(TPT * q)] = w[0][q];
+      for (int q = 0; q < P; ++q) L[padg<XPad<T>::G>(t) + padg<XPad<T>::G>(TPT * q)] = w[0][q];
       __syncthreads();
       if (active) {
         cplx<T>* o = a.out + f * a.ostride;
         for (int it = t; it < 2 * a.kout; it += TPT) {
           const int row = it & 1, kr = it >> 1;
-          o[boff(kr, y0 + row, ny)] = split_row(L[pad(kr)], L[pad((NX - kr) & (NX - 1))], row);
+          o[boff(kr, y0 + row, ny)] = split_row(L[padg<XPad<T>::G>(kr)], L[padg<XPad<T>::G>((NX - kr) & (NX - 1))], row);
         }
       }
       __syncthreads();
@@ -441,7 +445,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
         const int k = t + TPT * q;
         if (k > NX / 2) continue;
         w[0][q] = mk(A[q].re - B[q].im, -(A[q].im + B[q].re));     // conj(A + i B)
-        if (k > 0 && k < NX / 2) L[pad(NX - k)] = mk(A[q].re + B[q].im, A[q].im - B[q].re);
+        if (k > 0 && k < NX / 2) L[padg<XPad<T>::G>(NX - k)] = mk(A[q].re + B[q].im, A[q].im - B[q].re);
       }
       __syncthreads();
 #pragma unroll
@@ -449,7 +453,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
         if (TPT * q + TPT - 1 <= NX / 2) continue;      // every lane of this q holds k <= NX/2
         const int k = t + TPT * q;
         if (k <= NX / 2) continue;
-        w[0][q] = L[pad(k)];
+        w[0][q] = L[padg<XPad<T>::G>(k)];
       }
       __syncthreads();
     };
@@ -459,7 +463,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
     };
     auto forward_rows = [&](cplx<T>* o, cplx<T> (&w)[1][P]) {
       fft_forward<T, NX, P, 1, (NX >= FreshFrom<T>::k2)>(w, L, t, tw);             // of (row y0 + i row y0 + 1)
-      const MirrorSlots<NX, TPT> ms(t);
+      const MirrorSlots<NX, TPT, XPad<T>::G> ms(t);
 #pragma unroll
       for (int q = 0; q < P; ++q) L[ms.k(q)] = w[0][q];
       __syncthreads();
@@ -592,7 +596,7 @@ __device__ __forceinline__ void k2_body(const K2Args<T>& a, int blk, cplx<T>* __
       // split the packed spectra; rows handled one at a time through LDS
       cplx<T>* oA = a.out + (long)(2 * jl + 0) * a.ostride;
       cplx<T>* oB = a.out + (long)(2 * jl + 1) * a.ostride;
-      const MirrorSlots<NX, TPT> ms(t);
+      const MirrorSlots<NX, TPT, XPad<T>::G> ms(t);
       for (int row = 0; row < 2; ++row) {
 #pragma unroll
         for (int q = 0; q < P; ++q) L[ms.k(q)] = z[row][q];
@@ -855,7 +859,7 @@ __device__ __forceinline__ void staged_take(cplx<T> (&w)[1][P], const cplx<T>* _
     if (k > NX / 2) continue;
     const cplx<T> A = staged<T, NX>(S, k, 0, kin), B = staged<T, NX>(S, k, 1, kin);
     w[0][q] = mk(A.re - B.im, -(A.im + B.re));
-    if (k > 0 && k < NX / 2) L[pad(NX - k)] = mk(A.re + B.im, A.im - B.re);
+    if (k > 0 && k < NX / 2) L[padg<XPad<T>::G>(NX - k)] = mk(A.re + B.im, A.im - B.re);
   }
   lds_barrier();
 #pragma unroll
@@ -863,7 +867,7 @@ __device__ __forceinline__ void staged_take(cplx<T> (&w)[1][P], const cplx<T>* _
     if (TPT * q + TPT - 1 <= NX / 2) continue;
     const int k = t + TPT * q;
     if (k <= NX / 2) continue;
-    w[0][q] = L[pad(k)];
+    w[0][q] = L[padg<XPad<T>::G>(k)];
   }
   lds_barrier();   // S and L are free again
 }
@@ -875,7 +879,7 @@ __device__ __forceinline__ void staged_store(cplx<T>* o, cplx<T> (&w)[1][P], int
   constexpr int MAXI = (NX / 2 + 1 + TPT - 1) / TPT;
   constexpr bool FR = NX >= FreshFrom<T>::k2;
   fft_forward<T, NX, P, 1, FR, true>(w, L, t, tw);
-  const MirrorSlots<NX, TPT> ms(t);
+  const MirrorSlots<NX, TPT, XPad<T>::G> ms(t);
 #pragma unroll
   for (int q = 0; q < P; ++q) L[ms.k(q)] = w[0][q];
   lds_barrier();

@@ -2,8 +2,9 @@
 LDS slots and the ring-forcing box rows.
 
 The kernels address the padded LDS buffers as a per-thread base slot plus compile-time offsets
-(``pad(a + b) = pad(a) + pad(b)`` when ``a mod 8 + b mod 8 < 8``).  tests/host/fft_slots_check.cpp
-compares that form with the direct slot for every point of every plan, exhaustively.
+(``padg<G>(a + b) = padg<G>(a) + padg<G>(b)`` when ``a mod G + b mod G < G``, for the padding
+period G of each exchange).  tests/host/fft_slots_check.cpp compares that form with the direct
+slot for every point of every plan, both point sizes and every period, exhaustively.
 """
 import os
 import shutil

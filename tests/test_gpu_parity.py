@@ -386,7 +386,7 @@ def test_full_size_2048_forced_steps_match_oracle(gf):
 
 def test_full_size_2048_fp32_shipped_schedule_matches_fp64_oracle(gf):
     """The Float32 variant of the benchmark workload under the shipped tuned schedule
-    (P = (8, 8, 8), K1 field split with the per-transform address refresh that FP32 K1 uses
+    (P = (8, 16, 4), K1 field split with the per-transform address refresh that FP32 K1 uses
     from 2048 points, K2 staged loads, K3 column split): 3 steps in one call against the FP64 oracle started
     from the same (FP32-rounded) state.  Tolerance 2e-5 relative L2, as for the other FP32
     checks: a few hundred FP32 roundings deep (eps 6e-8) per step."""
@@ -395,7 +395,7 @@ def test_full_size_2048_fp32_shipped_schedule_matches_fp64_oracle(gf):
     z = np.random.default_rng(22).standard_normal((n, n))
     p = gf.TwoDNavierStokes.Problem(gf.GPU(), ν=1e-21, nν=4, μ=0.1, nμ=0, T=np.float32,
                                     calcF=gf.RingForcing(14.0, 1.5, 0.1, seed=2), **kw)
-    assert gf.schedule(p) == (8, 8, 8) and gf.splits(p) == (1, 2, 1)
+    assert gf.schedule(p) == (8, 16, 4) and gf.splits(p) == (1, 2, 1)
     gf.TwoDNavierStokes.set_ζ(p, z)
     s0 = p.sol.astype(np.complex128)
     r = o2d.Problem(nu=1e-21, nnu=4, mu=0.1, nmu=0, calcF=("ring", 14.0, 1.5, 0.1, 2), **kw)
