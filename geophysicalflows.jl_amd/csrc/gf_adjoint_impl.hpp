@@ -142,7 +142,7 @@ __device__ __forceinline__ void ka2_body(const AdjK2Args<T>& a, int blk, cplx<T>
   // forward x-transform of two real rows packed as r0 + i r1, split, stored as flux field f
   auto forward_rows = [&](int f, cplx<T> (&w)[1][P]) {
     fft_forward<T, NX, P, 1>(w, L, t, tw);
-    const MirrorSlots<NX, TPT> ms(t);
+    const MirrorSlots<NX, TPT, XPad<T>::G> ms(t);
 #pragma unroll
     for (int q = 0; q < P; ++q) L[ms.k(q)] = w[0][q];
     __syncthreads();

@@ -214,6 +214,29 @@ def test_mlqg_fluxes_on_the_reference_grid(gf, nlayers):
     assert abs(lat[0] - latr[0]) <= tol * abs(latr[0]) and abs(ver[0] - verr[0]) <= tol * abs(verr[0])
 
 
+def test_mlqg_fluxes_on_nonpow2_rows(gf):
+    # the flux diagnostic (x sums by Parseval over a row length that is not a power of two)
+    # against the oracle on 126 x 128, two layers, U(y) in the upper layer; the fields are
+    # resolved on both grids, so the reference's values for its 128 x 126 grid hold too
+    # (test/test_multilayerqg.jl:567-625: 0.00626267 and -0.196539, rtol 1e-6)
+    nx, ny = 126, 128
+    g = off.TwoDGrid(nx, TWO_PI, ny, TWO_PI)
+    U = np.zeros((ny, 2))
+    U[:, 0] = 1 / np.cosh(g.y / 0.2) ** 2
+    kw = dict(nx=nx, ny=ny, Lx=TWO_PI, Ly=TWO_PI, f0=1, H=[0.2, 0.8], b=[-1.0, -1.2], U=U)
+    p = gf.MultiLayerQG.Problem(2, gf.GPU(), dt=0.01, stepper="RK4", **kw)
+    r = oml.Problem(2, **kw)
+    x, y = r.grid.gridpoints()
+    psi = np.array([np.cos(x) * np.cos(y) + np.sin(x), np.cos(x + np.pi / 10) * np.cos(y)])
+    gf.MultiLayerQG.set_ψ(p, psi)
+    oml.set_psi(r, psi)
+    lat, ver = gf.MultiLayerQG.fluxes(p)
+    latr, verr = oml.fluxes(r)
+    assert abs(lat[0] - latr[0]) <= 1e-12 * abs(latr[0]) and abs(ver[0] - verr[0]) <= 1e-12 * abs(verr[0])
+    assert abs(lat[1] - latr[1]) <= 1e-12 * abs(latr[0])
+    assert abs(lat[0] - 0.00626267) <= 1e-6 * 0.00626267 and abs(ver[0] + 0.196539) <= 1e-6 * 0.196539
+
+
 def test_mlqg_single_layer_fluxes_on_the_reference_grid(gf):
     # test/test_multilayerqg.jl:632-655 on 128 x 126: 0.0313134 (atol 1e-7)
     nx, ny = 128, 126
@@ -225,10 +248,10 @@ def test_mlqg_single_layer_fluxes_on_the_reference_grid(gf):
     assert abs(gf.MultiLayerQG.fluxes(p) - 0.0313134) <= 1e-7
 
 
-def test_mlqg_set_psi_set_q_round_trip_32x34(gf):
-    # test/test_multilayerqg.jl:663-701 on its 32 x 34 grid: set_psi! gives psi back through
-    # updatevars!, set_q! gives q back
-    nx, ny = 32, 34
+@pytest.mark.parametrize("nx,ny", [(32, 34), (34, 32)])
+def test_mlqg_set_psi_set_q_round_trip(gf, nx, ny):
+    # test/test_multilayerqg.jl:663-701 on its 32 x 34 grid (and 34 x 32): set_psi! gives psi back
+    # through updatevars!, set_q! gives q back
     p = gf.MultiLayerQG.Problem(2, gf.GPU(), nx=nx, ny=ny, Lx=TWO_PI, Ly=TWO_PI, f0=1, H=[0.2, 0.8],
                                 b=[-1.0, -1.2], dt=0.01, stepper="RK4")
     x, y = p.grid.gridpoints()
