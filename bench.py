@@ -180,6 +180,15 @@ def ensemble_run(gf, steps, seq_steps):
     gf.stepforward(ens, steps)
     ens.synchronize()
     tb = time.perf_counter() - t0
+    # the same steps with each stage as one persistent cooperative launch (gf_batch_persistent)
+    ens.persistent(True)
+    gf.stepforward(ens, 3)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    gf.stepforward(ens, steps)
+    ens.synchronize()
+    tp = time.perf_counter() - t0
+    ens.persistent(False)
     lib.gf_timing(ens._ctx, 1)
     gf.stepforward(ens, steps)
     ens.synchronize()
@@ -207,6 +216,9 @@ def ensemble_run(gf, steps, seq_steps):
                         "FilteredRK4 dt=0.01, Gaussian-bump topography, q0 at k_peak=8 with energy 0.08, FP64",
             "members": sum(c for _, c in ENSEMBLE), "grid_points": pts,
             "batched": {"value": batched, "unit": "grid-pt-steps/s", "steps": steps, "ms_per_step": tb / steps * 1e3},
+            "batched_persistent": {"value": pts * steps / tp, "unit": "grid-pt-steps/s", "steps": steps,
+                                   "ms_per_step": tp / steps * 1e3,
+                                   "form": "each stage as one cooperative launch (gf_batch_persistent)"},
             "sequential": {"value": sequential, "unit": "grid-pt-steps/s", "steps": seq_steps,
                            "ms_per_step": ts / seq_steps * 1e3},
             "batched_over_sequential": batched / sequential, "kernels": kern, "state_finite": finite}

@@ -30,7 +30,8 @@ EXPORTS = ("gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_error_string"
            "gf_device_alloc", "gf_device_free", "gf_device_copy", "gf_use_graphs",
            "gf_set_schedule", "gf_get_schedule", "gf_autotune", "gf_schedules_save", "gf_schedules_load",
            "gf_schedules_clear", "gf_schedules_error", "gf_set_splits", "gf_get_splits",
-           "gf_set_forcing_harmonic", "gf_set_psi", "gf_diagnostic_async_noalloc", "gf_adjoint_tape_budget")
+           "gf_set_forcing_harmonic", "gf_set_psi", "gf_diagnostic_async_noalloc", "gf_adjoint_tape_budget",
+           "gf_batch_persistent")
 
 
 class ProblemDesc(ctypes.Structure):
@@ -112,6 +113,7 @@ def _load():
     lib.gf_device_free.restype = None
     lib.gf_device_copy.argtypes = [vp, vp, ctypes.c_size_t]
     lib.gf_use_graphs.argtypes = [vp, ip]
+    lib.gf_batch_persistent.argtypes = [vp, ip]
     lib.gf_set_schedule.argtypes = [vp, ip, ip, ip]
     lib.gf_get_schedule.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
     lib.gf_autotune.argtypes = [vp, ip, dp]

@@ -10,8 +10,11 @@
 // when single members are far too small to.
 //
 // GF_BATCH_K (1, 2 or 3) selects the kernel class this translation unit
-// instantiates, so the three compile in parallel.
+// instantiates, so the three compile in parallel; GF_BATCH_K = 0 instantiates the persistent
+// stage kernel, which runs all three classes of a stage in one cooperative launch.
 #include "gf_kernels_impl.hpp"
+#include <hip/hip_cooperative_groups.h>
+#include <algorithm>
 
 namespace gf {
 
@@ -60,12 +63,10 @@ __device__ __forceinline__ void batch_body(const BatchItem<T>& it, int blk, cplx
   }
 }
 
+// kernel class K of work entry w, at the entry's instance size
 template <typename T, int K, bool PLAIN>
-__global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
-void batch_kernel(const BatchItem<T>* __restrict__ items, const BatchWork* __restrict__ work) {
-  __shared__ cplx<T> lds[BatchLds<T>::value];
-  __shared__ cplx<T> tw[BATCH_TW];
-  const BatchWork w = work[blockIdx.x];
+__device__ __forceinline__ void batch_entry(const BatchItem<T>* __restrict__ items, const BatchWork w,
+                                            cplx<T>* lds, cplx<T>* tw) {
   const BatchItem<T>& it = items[w.item];
   switch (it.n) {
     case 32: batch_body<T, 32, K, PLAIN>(it, w.blk, lds, tw); break;
@@ -77,10 +78,19 @@ void batch_kernel(const BatchItem<T>* __restrict__ items, const BatchWork* __res
   }
 }
 
+template <typename T, int K, bool PLAIN>
+__global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+void batch_kernel(const BatchItem<T>* __restrict__ items, const BatchWork* __restrict__ work) {
+  __shared__ cplx<T> lds[BatchLds<T>::value];
+  __shared__ cplx<T> tw[BATCH_TW];
+  batch_entry<T, K, PLAIN>(items, work[blockIdx.x], lds, tw);
+}
+
 #ifndef GF_BATCH_K
 #define GF_BATCH_K 1
 #endif
 
+#if GF_BATCH_K >= 1
 template <typename T, int K>
 void launch_batch(const BatchItem<T>* items, const BatchWork* work, int nwork, int plain, hipStream_t s) {
   if (nwork <= 0) return;
@@ -89,6 +99,66 @@ void launch_batch(const BatchItem<T>* items, const BatchWork* work, int nwork, i
 }
 template void launch_batch<double, GF_BATCH_K>(const BatchItem<double>*, const BatchWork*, int, int, hipStream_t);
 template void launch_batch<float, GF_BATCH_K>(const BatchItem<float>*, const BatchWork*, int, int, hipStream_t);
+#else
+// One stage of every member in one persistent launch: the workgroups the GPU holds at once walk
+// the K1 work entries, then (after a grid-wide barrier, which also makes K1's stores visible)
+// the K2 entries, then the K3 entries. Every workgroup reaches both barriers: the loops hold no
+// early exit and the grid never exceeds the co-resident workgroups (cooperative launch).
+// Each class's walk is a function of its own (not inlined), so that its registers are allocated
+// for that class alone; the LDS is the launch's dynamic allocation, which each walk addresses
+// directly (so its accesses stay LDS instructions).
+template <typename T> struct BatchStageLds {
+  static constexpr size_t bytes = sizeof(cplx<T>) * (BatchLds<T>::value + BATCH_TW);
+};
+template <typename T, int K, bool PLAIN>
+__device__ __attribute__((noinline)) void batch_walk(const BatchItem<T>* __restrict__ items, const BatchWork* __restrict__ w, int n) {
+  extern __shared__ __align__(16) char batch_dyn_lds[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(batch_dyn_lds);
+  cplx<T>* tw = lds + BatchLds<T>::value;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    __syncthreads();   // the previous entry's LDS reads are done before this entry's twiddles land
+    batch_entry<T, K, PLAIN>(items, w[i], lds, tw);
+  }
+}
+// (3 waves per SIMD where the x pass is the plain one, as the separate kernels of each class run;
+// with the background-gradient x pass beside them, 2)
+template <typename T, bool PLAIN>
+__global__ __launch_bounds__(BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(PLAIN ? 3 : 2)))
+void batch_stage_kernel(const BatchItem<T>* __restrict__ items, BatchStageWork sw) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  batch_walk<T, 1, PLAIN>(items, sw.w[0], sw.n[0]);
+  grid.sync();
+  batch_walk<T, 2, PLAIN>(items, sw.w[1], sw.n[1]);
+  grid.sync();
+  batch_walk<T, 3, PLAIN>(items, sw.w[2], sw.n[2]);
+}
+template <typename T, bool PLAIN>
+int batch_stage_grid() {
+  static int g = 0;   // co-resident workgroups of the kernel on device 0's kind of GPU
+  if (g == 0) {
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_stage_kernel<T, PLAIN>, BATCH_THREADS, BatchStageLds<T>::bytes) != hipSuccess)
+      return -1;
+    g = ncu * per;
+  }
+  return g;
+}
+template <typename T>
+hipError_t launch_batch_stage(const BatchItem<T>* items, const BatchStageWork& sw, int plain, hipStream_t s) {
+  const int g = plain ? batch_stage_grid<T, true>() : batch_stage_grid<T, false>();
+  if (g <= 0) return hipErrorInvalidConfiguration;
+  const int most = std::max(sw.n[0], std::max(sw.n[1], sw.n[2]));
+  const dim3 grid(std::max(1, std::min(g, most))), blk(BATCH_THREADS);
+  BatchStageWork a = sw;
+  void* args[] = {(void*)&items, (void*)&a};
+  const size_t sh = BatchStageLds<T>::bytes;
+  return plain ? hipLaunchCooperativeKernel((const void*)batch_stage_kernel<T, true>, grid, blk, args, sh, s)
+               : hipLaunchCooperativeKernel((const void*)batch_stage_kernel<T, false>, grid, blk, args, sh, s);
+}
+template hipError_t launch_batch_stage<double>(const BatchItem<double>*, const BatchStageWork&, int, hipStream_t);
+template hipError_t launch_batch_stage<float>(const BatchItem<float>*, const BatchStageWork&, int, hipStream_t);
+#endif
 
 #if GF_BATCH_K == 1
 bool batch_size_supported(int n) { return n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }

@@ -298,6 +298,9 @@ struct CtxBase {
   virtual int64_t snapshot(void*) { throw GfError(GF_EINVAL, "snapshots are taken per problem; address one member through gf_ctx_instance"); }
   virtual void snapshot_wait(int64_t) {}
   virtual void set_graphs(int) {}
+  virtual void set_persistent(int) {
+    throw GfError(GF_EINVAL, "persistent stage launches are for ensembles (gf_ctx_create_batch)");
+  }
   virtual void set_schedule(const int*) { throw GfError(GF_EUNSUPPORTED, "radix schedules are set per problem"); }
   virtual void get_schedule(int*) const { throw GfError(GF_EUNSUPPORTED, "radix schedules are set per problem"); }
   virtual void set_splits(const int*) { throw GfError(GF_EUNSUPPORTED, "workgroup splits are set per problem"); }
@@ -1777,6 +1780,12 @@ struct BatchCtx final : CtxBase {
     tm.mark(stream);
     for (int k = 0; k < nst; ++k) {
       const BatchItem<T>* it = items[k][full].as<BatchItem<T>>();
+      if (persistent) {
+        const BatchStageWork sw{{work[0][full].as<BatchWork>(), work[1][full].as<BatchWork>(), work[2][full].as<BatchWork>()},
+                                {nwork[0][full], nwork[1][full], nwork[2][full]}};
+        HIPCHECK(launch_batch_stage<T>(it, sw, k2_plain, stream));
+        continue;
+      }
       tm.rec(0, true, stream);
       launch_batch<T, 1>(it, work[0][full].as<BatchWork>(), nwork[0][full], 0, stream);
       tm.rec(0, false, stream);
@@ -1794,6 +1803,10 @@ struct BatchCtx final : CtxBase {
   bool graphs = true;
   hipGraphExec_t gexec = nullptr;
   void set_graphs(int enable) override { graphs = enable != 0; }
+  // each stage as one persistent cooperative launch (launch_batch_stage) instead of three
+  // launches; such steps run without graph replay
+  bool persistent = false;
+  void set_persistent(int enable) override { persistent = enable != 0; }
   void build_graph() {
     gexec = dw::capture(stream, [&] { step_launches(0); });
   }
@@ -1825,7 +1838,7 @@ struct BatchCtx final : CtxBase {
           m->prevsol_valid = true;
         }
       }
-      if (!last && v == 0 && graphs && !tm.on) {
+      if (!last && v == 0 && graphs && !tm.on && !persistent) {
         if (!gexec) build_graph();
         HIPCHECK(hipGraphLaunch(gexec, stream));
       } else {
@@ -2184,6 +2197,9 @@ void gf_schedules_clear(void) { autoload_schedules(); gf::sched_clear(); }
 const char* gf_schedules_error(void) { return g_sched_err.c_str(); }
 int gf_use_graphs(gf_ctx* c, int enable) {
   return guarded(c, [&] { c->impl->set_graphs(enable); });
+}
+int gf_batch_persistent(gf_ctx* c, int enable) {
+  return guarded(c, [&] { c->impl->set_persistent(enable); });
 }
 int gf_snapshot_wait(gf_ctx* c, int64_t ticket) {
   return guarded(c, [&] { c->impl->snapshot_wait(ticket); });

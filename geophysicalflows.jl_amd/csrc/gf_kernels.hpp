@@ -347,6 +347,14 @@ struct BatchWork {
 // plain (K = 2): every member's x pass is K2_ADVECT, which selects the kernel without the
 // background-gradient body
 template <typename T, int K> void launch_batch(const BatchItem<T>* items, const BatchWork* work, int nwork, int plain, hipStream_t s);
+// The three work tables of one stage (K1, K2, K3 entries) for the persistent stage launch.
+struct BatchStageWork {
+  const BatchWork* w[3];
+  int n[3];
+};
+// All three kernel classes of one stage for every instance in one cooperative launch of the
+// workgroups the GPU holds at once (grid-wide barriers between the classes).
+template <typename T> hipError_t launch_batch_stage(const BatchItem<T>* items, const BatchStageWork& sw, int plain, hipStream_t s);
 // workgroups of kernel class k for an instance of size n: count = columns (k = 1, 3) or ny (k = 2)
 int batch_blocks(int k, int n, int count);
 bool batch_size_supported(int n);

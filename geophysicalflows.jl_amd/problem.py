@@ -479,6 +479,11 @@ class Ensemble:
         launch every kernel (the default for single problems)."""
         check(lib.gf_use_graphs(self._ctx, int(bool(enable))), self._ctx)
 
+    def persistent(self, enable=True):
+        """Run each stage's three kernel classes over every member as one persistent cooperative
+        launch (``gf_batch_persistent``) instead of three launches; results are identical."""
+        check(lib.gf_batch_persistent(self._ctx, int(bool(enable))), self._ctx)
+
     def _diagnostic(self, kind, xi=0.0, nxi=0, nout=1):
         out = (ctypes.c_double * nout)()
         n = lib.gf_diagnostic(self._ctx, int(kind), float(xi), int(nxi), out, nout)

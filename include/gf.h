@@ -150,6 +150,15 @@ int gf_step(gf_ctx* ctx, int nsteps);
  * for ensembles, off by default for single problems. No counterpart in the reference. */
 int gf_use_graphs(gf_ctx* ctx, int enable);
 
+/* Ensembles: run the three kernel classes of each stage (K1, K2, K3 over every member) as one
+ * persistent cooperative launch of the workgroups the GPU holds at once, with grid-wide
+ * barriers between the classes (1), or as three launches (0, the default). Results agree to
+ * rounding; the persistent form runs without graph replay and is slower on MI355X (a
+ * cooperative launch and its grid barriers cost more than the launches they replace).
+ * GF_EINVAL for a single problem. No counterpart in the reference, whose ensemble members are
+ * separate problems. */
+int gf_batch_persistent(gf_ctx* ctx, int enable);
+
 /* Radix schedules (no counterpart in the reference, whose FFTs are FFTW's or cuFFT's plans).
  * For single-layer problems the FFT of each calcN kernel runs with P points per thread:
  *   K1 (inverse y-pass), K2 (x-pass with the products), K3 (forward y-pass and stage).

@@ -293,3 +293,44 @@ def test_ensemble_single_layer_mlqg_members(gf):
     gf.stepforward(ens, 3)
     off.stepforward(r, 3)
     assert rel(ens[0].sol.reshape(r.sol.shape), r.sol) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_persistent_stage_launches_match_three_launches(gf, dtype):
+    # gf_batch_persistent: each stage's K1, K2 and K3 over every member in one cooperative
+    # launch with grid-wide barriers; the same bodies as the three launches, so the members agree
+    # to rounding (1e-14 FP64, 1e-6 FP32 relative L2) after each call
+    rng = np.random.default_rng(36)
+    sizes = [64, 512, 32, 128, 256, 64]
+    members = [(gf.SingleLayerQG.Problem, _slqg_kw(gf, n, rng, T=dtype)) for n in sizes[:3]]
+    members.append((gf.TwoDNavierStokes.Problem, dict(nx=128, ν=1e-4, nν=2, μ=0.1, dt=0.005, stepper="FilteredRK4",
+                                                       T=dtype, calcF=gf.RingForcing(6.0, 1.5, 0.1, seed=5, instance=3))))
+    members.append((gf.SingleLayerQG.Problem, dict(nx=256, β=1.0, U=0.3 * np.tanh(np.linspace(-3, 3, 256)), dt=0.005,
+                                                   stepper="FilteredRK4", T=dtype)))
+    members.append((gf.TwoDNavierStokes.Problem, dict(nx=64, ν=1e-4, nν=2, μ=0.1, dt=0.005, stepper="FilteredRK4",
+                                                       T=dtype, calcF=gf.RingForcing(5.0, 1.5, 0.1, seed=6, instance=5))))
+    a = gf.Ensemble(members)
+    b = gf.Ensemble(members)
+    b.persistent(True)
+    for i, n in enumerate(sizes):
+        f = _q0(gf, n, rng)
+        mod = gf.TwoDNavierStokes if members[i][0] is gf.TwoDNavierStokes.Problem else gf.SingleLayerQG
+        for e in (a, b):
+            (mod.set_ζ if mod is gf.TwoDNavierStokes else mod.set_q)(e[i], f)
+    tol = 1e-14 if dtype == np.float64 else 1e-6
+    for nsteps in (7, 3):
+        gf.stepforward(a, nsteps)
+        gf.stepforward(b, nsteps)
+        for i in range(len(sizes)):
+            x, y = a[i].sol, b[i].sol
+            assert np.isfinite(x).all() and np.isfinite(y).all(), i
+            e = rel(y, x)
+            print(f"member {i} n={sizes[i]} persistent vs three launches {e:.2e} bitwise {np.array_equal(x, y)}")
+            assert e < tol, (i, e)
+    b.persistent(False)           # and back
+    gf.stepforward(a, 2)
+    gf.stepforward(b, 2)
+    assert all(rel(b[i].sol, a[i].sol) < tol for i in range(len(sizes)))
+    p = gf.TwoDNavierStokes.Problem(gf.GPU(), nx=64, dt=0.01, stepper="RK4")
+    with pytest.raises(gf._lib.GFError, match="for ensembles"):
+        gf._lib.check(gf._lib.lib.gf_batch_persistent(p._ctx, 1), p._ctx)
