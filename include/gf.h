@@ -162,9 +162,10 @@ int gf_batch_persistent(gf_ctx* ctx, int enable);
 /* Radix schedules (no counterpart in the reference, whose FFTs are FFTW's or cuFFT's plans).
  * For single-layer problems the FFT of each calcN kernel runs with P points per thread:
  *   K1 (inverse y-pass), K2 (x-pass with the products), K3 (forward y-pass and stage).
- * P is 4, 8 (from 32 points) or 16 (from 1024 points); 0 restores the built-in choice (4
- * below 32 points, else 8). Results differ from another schedule's only by rounding.
- * gf_get_schedule reports the P in use. MultiLayerQG has a fixed schedule. */
+ * P is 4 (up to 2048 points), 8 (from 32 points) or 16 (from 1024 points); 0 restores the
+ * built-in choice (4 below 32 points, 16 from 4096, else 8). Results differ from another
+ * schedule's only by rounding. gf_get_schedule reports the P in use. MultiLayerQG, and grids
+ * with a length that is not a power of two, have a fixed schedule. */
 int gf_set_schedule(gf_ctx* ctx, int p_k1, int p_k2, int p_k3);
 int gf_get_schedule(gf_ctx* ctx, int* p_k1, int* p_k2, int* p_k3);
 
