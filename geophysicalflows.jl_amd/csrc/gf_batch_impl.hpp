@@ -132,16 +132,17 @@ void batch_stage_kernel(const BatchItem<T>* __restrict__ items, BatchStageWork s
   grid.sync();
   batch_walk<T, 3, PLAIN>(items, sw.w[2], sw.n[2]);
 }
+// Co-resident workgroups of the stage kernel (CUs times workgroups per CU), computed once per
+// process on the current device (every device of a process is taken to be the same GPU).
 template <typename T, bool PLAIN>
 int batch_stage_grid() {
-  static int g = 0;   // co-resident workgroups of the kernel on device 0's kind of GPU
-  if (g == 0) {
+  static const int g = [] {
     int dev = 0, ncu = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_stage_kernel<T, PLAIN>, BATCH_THREADS, BatchStageLds<T>::bytes) != hipSuccess)
       return -1;
-    g = ncu * per;
-  }
+    return ncu * per;
+  }();
   return g;
 }
 template <typename T>
