@@ -29,7 +29,8 @@ def rand_spec(rng, shape):
 @pytest.mark.parametrize("nx,ny,dtype", [(64, 66, np.float64), (128, 126, np.float64), (32, 34, np.float64),
                                          (256, 250, np.float64), (64, 18, np.float64), (64, 66, np.float32),
                                          (66, 64, np.float64), (126, 126, np.float64), (254, 128, np.float64),
-                                         (18, 32, np.float64), (68, 66, np.float64), (66, 64, np.float32)])
+                                         (18, 32, np.float64), (68, 66, np.float64), (66, 64, np.float32),
+                                         (18, 18, np.float64), (250, 66, np.float64), (126, 126, np.float32)])
 def test_twodns_ring_forced_calcN_steps_updatevars(gf, nx, ny, dtype):
     Ly = TWO_PI * ny / nx
     kw = dict(nx=nx, ny=ny, Lx=TWO_PI, Ly=Ly, dt=0.002, stepper="ETDRK4")
