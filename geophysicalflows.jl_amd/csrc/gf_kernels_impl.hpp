@@ -712,8 +712,10 @@ __global__ __launch_bounds__(K2G_THREADS) void k2_gen_kernel(const K2Args<T> a) 
       for (int kr = t; kr < a.kout; kr += K2G_THREADS) at(o, boff32(kr, y, ny)) = r2c_point(S.ph[0], nx, kr, S.roots);
     }
   } else {
-    // K2_ADVECT: u (q [+ eta]), v (q [+ eta]); K2_ADVECT_FULL, per layer: u += U,
-    // (-i kr (u q)^ - (u Qx + v Qy)^, (v q)^), or with `linear` U q in place of u q and no v q
+    // K2_ADVECT: u (q [+ eta]), v (q [+ eta]) (src/twodnavierstokes.jl:262-270,
+    // src/singlelayerqg.jl:368-375); K2_ADVECT_FULL, per layer: u += U,
+    // (-i kr (u q)^ - (u Qx + v Qy)^, (v q)^) (src/multilayerqg.jl:695-731), or with `linear`
+    // U q in place of u q and no v q (src/multilayerqg.jl:744-779)
     constexpr bool FULL = MODE == K2_ADVECT_FULL;
     const int nl = FULL ? g.nlayers : 1;
     for (int jl = 0; jl < nl; ++jl) {
