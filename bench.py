@@ -73,6 +73,9 @@ def measured_traffic(sched, splits):
 KERNEL_REPS = 50
 KERNEL_WARM = 200
 
+# steps of the sequential comparison of the ensemble (every member stepped alone, one after another)
+ENSEMBLE_SEQ_STEPS = 10
+
 # Steps before the untimed warm-up steps, so that the timed steps do not start on a GPU whose
 # clocks are still ramping up from idle (about 0.1 s of work at 2048^2)
 CLOCK_RAMP_STEPS = 300
@@ -481,7 +484,8 @@ def main():
         return
     if args.only_ensemble:
         import __graft_entry__ as ge
-        print(json.dumps({"ensemble": ensemble_run(ge.load_package(), max(1, args.ensemble_steps), 3)}), flush=True)
+        print(json.dumps({"ensemble": ensemble_run(ge.load_package(), max(1, args.ensemble_steps), ENSEMBLE_SEQ_STEPS)}),
+              flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -540,7 +544,7 @@ def main():
         es, _, fs = gpu_run(gf, steps, warmup, np.float64, dt=0.001)
         out["stable_dt_0.001"] = {"value": NX * NX * steps / es, "ms_per_step": es / steps * 1e3, "state_finite": fs}
     if not args.no_ensemble:
-        out["ensemble"] = ensemble_run(gf, max(1, args.ensemble_steps), 3)
+        out["ensemble"] = ensemble_run(gf, max(1, args.ensemble_steps), ENSEMBLE_SEQ_STEPS)
         out["mlqg_512x3"] = mlqg_run(gf, 200)
         out["decaying_128"] = decaying_run(gf)
         out["adjoint_512"] = adjoint_run(gf)
